@@ -1,0 +1,245 @@
+/*
+ * hussar_hip.h — C ABI of the MI355X (gfx950) backend for libhussar's Monte Carlo
+ * FMCW radar integrator.
+ *
+ * The reference selects a backend at compile time through a two-method "Backend
+ * concept": a constructor `Backend(const TriangleMesh &, Integrator &)` and
+ * `void run(const Scene &, long budget, bool *interruptFlag)`
+ * (libhussar/include/hussar/arch/cpu.h:17-46, libhussar/include/hussar/arch/gpu.h:17-32).
+ * `PathTracer::run` (libhussar/include/hussar/integrators/path.h:171-224) drives it.
+ * The entry points below are what a binding for that concept needs once the per-path
+ * work (`PathTracer::sample`, path.h:235-451) runs as HIP kernels instead of as a C++
+ * template executed by the backend: every input crosses as plain-old-data, every
+ * buffer as a pointer plus a count, and every failure as a status code with a message
+ * retrievable through hsr_last_error() (exceptions cannot cross a C ABI).
+ *
+ * Threading: a backend is not re-entrant (as in the reference, gpu.cpp:35-37); calls on
+ * one backend must be serialised by the caller. Different backends are independent.
+ */
+#ifndef HUSSAR_HIP_H
+#define HUSSAR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HSR_ABI_VERSION 1
+#define HSR_MAX_CHANNELS 16
+
+typedef enum hsr_status {
+    HSR_OK = 0,
+    HSR_ERR_INVALID = 1,     /* bad argument (null pointer, size out of range, ...) */
+    HSR_ERR_HIP = 2,         /* a HIP runtime call failed */
+    HSR_ERR_OOM = 3,         /* device memory exhausted / admission refused */
+    HSR_ERR_INTERRUPTED = 4, /* the caller's interrupt flag was raised */
+    HSR_ERR_UNSUPPORTED = 5, /* knob combination the backend does not implement */
+    HSR_ERR_BUSY = 6         /* serving queue full */
+} hsr_status;
+
+/* radar::RFConfig (libradar/include/radar/radar.h:113-136). SI units, float like the reference. */
+typedef struct hsr_rf_config {
+    float startFreq;    /* [Hz] */
+    float freqSlope;    /* [Hz/s] */
+    float adcRate;      /* [Hz] */
+    float idleTime;     /* [s] */
+    float rampTime;     /* [s] */
+    float antennaDelay; /* [s] */
+} hsr_rf_config;
+
+/* radar::FrameConfig (radar.h:141-164). The frame is laid out [chirp][sample][channel]
+ * row-major (Frame::makeIndex, radar.h:487-494), complex interleaved (re, im). */
+typedef struct hsr_frame_config {
+    int32_t chirpCount;
+    int32_t samplesPerChirp;
+    int32_t channelCount;
+} hsr_frame_config;
+
+/* hussar::NFAntenna with its AWRAngularDistribution (libhussar/include/hussar/core/emitter.h:159-214).
+ * rotation is the 3x3 matrix, row-major: rotation[3*row + col]. Its columns are the
+ * antenna's local axes in world space; local +z is the boresight. */
+typedef struct hsr_antenna {
+    float position[3];
+    float rotation[9];
+} hsr_antenna;
+
+/* hussar::Scene (libhussar/include/hussar/core/scene.h:10-16): one RF config, one Tx and
+ * rxCount receivers. The reference has exactly one receiver and fills channel 0 only
+ * (path.h:291-292); rx[1..rxCount-1] are the multi-receiver extension and feed frame
+ * channels 1..rxCount-1. rxCount must not exceed the frame's channelCount. */
+typedef struct hsr_scene {
+    hsr_rf_config rfConfig;
+    hsr_antenna tx;
+    hsr_antenna rx[HSR_MAX_CHANNELS];
+    int32_t rxCount;
+    int32_t reserved0;
+} hsr_scene;
+
+/* PathTracer public knobs (path.h:153-169), same names and defaults (see
+ * hsr_knobs_default). correctPhase=1 or useGeometricalOptics=0 return
+ * HSR_ERR_UNSUPPORTED. primaryOrder fixes the unspecified evaluation order of
+ * `Vector2f(get1D(), get1D())` (libhussar/include/hussar/samplers/halton.h:24-27):
+ * 0 = right-to-left as g++ evaluates it (primary = (RI_5, RI_3)), 1 = left-to-right as
+ * clang does (primary = (RI_3, RI_5)). */
+typedef struct hsr_knobs {
+    int32_t onlyIndirect;
+    int32_t maxDepth;
+    int32_t useGeometricalOptics;
+    int32_t poDiffraction;
+    int32_t doGuiding;
+    int32_t clearBeforeIteration;
+    int32_t correctPhase;
+    int32_t filterGuiding;
+    int32_t doFiltering;
+    int32_t filteringSphere;
+    float filteringMin;
+    float filteringMax;
+    float filteringRadius;
+    int32_t primaryOrder;
+} hsr_knobs;
+
+/* Fills the reference defaults of path.h:153-169 (maxDepth 10, guiding on, ...). */
+void hsr_knobs_default(hsr_knobs *knobs);
+
+typedef struct hsr_backend hsr_backend;
+typedef struct hsr_guide hsr_guide;
+
+/* Replaces the backend constructor Backend(const TriangleMesh &, Integrator &)
+ * (cpu.h:18-20 -> cpu.cpp:54-69 Embree scene build; gpu.h:18-20 -> gpu.cpp:482-493 OptiX
+ * GAS build). vertices: float[3*vertexCount]; indices: int32[3*triangleCount] (the
+ * TriangleMesh vertexBuffer / indexBuffer, mesh.h:16-28). The mesh is copied, a BVH is
+ * built on the host and uploaded; the caller may free its buffers afterwards.
+ * device: HIP device ordinal. */
+hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount,
+                          const int32_t *indices, int64_t triangleCount,
+                          int32_t device, hsr_backend **out);
+void hsr_hip_destroy(hsr_backend *backend);
+
+/* Integrator::configureFrame (libhussar/include/hussar/core/integrator.h:60-62). Allocates
+ * the device-resident accumulation frame and clears it. */
+hsr_status hsr_hip_configure_frame(hsr_backend *backend, const hsr_frame_config *config);
+
+/* PathTracer::clearFrame (path.h:147-150): zero the frame and totalWeight. */
+hsr_status hsr_hip_clear_frame(hsr_backend *backend);
+
+/* Backend::run (cpu.h:44-46, gpu.h:26-32) with the PathTracer state the reference
+ * backends read from the integrator object (path.h:138-141,458-464): the per-path
+ * work of PathTracer::sample (path.h:236-451) for Halton indices
+ * sampleIndexOffset .. sampleIndexOffset+budget-1, accumulated into the frame.
+ * Blocking, like the reference. guide may be NULL when knobs->doGuiding is 0.
+ * interruptFlag (may be NULL) is polled between device launches; when it reads non-zero
+ * the call returns HSR_ERR_INTERRUPTED, with a partial frame (cpu.h:25 semantics). */
+hsr_status hsr_hip_run(hsr_backend *backend, const hsr_scene *scene, const hsr_knobs *knobs,
+                       int64_t sampleIndexOffset, int64_t budget, float sampleWeight,
+                       int32_t isFinalIteration, hsr_guide *guide,
+                       const volatile int32_t *interruptFlag);
+
+/* PathTracer::fetchFrame (path.h:453-455): frame / totalWeight.
+ * out_c64: float[2*sampleCount] (complex64, the reference's output type) or NULL.
+ * out_c128: double[2*sampleCount], same values before rounding to float, or NULL.
+ * totalWeight: receives the sum of sample weights, or NULL. */
+hsr_status hsr_hip_fetch_frame(hsr_backend *backend, float *out_c64, double *out_c128,
+                               double *totalWeight);
+
+/* PathTracer::run (path.h:171-224) including the doubling guiding schedule
+ * (milestones 16384 * 2^k, clearBeforeIteration, final-iteration rule). The frame must
+ * have been configured. guide may be NULL when knobs->doGuiding is 0. */
+hsr_status hsr_path_tracer_run(hsr_backend *backend, const hsr_scene *scene,
+                               const hsr_knobs *knobs, int64_t samples, hsr_guide *guide,
+                               const volatile int32_t *interruptFlag);
+
+/* An explicit training schedule (an extension): trainingPasses passes of trainingPaths
+ * paths each (each pass restarts at Halton index 0, records into the guide and is
+ * followed by a guide step), then one final pass of finalPaths paths into a cleared
+ * frame, which is what hsr_hip_fetch_frame returns afterwards. */
+hsr_status hsr_path_tracer_run_schedule(hsr_backend *backend, const hsr_scene *scene,
+                                        const hsr_knobs *knobs, int32_t trainingPasses,
+                                        int64_t trainingPaths, int64_t finalPaths,
+                                        hsr_guide *guide,
+                                        const volatile int32_t *interruptFlag);
+
+/* GuidingWrapper<BTree<2,...>> (path.h:17-131) with the settings PathTracer::setup
+ * assigns (path.h:229-232: uniformProb 0.1, splitThreshold 0.005, secondMoment true).
+ * The guide lives on the backend's device. */
+hsr_status hsr_guide_create(hsr_backend *backend, float uniformProb, float splitThreshold,
+                            int32_t secondMoment, hsr_guide **out);
+void hsr_guide_destroy(hsr_guide *guide);
+/* GuidingWrapper::reset (path.h:48-51): both trees back to a single uniform leaf. */
+hsr_status hsr_guide_reset(hsr_guide *guide);
+/* GuidingWrapper::step (path.h:119-126): build the training tree from its statistics,
+ * copy it to the sampling tree, refine the training tree and clear its statistics. */
+hsr_status hsr_guide_step(hsr_guide *guide);
+
+/* Inspection of a guide (no reference counterpart; for tests and persistence).
+ * Sampling tree: nodeCount nodes; child[i] is 0 for a leaf or the index of the first of
+ * two consecutive children; p0[i] is the probability of child 0 (inner nodes).
+ * Training tree: child[] plus one double statistic per node (leaves carry data). */
+hsr_status hsr_guide_sampling_size(hsr_guide *guide, int32_t *nodeCount);
+hsr_status hsr_guide_get_sampling(hsr_guide *guide, int32_t *child, float *p0, int32_t capacity);
+hsr_status hsr_guide_set_sampling(hsr_guide *guide, const int32_t *child, const float *p0,
+                                  int32_t nodeCount);
+hsr_status hsr_guide_training_size(hsr_guide *guide, int32_t *nodeCount);
+hsr_status hsr_guide_get_training(hsr_guide *guide, int32_t *child, double *stat,
+                                  int32_t capacity);
+
+/* Per-run statistics, measured with HIP events on the stream the kernels run on. */
+typedef struct hsr_run_stats {
+    double trace_ms;        /* device time of the path-tracing launches of the last run */
+    double reduce_ms;       /* device time of the frame reduction */
+    double total_ms;        /* device time of the whole last run (first to last launch) */
+    int64_t paths;          /* paths traced by the last run */
+    int64_t launches;       /* kernel launches of the last run */
+    int64_t nodes_visited;  /* BVH nodes fetched (only when counters are enabled) */
+    int64_t tris_tested;    /* triangles tested (only when counters are enabled) */
+    int64_t rays_closest;   /* closest-hit rays traced (counters enabled) */
+    int64_t rays_shadow;    /* shadow rays traced (counters enabled) */
+    int64_t events;         /* frame splat events (counters enabled) */
+    double bvh_build_ms;    /* host BVH build time at create */
+} hsr_run_stats;
+
+hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
+
+/* Options (no reference counterpart):
+ *  HSR_OPT_COUNTERS      1 = count nodes / triangles / rays / events in the next runs
+ *                        (a separate kernel instance; deterministic counts).
+ *  HSR_OPT_RECORD_HITS   >0 = record, for the first N paths of each run, the triangle
+ *                        id hit at every depth and the shadow-ray outcome of every NEE;
+ *                        read back with hsr_hip_get_hit_record.
+ *  HSR_OPT_PATHS_PER_LAUNCH  maximum paths per kernel launch (interrupt granularity).
+ *  HSR_OPT_PIPELINE      0 = fused path kernel (default), 1 = wavefront stages.
+ *  HSR_OPT_PHASE_MODE    0 = FP64 phase (default), 1 = FP32 phase as in the reference,
+ *                        2 = double-float (two-float) phase. */
+#define HSR_OPT_COUNTERS 1
+#define HSR_OPT_RECORD_HITS 2
+#define HSR_OPT_PATHS_PER_LAUNCH 3
+#define HSR_OPT_PIPELINE 4
+#define HSR_OPT_PHASE_MODE 5
+hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
+
+/* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]
+ * (-1 = no hit), nee int8[paths*maxDepth*rxCount] (0 = not attempted, 1 = occluded,
+ * 2 = visible, 3 = visible and splatted). */
+hsr_status hsr_hip_get_hit_record(hsr_backend *backend, int32_t *hit_ids, int8_t *nee,
+                                  int64_t paths, int32_t maxDepth, int32_t rxCount);
+
+/* Device information for roofline reporting. */
+typedef struct hsr_device_info {
+    char name[64];
+    char arch[32];
+    int32_t compute_units;
+    int32_t xcds;
+    int64_t hbm_bytes;
+    int32_t lds_bytes_per_cu;
+    int32_t clock_mhz;
+} hsr_device_info;
+hsr_status hsr_hip_device_info(int32_t device, hsr_device_info *out);
+
+const char *hsr_last_error(void);
+int32_t hsr_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HUSSAR_HIP_H */

@@ -1,0 +1,252 @@
+"""ctypes bindings of the oracle (oracle/liboracle.so) — test infrastructure only.
+
+The oracle restates libhussar's integrator on the CPU; tests use it as the checker and the
+benchmark's CPU-baseline leg times it. Its structs have the layout of include/hussar_hip.h,
+so the ctypes structs of hussar_amd.types are passed to it unchanged.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from hussar_amd.types import c_frame_config, c_knobs, c_rf_config, c_scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+PREC_REF32, PREC_MIXED, PREC_FP64 = 0, 1, 2
+
+_lib = None
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = C.CDLL(ORACLE_SO)
+        P = C.POINTER
+        vp = C.c_void_p
+        L.orc_radical_inverse.restype = C.c_float
+        L.orc_radical_inverse.argtypes = [C.c_uint64, C.c_int]
+        L.orc_halton_batch.argtypes = [C.c_uint64, C.c_int64, C.c_int, P(C.c_float)]
+        L.orc_sincos_f32.argtypes = [P(C.c_float), C.c_int64, P(C.c_float), P(C.c_float)]
+        L.orc_sphere_sample.argtypes = [C.c_int, P(C.c_float), C.c_int64, P(C.c_double)]
+        L.orc_awr_evaluate.argtypes = [C.c_int, P(C.c_double), C.c_int64, P(C.c_double)]
+        L.orc_mesh_create.restype = vp
+        L.orc_mesh_create.argtypes = [P(C.c_float), C.c_int64, P(C.c_int32), C.c_int64]
+        L.orc_mesh_destroy.argtypes = [vp]
+        L.orc_mesh_set_brute.argtypes = [vp, C.c_int]
+        L.orc_mesh_node_count.restype = C.c_int64
+        L.orc_mesh_node_count.argtypes = [vp]
+        L.orc_intersect.argtypes = [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), C.c_int64, C.c_float,
+                                    C.c_float, P(C.c_int32), P(C.c_double), P(C.c_int64)]
+        L.orc_splat.argtypes = [C.c_int, P(c_rf_config), C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                C.c_double, C.c_double, P(C.c_double)]
+        L.orc_guide_create.restype = vp
+        L.orc_guide_create.argtypes = [C.c_float, C.c_float, C.c_int]
+        for f in ("orc_guide_destroy", "orc_guide_reset", "orc_guide_step"):
+            getattr(L, f).argtypes = [vp]
+        L.orc_guide_sample.argtypes = [vp, P(C.c_float), C.c_int64, P(C.c_float)]
+        L.orc_tree_sample.argtypes = [vp, P(C.c_float), C.c_int64, P(C.c_float)]
+        L.orc_tree_pdf.argtypes = [vp, P(C.c_float), C.c_int64, P(C.c_float)]
+        L.orc_guide_record.argtypes = [vp, P(C.c_float), P(C.c_float), P(C.c_float), C.c_int64]
+        L.orc_guide_size.restype = C.c_int32
+        L.orc_guide_size.argtypes = [vp, C.c_int]
+        L.orc_guide_get.argtypes = [vp, C.c_int, P(C.c_int32), P(C.c_float), P(C.c_double)]
+        L.orc_guide_set_sampling.argtypes = [vp, P(C.c_int32), P(C.c_float), C.c_int32]
+        L.orc_render.restype = C.c_int
+        L.orc_render.argtypes = [vp, P(c_scene), P(c_knobs), P(c_frame_config), C.c_int, C.c_int64, C.c_int64,
+                                 C.c_float, C.c_int, vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int32),
+                                 P(C.c_int8), C.c_int64, P(C.c_int64)]
+        L.orc_path_tracer_run.restype = C.c_int
+        L.orc_path_tracer_run.argtypes = [vp, P(c_scene), P(c_knobs), P(c_frame_config), C.c_int, C.c_int64, vp,
+                                          C.c_int, P(C.c_double), P(C.c_double)]
+        L.orc_path_tracer_run_schedule.restype = C.c_int
+        L.orc_path_tracer_run_schedule.argtypes = [vp, P(c_scene), P(c_knobs), P(c_frame_config), C.c_int, C.c_int,
+                                                   C.c_int64, C.c_int64, vp, C.c_int, P(C.c_double),
+                                                   P(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def halton(start: int, n: int, dim: int) -> np.ndarray:
+    out = np.empty(n, np.float32)
+    lib().orc_halton_batch(start, n, dim, _p(out, C.c_float))
+    return out
+
+
+def sincos_f32(x: np.ndarray):
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().orc_sincos_f32(_p(x, C.c_float), x.size, _p(s, C.c_float), _p(c, C.c_float))
+    return s, c
+
+
+def sphere_sample(u: np.ndarray, prec: int = 0) -> np.ndarray:
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    out = np.empty((u.shape[0], 3), np.float64)
+    lib().orc_sphere_sample(prec, _p(u, C.c_float), u.shape[0], _p(out, C.c_double))
+    return out
+
+
+def awr_evaluate(d: np.ndarray, prec: int = 2) -> np.ndarray:
+    d = np.ascontiguousarray(d, np.float64).reshape(-1, 3)
+    out = np.empty((d.shape[0], 6), np.float64)
+    lib().orc_awr_evaluate(prec, _p(d, C.c_double), d.shape[0], _p(out, C.c_double))
+    return out[:, 0::2] + 1j * out[:, 1::2]
+
+
+class OracleMesh:
+    def __init__(self, vertices: np.ndarray, indices: np.ndarray):
+        self.V = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.I = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+        self.h = lib().orc_mesh_create(_p(self.V, C.c_float), self.V.shape[0], _p(self.I, C.c_int32),
+                                       self.I.shape[0])
+
+    @classmethod
+    def from_mesh(cls, mesh):
+        return cls(mesh.vertexBuffer, mesh.indexBuffer)
+
+    def set_brute(self, brute: bool):
+        lib().orc_mesh_set_brute(self.h, int(brute))
+
+    def intersect(self, rays: np.ndarray, prec=0, brute=False, any_hit=False, tnear=5e-5, tfar=np.inf):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        ids = np.empty(rays.shape[0], np.int32)
+        t = np.empty(rays.shape[0], np.float64)
+        cnt = np.zeros(2, np.int64)
+        lib().orc_intersect(self.h, prec, int(brute), int(any_hit), _p(rays, C.c_float), rays.shape[0], tnear, tfar,
+                            _p(ids, C.c_int32), _p(t, C.c_double), _p(cnt, C.c_int64))
+        return ids, t, cnt
+
+    def __del__(self):
+        try:
+            lib().orc_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+
+class OracleGuide:
+    def __init__(self, uniformProb=0.1, splitThreshold=0.005, secondMoment=True):
+        self.h = lib().orc_guide_create(uniformProb, splitThreshold, int(secondMoment))
+
+    def reset(self):
+        lib().orc_guide_reset(self.h)
+
+    def step(self):
+        lib().orc_guide_step(self.h)
+
+    def sample(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 2).copy()
+        pdf = np.empty(x.shape[0], np.float32)
+        lib().orc_guide_sample(self.h, _p(x, C.c_float), x.shape[0], _p(pdf, C.c_float))
+        return x, pdf
+
+    def tree_sample(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 2).copy()
+        pdf = np.empty(x.shape[0], np.float32)
+        lib().orc_tree_sample(self.h, _p(x, C.c_float), x.shape[0], _p(pdf, C.c_float))
+        return x, pdf
+
+    def tree_pdf(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 2)
+        pdf = np.empty(x.shape[0], np.float32)
+        lib().orc_tree_pdf(self.h, _p(x, C.c_float), x.shape[0], _p(pdf, C.c_float))
+        return pdf
+
+    def record(self, x, density, weight):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 2)
+        d = np.ascontiguousarray(density, np.float32)
+        w = np.ascontiguousarray(weight, np.float32)
+        lib().orc_guide_record(self.h, _p(x, C.c_float), _p(d, C.c_float), _p(w, C.c_float), x.shape[0])
+
+    def get(self, which=0):
+        n = lib().orc_guide_size(self.h, which)
+        child = np.empty(n, np.int32)
+        p0 = np.empty(n, np.float32)
+        stat = np.empty(n, np.float64)
+        lib().orc_guide_get(self.h, which, _p(child, C.c_int32), _p(p0, C.c_float), _p(stat, C.c_double))
+        return child, p0, stat
+
+    def set_sampling(self, child, p0):
+        child = np.ascontiguousarray(child, np.int32)
+        p0 = np.ascontiguousarray(p0, np.float32)
+        lib().orc_guide_set_sampling(self.h, _p(child, C.c_int32), _p(p0, C.c_float), child.size)
+
+    def __del__(self):
+        try:
+            lib().orc_guide_destroy(self.h)
+        except Exception:
+            pass
+
+
+def frame_size(fc) -> int:
+    return max(1, fc.chirpCount) * fc.samplesPerChirp * fc.channelCount
+
+
+def render(mesh: OracleMesh, scene, knobs, fc, prec=PREC_FP64, offset=0, count=1 << 12, sample_weight=1.0,
+           is_final=False, guide: OracleGuide | None = None, threads=0, record_paths=0, frame=None):
+    """Backend::run on the oracle. Returns (frame complex128 [chirps, N, C], totalWeight, record, counters)."""
+    sc = scene.to_c() if hasattr(scene, "to_c") else scene
+    kn = knobs.to_c() if hasattr(knobs, "to_c") else knobs
+    fcc = fc.to_c() if hasattr(fc, "to_c") else fc
+    size = frame_size(fcc)
+    buf = np.zeros(2 * size, np.float64) if frame is None else frame
+    tw = C.c_double(0.0)
+    maxd = max(1, kn.maxDepth)
+    C_ = fcc.channelCount
+    hits = np.full((max(record_paths, 1), maxd), -1, np.int32)
+    nee = np.zeros((max(record_paths, 1), maxd, C_), np.int8)
+    cnt = np.zeros(5, np.int64)
+    rc = lib().orc_render(mesh.h, C.byref(sc), C.byref(kn), C.byref(fcc), prec, offset, count, sample_weight,
+                          int(is_final), guide.h if guide else None, threads, _p(buf, C.c_double), C.byref(tw),
+                          _p(hits, C.c_int32), _p(nee, C.c_int8), record_paths, _p(cnt, C.c_int64))
+    if rc:
+        raise RuntimeError(f"orc_render failed with {rc}")
+    z = (buf[0::2] + 1j * buf[1::2]).reshape(max(1, fcc.chirpCount), fcc.samplesPerChirp, fcc.channelCount)
+    rec = (hits[:record_paths], nee[:record_paths]) if record_paths else None
+    return z, tw.value, rec, cnt
+
+
+def path_tracer_run(mesh: OracleMesh, scene, knobs, fc, samples, prec=PREC_FP64, guide: OracleGuide | None = None,
+                    threads=0):
+    """PathTracer::run on the oracle. Returns (un-normalised frame, totalWeight)."""
+    sc, kn, fcc = scene.to_c(), knobs.to_c(), fc.to_c()
+    size = frame_size(fcc)
+    buf = np.zeros(2 * size, np.float64)
+    tw = C.c_double(0.0)
+    rc = lib().orc_path_tracer_run(mesh.h, C.byref(sc), C.byref(kn), C.byref(fcc), prec, samples,
+                                   guide.h if guide else None, threads, _p(buf, C.c_double), C.byref(tw))
+    if rc:
+        raise RuntimeError(f"orc_path_tracer_run failed with {rc}")
+    z = (buf[0::2] + 1j * buf[1::2]).reshape(max(1, fcc.chirpCount), fcc.samplesPerChirp, fcc.channelCount)
+    return z, tw.value
+
+
+def path_tracer_run_schedule(mesh: OracleMesh, scene, knobs, fc, passes, train, final, prec=PREC_FP64,
+                             guide: OracleGuide | None = None, threads=0):
+    sc, kn, fcc = scene.to_c(), knobs.to_c(), fc.to_c()
+    size = frame_size(fcc)
+    buf = np.zeros(2 * size, np.float64)
+    tw = C.c_double(0.0)
+    rc = lib().orc_path_tracer_run_schedule(mesh.h, C.byref(sc), C.byref(kn), C.byref(fcc), prec, passes, train,
+                                            final, guide.h if guide else None, threads, _p(buf, C.c_double),
+                                            C.byref(tw))
+    if rc:
+        raise RuntimeError(f"orc_path_tracer_run_schedule failed with {rc}")
+    z = (buf[0::2] + 1j * buf[1::2]).reshape(max(1, fcc.chirpCount), fcc.samplesPerChirp, fcc.channelCount)
+    return z, tw.value
