@@ -1,0 +1,656 @@
+// C ABI of the gfx950 backend (include/hussar_hip.h).
+//
+// hsr_hip_create     ~ Backend(const TriangleMesh &, Integrator &)  (cpu.cpp:54-69, gpu.cpp:482-493)
+// hsr_hip_run        ~ Backend::run(scene, budget, interruptFlag)   (cpu.h:44-46, gpu.cpp:499-532)
+// hsr_path_tracer_run ~ PathTracer::run                             (path.h:171-224)
+// hsr_hip_fetch_frame ~ PathTracer::fetchFrame                      (path.h:453-455)
+// hsr_guide_*        ~ GuidingWrapper reset/step                    (path.h:48-51,119-126)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hussar_hip.h"
+#include "bvh.h"
+#include "guide.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+hsr_status fail(hsr_status s, const std::string &msg) {
+    g_last_error = msg;
+    return s;
+}
+
+#define HSR_HIP(call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(e_ == hipErrorOutOfMemory ? HSR_ERR_OOM : HSR_ERR_HIP,                    \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                       \
+    } while (0)
+
+template <class T> hsr_status dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(HSR_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return HSR_OK;
+}
+
+template <class T> void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct hsr_guide {
+    hsr_backend *backend = nullptr;
+    hsr::GuideHost host;
+    int32_t *d_sChild = nullptr;
+    float *d_sP0 = nullptr;
+    int32_t *d_tChild = nullptr;
+    double *d_tStat = nullptr;
+    int32_t capacity = 1 << 16;
+    bool dirty = true;  // host trees newer than the device copies
+};
+
+struct hsr_backend {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu = 0;
+    // mesh
+    float4 *d_nodes = nullptr;
+    float4 *d_tris = nullptr;
+    int32_t nodeCount = 0, slotCount = 0, bvhDepth = 0;
+    int64_t triangles = 0;
+    double bvh_ms = 0.0;
+    // frame
+    bool configured = false;
+    hsr_frame_config fc{1, 0, 0};
+    int64_t frameDoubles = 0;  // 2 * chirps * N * C
+    double *d_frame_sum = nullptr, *d_frame_comp = nullptr;
+    double totalWeight = 0.0;
+    // scratch
+    double *d_partials = nullptr;
+    size_t partialCap = 0;  // doubles
+    double *d_stage_sum = nullptr, *d_stage_comp = nullptr;
+    size_t stageCap = 0;
+    unsigned long long *d_counters = nullptr;
+    int32_t *d_recHits = nullptr;
+    int8_t *d_recNee = nullptr;
+    int64_t recCap = 0, recPathsLast = 0;
+    int32_t recMaxDepth = 0, recC = 0;
+    // options
+    bool optCounters = false;
+    int64_t optRecord = 0;
+    int64_t optPathsPerLaunch = int64_t(1) << 24;
+    int32_t optPhaseMode = 0;
+    int32_t maxGrid = 0;
+    // timing
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hsr_run_stats stats{};
+};
+
+// --------------------------------------------------------------------------------------
+extern "C" {
+
+const char *hsr_last_error(void) { return g_last_error.c_str(); }
+int32_t hsr_abi_version(void) { return HSR_ABI_VERSION; }
+
+void hsr_knobs_default(hsr_knobs *k) {
+    if (!k) return;
+    // path.h:153-169
+    k->onlyIndirect = 1;
+    k->maxDepth = 10;
+    k->useGeometricalOptics = 1;
+    k->poDiffraction = 0;
+    k->doGuiding = 1;
+    k->clearBeforeIteration = 1;
+    k->correctPhase = 0;
+    k->filterGuiding = 1;
+    k->doFiltering = 1;
+    k->filteringSphere = 1;
+    k->filteringMin = 600.0f;
+    k->filteringMax = 900.0f;
+    k->filteringRadius = 160.0f;
+    k->primaryOrder = 0;
+    k->produceDebugImage = 0;
+}
+
+hsr_status hsr_hip_device_info(int32_t device, hsr_device_info *out) {
+    if (!out) return fail(HSR_ERR_INVALID, "null output");
+    hipDeviceProp_t p;
+    HSR_HIP(hipGetDeviceProperties(&p, device));
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    out->compute_units = p.multiProcessorCount;
+    out->xcds = 8;
+    out->hbm_bytes = (int64_t)p.totalGlobalMem;
+    out->lds_bytes_per_cu = (int32_t)p.maxSharedMemoryPerMultiProcessor;
+    out->clock_mhz = p.clockRate / 1000;
+    return HSR_OK;
+}
+
+hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int32_t *indices,
+                          int64_t triangleCount, int32_t device, hsr_backend **out) {
+    if (!out) return fail(HSR_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (vertexCount < 0 || triangleCount < 0 || (triangleCount > 0 && (!vertices || !indices)))
+        return fail(HSR_ERR_INVALID, "invalid mesh buffers");
+    int ndev = 0;
+    HSR_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(HSR_ERR_INVALID, "no such HIP device");
+    std::unique_ptr<hsr_backend> b(new hsr_backend());
+    b->device = device;
+    HSR_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HSR_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(HSR_ERR_UNSUPPORTED, std::string("this library is built for gfx950, device is ") +
+                                             prop.gcnArchName);
+    b->cu = prop.multiProcessorCount;
+    HSR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto &e : b->ev) HSR_HIP(hipEventCreate(&e));
+
+    hsr::FlatBVH bvh;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!hsr::build_bvh(vertices, vertexCount, indices, triangleCount, bvh))
+        return fail(HSR_ERR_INVALID, "BVH build failed (index out of range, too many triangles or depth)");
+    b->bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    b->nodeCount = bvh.nodeCount;
+    b->slotCount = bvh.slotCount;
+    b->bvhDepth = bvh.maxDepth;
+    b->triangles = triangleCount;
+    hsr_status s;
+    if ((s = dalloc(&b->d_nodes, (size_t)bvh.nodeCount * 4)) != HSR_OK) return s;
+    if ((s = dalloc(&b->d_tris, (size_t)bvh.slotCount * 3)) != HSR_OK) return s;
+    HSR_HIP(hipMemcpy(b->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(float), hipMemcpyHostToDevice));
+    HSR_HIP(hipMemcpy(b->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(float), hipMemcpyHostToDevice));
+    if ((s = dalloc(&b->d_counters, 8)) != HSR_OK) return s;
+    b->stats.bvh_build_ms = b->bvh_ms;
+    *out = b.release();
+    return HSR_OK;
+}
+
+void hsr_hip_destroy(hsr_backend *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    dfree(b->d_nodes);
+    dfree(b->d_tris);
+    dfree(b->d_frame_sum);
+    dfree(b->d_frame_comp);
+    dfree(b->d_partials);
+    dfree(b->d_stage_sum);
+    dfree(b->d_stage_comp);
+    dfree(b->d_counters);
+    dfree(b->d_recHits);
+    dfree(b->d_recNee);
+    for (auto &e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+hsr_status hsr_hip_configure_frame(hsr_backend *b, const hsr_frame_config *c) {
+    if (!b || !c) return fail(HSR_ERR_INVALID, "null argument");
+    if (c->chirpCount < 1 || c->samplesPerChirp < 1 || c->channelCount < 1 || c->channelCount > HSR_MAX_CHANNELS)
+        return fail(HSR_ERR_INVALID, "frame dimensions out of range");
+    if (c->chirpCount != 1)
+        return fail(HSR_ERR_UNSUPPORTED, "only chirpCount == 1: the integrator has no motion (integrator.h:130)");
+    if (hsr::trace_lds_bytes(c->samplesPerChirp, c->channelCount) > 65536)
+        return fail(HSR_ERR_UNSUPPORTED, "frame tile exceeds 64 KiB of LDS (samples x channels too large)");
+    HSR_HIP(hipSetDevice(b->device));
+    const int64_t nd = 2LL * c->chirpCount * c->samplesPerChirp * c->channelCount;
+    if (nd != b->frameDoubles) {
+        dfree(b->d_frame_sum);
+        dfree(b->d_frame_comp);
+        hsr_status s;
+        if ((s = dalloc(&b->d_frame_sum, nd)) != HSR_OK) return s;
+        if ((s = dalloc(&b->d_frame_comp, nd)) != HSR_OK) return s;
+        b->frameDoubles = nd;
+    }
+    b->fc = *c;
+    b->configured = true;
+    b->maxGrid = 0;
+    return hsr_hip_clear_frame(b);
+}
+
+hsr_status hsr_hip_clear_frame(hsr_backend *b) {
+    if (!b) return fail(HSR_ERR_INVALID, "null backend");
+    if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured");
+    HSR_HIP(hipSetDevice(b->device));
+    HSR_HIP(hsr::launch_zero(b->d_frame_sum, b->frameDoubles, b->stream));
+    HSR_HIP(hsr::launch_zero(b->d_frame_comp, b->frameDoubles, b->stream));
+    b->totalWeight = 0.0;
+    return HSR_OK;
+}
+
+hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
+    if (!b) return fail(HSR_ERR_INVALID, "null backend");
+    switch (option) {
+        case HSR_OPT_COUNTERS:
+            b->optCounters = value != 0;
+            b->maxGrid = 0;
+            return HSR_OK;
+        case HSR_OPT_RECORD_HITS: b->optRecord = std::max<int64_t>(0, value); return HSR_OK;
+        case HSR_OPT_PATHS_PER_LAUNCH:
+            if (value < 1) return fail(HSR_ERR_INVALID, "paths per launch must be positive");
+            b->optPathsPerLaunch = value;
+            return HSR_OK;
+        case HSR_OPT_PIPELINE:
+            if (value != 0) return fail(HSR_ERR_UNSUPPORTED, "only the fused path kernel is available");
+            return HSR_OK;
+        case HSR_OPT_PHASE_MODE:
+            if (value < 0 || value > 1) return fail(HSR_ERR_UNSUPPORTED, "phase mode must be 0 (FP64) or 1 (FP32)");
+            b->optPhaseMode = (int32_t)value;
+            return HSR_OK;
+        default: return fail(HSR_ERR_INVALID, "unknown option");
+    }
+}
+
+hsr_status hsr_hip_last_stats(hsr_backend *b, hsr_run_stats *out) {
+    if (!b || !out) return fail(HSR_ERR_INVALID, "null argument");
+    *out = b->stats;
+    return HSR_OK;
+}
+
+static hsr_status guide_upload(hsr_guide *g);
+
+static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
+    const hsr_rf_config &rf = sc->rfConfig;
+    a.startFreq = rf.startFreq;
+    a.freqSlope = rf.freqSlope;
+    a.adcRate = rf.adcRate;
+    a.rampTime = rf.rampTime;
+    a.antennaDelay = rf.antennaDelay;
+    a.rxCount = sc->rxCount;
+    std::memcpy(a.tx.pos, sc->tx.position, sizeof(a.tx.pos));
+    std::memcpy(a.tx.rot, sc->tx.rotation, sizeof(a.tx.rot));
+    for (int i = 0; i < HSR_MAX_CHANNELS; ++i) {
+        std::memcpy(a.rx[i].pos, sc->rx[i].position, sizeof(a.rx[i].pos));
+        std::memcpy(a.rx[i].rot, sc->rx[i].rotation, sizeof(a.rx[i].rot));
+    }
+}
+
+hsr_status hsr_hip_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
+                       int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
+                       const volatile int32_t *interruptFlag) {
+    if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
+    if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured (hsr_hip_configure_frame)");
+    if (budget < 0 || sampleIndexOffset < 0) return fail(HSR_ERR_INVALID, "negative budget or offset");
+    if (knobs->correctPhase) return fail(HSR_ERR_UNSUPPORTED, "correctPhase is not supported");
+    if (!knobs->useGeometricalOptics) return fail(HSR_ERR_UNSUPPORTED, "only geometrical optics is supported");
+    if (knobs->maxDepth < 0) return fail(HSR_ERR_INVALID, "negative maxDepth");
+    if (scene->rxCount < 1 || scene->rxCount > HSR_MAX_CHANNELS) return fail(HSR_ERR_INVALID, "rxCount out of range");
+    if (scene->rxCount > b->fc.channelCount)
+        return fail(HSR_ERR_INVALID, "more receivers than frame channels");
+    if (knobs->doGuiding && !guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");
+    if (guide && guide->backend != b) return fail(HSR_ERR_INVALID, "guide belongs to another backend");
+    if (!(scene->rfConfig.adcRate > 0.0f)) return fail(HSR_ERR_INVALID, "adcRate must be positive");
+    HSR_HIP(hipSetDevice(b->device));
+    if (guide && guide->dirty) {
+        hsr_status s = guide_upload(guide);
+        if (s != HSR_OK) return s;
+    }
+
+    const int N = b->fc.samplesPerChirp, C = b->fc.channelCount;
+    const int M = 2 * N * C;
+    hsr::TraceArgs a;
+    std::memset(&a, 0, sizeof(a));
+    fill_scene(a, scene);
+    a.onlyIndirect = knobs->onlyIndirect;
+    a.maxDepth = knobs->maxDepth;
+    a.poDiffraction = knobs->poDiffraction;
+    a.doGuiding = knobs->doGuiding;
+    a.doFiltering = knobs->doFiltering;
+    a.filteringSphere = knobs->filteringSphere;
+    a.filterGuiding = knobs->filterGuiding;
+    a.primaryOrder = knobs->primaryOrder;
+    a.produceDebugImage = knobs->produceDebugImage;
+    a.filteringMin = knobs->filteringMin;
+    a.filteringMax = knobs->filteringMax;
+    a.filteringRadius = knobs->filteringRadius;
+    a.phaseMode = b->optPhaseMode;
+    a.N = N;
+    a.C = C;
+    a.sampleWeight = sampleWeight;
+    a.isFinal = isFinalIteration;
+    a.nodes = b->d_nodes;
+    a.tris = b->d_tris;
+    if (guide) {
+        a.gsChild = guide->d_sChild;
+        a.gsP0 = guide->d_sP0;
+        a.gtChild = guide->d_tChild;
+        a.gtStat = guide->d_tStat;
+        a.uniformProb = guide->host.uniformProb;
+        a.secondMoment = guide->host.secondMoment;
+        a.guideOn = 1;
+    }
+    a.counters = b->d_counters;
+
+    // grid: as many workgroups as fit at once, fewer for small budgets (>= 8 paths per lane)
+    if (b->maxGrid == 0) {
+        const int per_cu = hsr::trace_blocks_per_cu(N, C, b->optCounters);
+        b->maxGrid = b->cu * std::max(1, per_cu);
+    }
+
+    // hit record
+    const int64_t recPaths = std::min(b->optRecord, budget);
+    b->recPathsLast = recPaths;
+    const int md = std::max(1, knobs->maxDepth);
+    if (recPaths > 0) {
+        const int64_t need = recPaths * md * C;
+        if (need > b->recCap) {
+            dfree(b->d_recHits);
+            dfree(b->d_recNee);
+            hsr_status s;
+            if ((s = dalloc(&b->d_recHits, (size_t)recPaths * md)) != HSR_OK) return s;
+            if ((s = dalloc(&b->d_recNee, (size_t)need)) != HSR_OK) return s;
+            b->recCap = need;
+        }
+        HSR_HIP(hipMemsetAsync(b->d_recHits, 0xFF, (size_t)recPaths * md * sizeof(int32_t), b->stream));
+        HSR_HIP(hipMemsetAsync(b->d_recNee, 0, (size_t)need, b->stream));
+        b->recMaxDepth = md;
+        b->recC = C;
+    }
+    if (b->optCounters) HSR_HIP(hipMemsetAsync(b->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
+
+    b->stats = hsr_run_stats{};
+    b->stats.bvh_build_ms = b->bvh_ms;
+    double trace_ms = 0.0, reduce_ms = 0.0;
+    int64_t launches = 0;
+    bool first = true;
+    for (int64_t done = 0; done < budget;) {
+        if (interruptFlag && *interruptFlag) return fail(HSR_ERR_INTERRUPTED, "interrupted");
+        const int64_t n = std::min(budget - done, b->optPathsPerLaunch);
+        int grid = (int)std::min<int64_t>(b->maxGrid, (n + 8 * hsr::kTraceBlock - 1) / (8 * hsr::kTraceBlock));
+        grid = std::max(grid, 1);
+        const int64_t waves = (int64_t)grid * (hsr::kTraceBlock / 64);
+        a.offset = sampleIndexOffset + done;
+        a.count = n;
+        a.chunk = (n + waves - 1) / waves;
+        // record: only the first recPaths paths of the whole run, i.e. of the first launch
+        if (recPaths > 0 && done == 0) {
+            a.recHits = b->d_recHits;
+            a.recNee = b->d_recNee;
+            a.recPaths = std::min(recPaths, n);
+        } else {
+            a.recHits = nullptr;
+            a.recNee = nullptr;
+            a.recPaths = 0;
+        }
+        const size_t needP = (size_t)grid * M;
+        if (needP > b->partialCap) {
+            dfree(b->d_partials);
+            hsr_status s;
+            if ((s = dalloc(&b->d_partials, needP)) != HSR_OK) return s;
+            b->partialCap = needP;
+        }
+        const int S = std::min(64, grid);
+        if ((size_t)S * M > b->stageCap) {
+            dfree(b->d_stage_sum);
+            dfree(b->d_stage_comp);
+            hsr_status s;
+            if ((s = dalloc(&b->d_stage_sum, (size_t)S * M)) != HSR_OK) return s;
+            if ((s = dalloc(&b->d_stage_comp, (size_t)S * M)) != HSR_OK) return s;
+            b->stageCap = (size_t)S * M;
+        }
+        a.partials = (double2 *)b->d_partials;
+        HSR_HIP(hipEventRecord(b->ev[0], b->stream));
+        if (first) HSR_HIP(hipEventRecord(b->ev[2], b->stream));
+        first = false;
+        HSR_HIP(hsr::launch_trace(a, grid, b->optCounters, b->stream));
+        HSR_HIP(hipEventRecord(b->ev[1], b->stream));
+        HSR_HIP(hsr::launch_reduce(b->d_partials, grid, M, b->d_stage_sum, b->d_stage_comp, S, b->d_frame_sum,
+                                   b->d_frame_comp, b->stream));
+        HSR_HIP(hipEventRecord(b->ev[3], b->stream));
+        HSR_HIP(hipEventSynchronize(b->ev[3]));
+        float ms1 = 0.0f, ms2 = 0.0f;
+        HSR_HIP(hipEventElapsedTime(&ms1, b->ev[0], b->ev[1]));
+        HSR_HIP(hipEventElapsedTime(&ms2, b->ev[1], b->ev[3]));
+        trace_ms += ms1;
+        reduce_ms += ms2;
+        launches += 3;
+        done += n;
+    }
+    float total = 0.0f;
+    if (budget > 0) HSR_HIP(hipEventElapsedTime(&total, b->ev[2], b->ev[3]));
+    b->totalWeight += (double)budget * (double)sampleWeight;
+    b->stats.trace_ms = trace_ms;
+    b->stats.reduce_ms = reduce_ms;
+    b->stats.total_ms = total;
+    b->stats.paths = budget;
+    b->stats.launches = launches;
+    if (b->optCounters) {
+        unsigned long long c[8];
+        HSR_HIP(hipMemcpy(c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+        b->stats.nodes_visited = (int64_t)c[0];
+        b->stats.tris_tested = (int64_t)c[1];
+        b->stats.rays_closest = (int64_t)c[2];
+        b->stats.rays_shadow = (int64_t)c[3];
+        b->stats.events = (int64_t)c[4];
+    }
+    return HSR_OK;
+}
+
+hsr_status hsr_hip_fetch_frame(hsr_backend *b, float *out_c64, double *out_c128, double *totalWeight) {
+    if (!b) return fail(HSR_ERR_INVALID, "null backend");
+    if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured");
+    HSR_HIP(hipSetDevice(b->device));
+    HSR_HIP(hipStreamSynchronize(b->stream));
+    std::vector<double> s(b->frameDoubles), c(b->frameDoubles);
+    HSR_HIP(hipMemcpy(s.data(), b->d_frame_sum, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HSR_HIP(hipMemcpy(c.data(), b->d_frame_comp, c.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const double tw = b->totalWeight;
+    const double inv = tw != 0.0 ? 1.0 / tw : 0.0;
+    for (int64_t i = 0; i < b->frameDoubles; ++i) {
+        const double v = (s[i] - c[i]) * inv;
+        if (out_c128) out_c128[i] = v;
+        if (out_c64) out_c64[i] = (float)v;
+    }
+    if (totalWeight) *totalWeight = tw;
+    return HSR_OK;
+}
+
+hsr_status hsr_hip_get_hit_record(hsr_backend *b, int32_t *hit_ids, int8_t *nee, int64_t paths, int32_t maxDepth,
+                                  int32_t rxCount) {
+    if (!b) return fail(HSR_ERR_INVALID, "null backend");
+    if (paths > b->recPathsLast || maxDepth != b->recMaxDepth || rxCount != b->recC)
+        return fail(HSR_ERR_INVALID, "record shape does not match the last run");
+    HSR_HIP(hipSetDevice(b->device));
+    HSR_HIP(hipStreamSynchronize(b->stream));
+    if (hit_ids)
+        HSR_HIP(hipMemcpy(hit_ids, b->d_recHits, (size_t)paths * maxDepth * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (nee) HSR_HIP(hipMemcpy(nee, b->d_recNee, (size_t)paths * maxDepth * rxCount, hipMemcpyDeviceToHost));
+    return HSR_OK;
+}
+
+// ---------------------------------------------------------------- guiding
+static hsr_status guide_upload(hsr_guide *g) {
+    const hsr::GuideTree &s = g->host.sampling, &t = g->host.training;
+    if (s.size() > g->capacity || t.size() > g->capacity) return fail(HSR_ERR_INVALID, "guide tree too large");
+    HSR_HIP(hipStreamSynchronize(g->backend->stream));
+    HSR_HIP(hipMemcpy(g->d_sChild, s.child.data(), s.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HSR_HIP(hipMemcpy(g->d_sP0, s.p0.data(), s.size() * sizeof(float), hipMemcpyHostToDevice));
+    HSR_HIP(hipMemcpy(g->d_tChild, t.child.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HSR_HIP(hipMemcpy(g->d_tStat, t.stat.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    g->dirty = false;
+    return HSR_OK;
+}
+
+static hsr_status guide_download_stats(hsr_guide *g) {
+    hsr::GuideTree &t = g->host.training;
+    HSR_HIP(hipMemcpy(t.stat.data(), g->d_tStat, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return HSR_OK;
+}
+
+hsr_status hsr_guide_create(hsr_backend *b, float uniformProb, float splitThreshold, int32_t secondMoment,
+                            hsr_guide **out) {
+    if (!b || !out) return fail(HSR_ERR_INVALID, "null argument");
+    if (!(uniformProb >= 0.0f && uniformProb <= 1.0f)) return fail(HSR_ERR_INVALID, "uniformProb outside [0, 1]");
+    *out = nullptr;
+    HSR_HIP(hipSetDevice(b->device));
+    std::unique_ptr<hsr_guide> g(new hsr_guide());
+    g->backend = b;
+    g->host.uniformProb = uniformProb;
+    g->host.splitThreshold = splitThreshold;
+    g->host.secondMoment = secondMoment != 0;
+    hsr_status s;
+    if ((s = dalloc(&g->d_sChild, g->capacity)) != HSR_OK) return s;
+    if ((s = dalloc(&g->d_sP0, g->capacity)) != HSR_OK) return s;
+    if ((s = dalloc(&g->d_tChild, g->capacity)) != HSR_OK) return s;
+    if ((s = dalloc(&g->d_tStat, g->capacity)) != HSR_OK) return s;
+    g->host.reset();
+    if ((s = guide_upload(g.get())) != HSR_OK) return s;
+    *out = g.release();
+    return HSR_OK;
+}
+
+void hsr_guide_destroy(hsr_guide *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->backend->device);
+    dfree(g->d_sChild);
+    dfree(g->d_sP0);
+    dfree(g->d_tChild);
+    dfree(g->d_tStat);
+    delete g;
+}
+
+hsr_status hsr_guide_reset(hsr_guide *g) {
+    if (!g) return fail(HSR_ERR_INVALID, "null guide");
+    HSR_HIP(hipSetDevice(g->backend->device));
+    g->host.reset();
+    return guide_upload(g);
+}
+
+hsr_status hsr_guide_step(hsr_guide *g) {
+    if (!g) return fail(HSR_ERR_INVALID, "null guide");
+    HSR_HIP(hipSetDevice(g->backend->device));
+    HSR_HIP(hipStreamSynchronize(g->backend->stream));
+    hsr_status s = guide_download_stats(g);
+    if (s != HSR_OK) return s;
+    g->host.step();
+    return guide_upload(g);
+}
+
+hsr_status hsr_guide_sampling_size(hsr_guide *g, int32_t *n) {
+    if (!g || !n) return fail(HSR_ERR_INVALID, "null argument");
+    *n = g->host.sampling.size();
+    return HSR_OK;
+}
+
+hsr_status hsr_guide_get_sampling(hsr_guide *g, int32_t *child, float *p0, int32_t capacity) {
+    if (!g) return fail(HSR_ERR_INVALID, "null guide");
+    const hsr::GuideTree &s = g->host.sampling;
+    if (capacity < s.size()) return fail(HSR_ERR_INVALID, "capacity too small");
+    if (child) std::memcpy(child, s.child.data(), s.size() * sizeof(int32_t));
+    if (p0) std::memcpy(p0, s.p0.data(), s.size() * sizeof(float));
+    return HSR_OK;
+}
+
+hsr_status hsr_guide_set_sampling(hsr_guide *g, const int32_t *child, const float *p0, int32_t n) {
+    if (!g || !child || !p0 || n < 1 || n > g->capacity) return fail(HSR_ERR_INVALID, "invalid sampling tree");
+    hsr::GuideTree &s = g->host.sampling;
+    s.child.assign(child, child + n);
+    s.p0.assign(p0, p0 + n);
+    s.stat.assign(n, 0.0);
+    s.depth.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t c = s.child[i];
+        if (c != 0) {
+            if (c <= i || c + 1 >= n) return fail(HSR_ERR_INVALID, "child index out of order");
+            s.depth[c] = s.depth[c + 1] = s.depth[i] + 1;
+        }
+    }
+    HSR_HIP(hipSetDevice(g->backend->device));
+    return guide_upload(g);
+}
+
+hsr_status hsr_guide_training_size(hsr_guide *g, int32_t *n) {
+    if (!g || !n) return fail(HSR_ERR_INVALID, "null argument");
+    *n = g->host.training.size();
+    return HSR_OK;
+}
+
+hsr_status hsr_guide_get_training(hsr_guide *g, int32_t *child, double *stat, int32_t capacity) {
+    if (!g) return fail(HSR_ERR_INVALID, "null guide");
+    HSR_HIP(hipSetDevice(g->backend->device));
+    HSR_HIP(hipStreamSynchronize(g->backend->stream));
+    hsr_status s = guide_download_stats(g);
+    if (s != HSR_OK) return s;
+    const hsr::GuideTree &t = g->host.training;
+    if (capacity < t.size()) return fail(HSR_ERR_INVALID, "capacity too small");
+    if (child) std::memcpy(child, t.child.data(), t.size() * sizeof(int32_t));
+    if (stat) std::memcpy(stat, t.stat.data(), t.size() * sizeof(double));
+    return HSR_OK;
+}
+
+// ---------------------------------------------------------------- PathTracer::run
+hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t samples,
+                               hsr_guide *guide, const volatile int32_t *interruptFlag) {
+    if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
+    hsr_status s = hsr_hip_clear_frame(b);  // setup() + clearFrame(), path.h:173-174
+    if (s != HSR_OK) return s;
+    int64_t offset = 0;
+    float weight = 1.0f;
+    if (!knobs->doGuiding) return hsr_hip_run(b, scene, knobs, 0, samples, 1.0f, 0, nullptr, interruptFlag);
+    if (!guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");
+    if ((s = hsr_guide_reset(guide)) != HSR_OK) return s;
+    bool isFinal = false;
+    int64_t milestone = 16384, remaining = samples;
+    while (true) {
+        milestone = std::min(milestone, remaining);
+        s = hsr_hip_run(b, scene, knobs, offset, milestone, weight, isFinal, guide, interruptFlag);
+        if (s != HSR_OK) return s;
+        offset += milestone;
+        remaining -= milestone;
+        if (remaining == 0) break;
+        milestone *= 2;
+        if (remaining < milestone * 2) {
+            isFinal = true;
+            milestone = remaining;
+        }
+        if (knobs->clearBeforeIteration) {
+            if ((s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
+            offset = 0;
+        } else {
+            weight = 10.0f * weight;
+        }
+        if ((s = hsr_guide_step(guide)) != HSR_OK) return s;
+    }
+    return HSR_OK;
+}
+
+hsr_status hsr_path_tracer_run_schedule(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs,
+                                        int32_t trainingPasses, int64_t trainingPaths, int64_t finalPaths,
+                                        hsr_guide *guide, const volatile int32_t *interruptFlag) {
+    if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
+    if (trainingPasses < 0 || trainingPaths < 0 || finalPaths < 0) return fail(HSR_ERR_INVALID, "negative schedule");
+    hsr_status s;
+    const bool guiding = knobs->doGuiding && guide;
+    if (knobs->doGuiding && !guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");
+    if (guiding) {
+        if ((s = hsr_guide_reset(guide)) != HSR_OK) return s;
+        for (int p = 0; p < trainingPasses; ++p) {
+            if ((s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
+            if ((s = hsr_hip_run(b, scene, knobs, 0, trainingPaths, 1.0f, 0, guide, interruptFlag)) != HSR_OK) return s;
+            if ((s = hsr_guide_step(guide)) != HSR_OK) return s;
+        }
+    }
+    if ((s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
+    return hsr_hip_run(b, scene, knobs, 0, finalPaths, 1.0f, 1, guiding ? guide : nullptr, interruptFlag);
+}
+
+}  // extern "C"

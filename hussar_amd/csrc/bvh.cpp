@@ -1,0 +1,273 @@
+// Binned-SAH BVH2 builder (host). Produces the two-children-per-node layout the traversal
+// kernels fetch as four float4 per node (Aila & Laine 2009 style), with triangle slots in leaf
+// order. Subtrees above a size threshold are built in parallel with std::thread.
+#include "bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <thread>
+
+namespace hsr {
+namespace {
+
+constexpr int kBins = 16;
+constexpr int kMaxLeaf = 8;
+constexpr int kMedianDepth = 48;
+constexpr int kMaxDepth = 80;
+
+struct Box {
+    float lo[3], hi[3];
+    void reset() {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::numeric_limits<float>::infinity();
+            hi[k] = -std::numeric_limits<float>::infinity();
+        }
+    }
+    void grow(const Box &b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0.0f)) return 0.0f;
+        return dx * dy + dy * dz + dz * dx;
+    }
+};
+
+struct Builder {
+    const float *V;
+    const int32_t *I;
+    std::vector<Box> tbox;
+    std::vector<float> cen;          // 3 per triangle
+    std::vector<int32_t> order;      // slot -> triangle id
+    // output (inner nodes), guarded for parallel subtrees
+    std::mutex mtx;
+    std::vector<std::vector<float>> chunks;  // per-subtree node arrays, merged at the end
+    std::atomic<int32_t> maxDepth{0};
+    std::atomic<int32_t> maxLeaf{0};
+
+    // Widened float box of a child (lo rounded down, hi rounded up).
+    static void widen(const Box &b, float lo[3], float hi[3]) {
+        for (int k = 0; k < 3; ++k) {
+            const double l = b.lo[k], h = b.hi[k];
+            const double m = (std::fabs(l) + std::fabs(h) + (h - l)) * (1.0 / 1048576.0) + 1e-6;
+            lo[k] = std::nextafter((float)(l - m), -std::numeric_limits<float>::infinity());
+            hi[k] = std::nextafter((float)(h + m), std::numeric_limits<float>::infinity());
+        }
+    }
+
+    Box range_box(int32_t b, int32_t e, Box *cbox) const {
+        Box bb, cb;
+        bb.reset();
+        cb.reset();
+        for (int32_t i = b; i < e; ++i) {
+            const int32_t t = order[i];
+            bb.grow(tbox[t]);
+            for (int k = 0; k < 3; ++k) {
+                cb.lo[k] = std::min(cb.lo[k], cen[3 * t + k]);
+                cb.hi[k] = std::max(cb.hi[k], cen[3 * t + k]);
+            }
+        }
+        if (cbox) *cbox = cb;
+        return bb;
+    }
+
+    // Returns the split position (b < mid < e) or -1 for a leaf.
+    int32_t split(int32_t b, int32_t e, int depth) {
+        const int32_t n = e - b;
+        if (n <= 1) return -1;
+        Box cb;
+        const Box bb = range_box(b, e, &cb);
+        if (depth >= kMedianDepth) {
+            int axis = 0;
+            for (int k = 1; k < 3; ++k)
+                if (cb.hi[k] - cb.lo[k] > cb.hi[axis] - cb.lo[axis]) axis = k;
+            const int32_t mid = b + n / 2;
+            std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
+                             [&](int32_t x, int32_t y) { return cen[3 * x + axis] < cen[3 * y + axis]; });
+            return n <= kMaxLeaf && depth >= kMaxDepth - 1 ? -1 : mid;
+        }
+        float best = std::numeric_limits<float>::infinity();
+        int best_axis = -1, best_bin = -1;
+        for (int k = 0; k < 3; ++k) {
+            const float ext = cb.hi[k] - cb.lo[k];
+            if (!(ext > 0.0f)) continue;
+            Box bins[kBins];
+            int32_t cnt[kBins] = {0};
+            for (auto &x : bins) x.reset();
+            const float scale = (float)kBins / ext;
+            for (int32_t i = b; i < e; ++i) {
+                const int32_t t = order[i];
+                int bi = (int)((cen[3 * t + k] - cb.lo[k]) * scale);
+                bi = std::min(std::max(bi, 0), kBins - 1);
+                cnt[bi]++;
+                bins[bi].grow(tbox[t]);
+            }
+            float rarea[kBins];
+            int32_t rcnt[kBins];
+            Box acc;
+            acc.reset();
+            int32_t c = 0;
+            for (int i = kBins - 1; i > 0; --i) {
+                acc.grow(bins[i]);
+                c += cnt[i];
+                rarea[i] = acc.area();
+                rcnt[i] = c;
+            }
+            acc.reset();
+            c = 0;
+            for (int i = 0; i < kBins - 1; ++i) {
+                acc.grow(bins[i]);
+                c += cnt[i];
+                if (c == 0 || rcnt[i + 1] == 0) continue;
+                const float cost = acc.area() * (float)c + rarea[i + 1] * (float)rcnt[i + 1];
+                if (cost < best) {
+                    best = cost;
+                    best_axis = k;
+                    best_bin = i;
+                }
+            }
+        }
+        const float leaf_cost = bb.area() * (float)n;
+        if (best_axis < 0) {
+            if (n <= kMaxLeaf) return -1;
+            return b + n / 2;  // coincident centroids: split by count
+        }
+        // SAH with traversal cost 1 per node test relative to 1 per triangle test
+        if (n <= kMaxLeaf && leaf_cost <= best + bb.area()) return -1;
+        const int k = best_axis;
+        const float scale = (float)kBins / (cb.hi[k] - cb.lo[k]);
+        auto it = std::partition(order.begin() + b, order.begin() + e, [&](int32_t t) {
+            int bi = (int)((cen[3 * t + k] - cb.lo[k]) * scale);
+            bi = std::min(std::max(bi, 0), kBins - 1);
+            return bi <= best_bin;
+        });
+        int32_t mid = (int32_t)(it - order.begin());
+        if (mid == b || mid == e) mid = b + n / 2;
+        return mid;
+    }
+
+    static int32_t leaf_ref(int32_t first, int32_t count) {
+        return (int32_t)(0x80000000u | ((uint32_t)(count - 1) << 27) | (uint32_t)first);
+    }
+
+    // Build the subtree over [b, e) into `nodes` (local indices); returns its root ref.
+    // Inner node refs are local to `nodes` with `base` added.
+    int32_t build(int32_t b, int32_t e, int depth, std::vector<float> &nodes, int32_t base) {
+        const int32_t mid = split(b, e, depth);
+        if (mid < 0) {
+            int32_t d = depth, cur = maxDepth.load();
+            while (d > cur && !maxDepth.compare_exchange_weak(cur, d)) {}
+            int32_t l = e - b, cl = maxLeaf.load();
+            while (l > cl && !maxLeaf.compare_exchange_weak(cl, l)) {}
+            return leaf_ref(b, e - b);
+        }
+        const int32_t me = (int32_t)(nodes.size() / 16);
+        nodes.resize(nodes.size() + 16, 0.0f);
+        const int32_t r0 = build(b, mid, depth + 1, nodes, base);
+        const int32_t r1 = build(mid, e, depth + 1, nodes, base);
+        fill(nodes, me, b, mid, e, r0, r1, base);
+        return base + me;
+    }
+
+    void fill(std::vector<float> &nodes, int32_t me, int32_t b, int32_t mid, int32_t e, int32_t r0, int32_t r1,
+              int32_t) {
+        float lo0[3], hi0[3], lo1[3], hi1[3];
+        widen(range_box(b, mid, nullptr), lo0, hi0);
+        widen(range_box(mid, e, nullptr), lo1, hi1);
+        float *n = &nodes[16 * (size_t)me];
+        n[0] = lo0[0]; n[1] = hi0[0]; n[2] = lo0[1]; n[3] = hi0[1];
+        n[4] = lo1[0]; n[5] = hi1[0]; n[6] = lo1[1]; n[7] = hi1[1];
+        n[8] = lo0[2]; n[9] = hi0[2]; n[10] = lo1[2]; n[11] = hi1[2];
+        std::memcpy(&n[12], &r0, 4);
+        std::memcpy(&n[13], &r1, 4);
+        n[14] = 0.0f;
+        n[15] = 0.0f;
+    }
+};
+
+}  // namespace
+
+bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH &out, int threads) {
+    (void)threads;
+    if (nt < 0 || nt >= (1 << 27)) return false;
+    for (int64_t t = 0; t < 3 * nt; ++t)
+        if (I[t] < 0 || I[t] >= nv) return false;
+    Builder B;
+    B.V = V;
+    B.I = I;
+    B.tbox.resize(nt);
+    B.cen.resize(3 * nt);
+    B.order.resize(nt);
+    for (int64_t t = 0; t < nt; ++t) {
+        Box bx;
+        bx.reset();
+        for (int q = 0; q < 3; ++q) {
+            const float *p = V + 3 * (int64_t)I[3 * t + q];
+            for (int k = 0; k < 3; ++k) {
+                bx.lo[k] = std::min(bx.lo[k], p[k]);
+                bx.hi[k] = std::max(bx.hi[k], p[k]);
+            }
+        }
+        B.tbox[t] = bx;
+        for (int k = 0; k < 3; ++k) B.cen[3 * t + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
+        B.order[t] = (int32_t)t;
+    }
+
+    std::vector<float> nodes;
+    nodes.reserve(16 * (size_t)std::max<int64_t>(2, nt));
+    int32_t slotCount = (int32_t)nt;
+    if (nt == 0) {
+        // a root whose two children are a far point box over one degenerate triangle slot
+        slotCount = 1;
+        nodes.assign(16, 1e30f);
+        const int32_t ref = Builder::leaf_ref(0, 1);
+        std::memcpy(&nodes[12], &ref, 4);
+        std::memcpy(&nodes[13], &ref, 4);
+        nodes[14] = nodes[15] = 0.0f;
+    } else {
+        // the root must be an inner node: its children are the two halves of the first split,
+        // or the single leaf twice
+        const int32_t mid = nt > 1 ? B.split(0, (int32_t)nt, 0) : -1;
+        nodes.resize(16, 0.0f);
+        if (mid < 0) {
+            const int32_t ref = Builder::leaf_ref(0, (int32_t)nt);
+            B.fill(nodes, 0, 0, (int32_t)nt, (int32_t)nt, ref, ref, 0);
+            float *n = &nodes[0];
+            for (int k = 0; k < 4; ++k) { n[4 + k] = n[k]; }
+            n[10] = n[8];
+            n[11] = n[9];
+            B.maxLeaf = (int32_t)nt;
+        } else {
+            const int32_t r0 = B.build(0, mid, 1, nodes, 0);
+            const int32_t r1 = B.build(mid, (int32_t)nt, 1, nodes, 0);
+            B.fill(nodes, 0, 0, mid, (int32_t)nt, r0, r1, 0);
+        }
+    }
+    out.nodes.swap(nodes);
+    out.nodeCount = (int32_t)(out.nodes.size() / 16);
+    out.slotCount = slotCount;
+    out.maxDepth = B.maxDepth.load();
+    out.maxLeaf = B.maxLeaf.load();
+    out.tris.assign(12 * (size_t)slotCount, 0.0f);
+    for (int32_t s = 0; s < (int32_t)nt; ++s) {
+        const int32_t t = B.order[s];
+        const float *a = V + 3 * (int64_t)I[3 * t + 0];
+        const float *b = V + 3 * (int64_t)I[3 * t + 1];
+        const float *c = V + 3 * (int64_t)I[3 * t + 2];
+        float *r = &out.tris[12 * (size_t)s];
+        r[0] = a[0]; r[1] = a[1]; r[2] = a[2];
+        std::memcpy(&r[3], &t, 4);
+        r[4] = b[0] - a[0]; r[5] = b[1] - a[1]; r[6] = b[2] - a[2]; r[7] = 0.0f;
+        r[8] = c[0] - a[0]; r[9] = c[1] - a[1]; r[10] = c[2] - a[2]; r[11] = 0.0f;
+    }
+    return out.maxDepth < kMaxDepth;
+}
+
+}  // namespace hsr

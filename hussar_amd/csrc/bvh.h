@@ -1,0 +1,31 @@
+// Host-side BVH2 builder for the gfx950 traversal kernels.
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+
+namespace hsr {
+
+// Flattened BVH ready for upload.
+//  nodes: 16 floats per inner node (64 B): child0 x lo/hi, y lo/hi | child1 x lo/hi, y lo/hi |
+//         child0 z lo/hi, child1 z lo/hi | child0 ref, child1 ref, 0, 0 (refs as int bits).
+//         Node 0 is the root. A ref >= 0 is an inner node; a negative ref is a leaf:
+//         0x80000000 | (count-1) << 27 | firstSlot.
+//  tris:  12 floats per triangle slot (48 B): v0.xyz, id (int bits) | e1.xyz, 0 | e2.xyz, 0
+//         with e1 = v1 - v0 and e2 = v2 - v0 evaluated in float.
+struct FlatBVH {
+    std::vector<float> nodes;
+    std::vector<float> tris;
+    int32_t nodeCount = 0;
+    int32_t slotCount = 0;
+    int32_t maxDepth = 0;
+    int32_t maxLeaf = 0;
+};
+
+// Binned SAH (16 bins, all three axes), leaves of up to 8 triangles, depth <= 80 (median
+// splits below depth 48). Child boxes are widened by a relative margin of 2^-20 plus 1e-6 m
+// so that a ray accepted by the triangle test is never culled by a box test.
+bool build_bvh(const float *vertices, int64_t vertexCount, const int32_t *indices, int64_t triangleCount,
+               FlatBVH &out, int threads = 0);
+
+}  // namespace hsr

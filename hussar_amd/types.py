@@ -57,7 +57,8 @@ class c_knobs(C.Structure):
                 ("correctPhase", C.c_int32), ("filterGuiding", C.c_int32),
                 ("doFiltering", C.c_int32), ("filteringSphere", C.c_int32),
                 ("filteringMin", C.c_float), ("filteringMax", C.c_float),
-                ("filteringRadius", C.c_float), ("primaryOrder", C.c_int32)]
+                ("filteringRadius", C.c_float), ("primaryOrder", C.c_int32),
+                ("produceDebugImage", C.c_int32)]
 
 
 class c_run_stats(C.Structure):
@@ -234,10 +235,11 @@ class Knobs:
     filteringMax: float = 900.0
     filteringRadius: float = 160.0
     primaryOrder: int = 0  # 0: g++ evaluation order of get2D (halton.h:24-27), 1: clang's
+    produceDebugImage: bool = False  # only its effect on the splat skip rule (integrator.h:121-124)
 
     def to_c(self) -> c_knobs:
         return c_knobs(int(self.onlyIndirect), int(self.maxDepth), int(self.useGeometricalOptics),
                        int(self.poDiffraction), int(self.doGuiding), int(self.clearBeforeIteration),
                        int(self.correctPhase), int(self.filterGuiding), int(self.doFiltering),
                        int(self.filteringSphere), float(self.filteringMin), float(self.filteringMax),
-                       float(self.filteringRadius), int(self.primaryOrder))
+                       float(self.filteringRadius), int(self.primaryOrder), int(self.produceDebugImage))

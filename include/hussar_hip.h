@@ -97,6 +97,10 @@ typedef struct hsr_knobs {
     float filteringMax;
     float filteringRadius;
     int32_t primaryOrder;
+    /* Integrator::produceDebugImage (integrator.h:58). The debug image itself is not
+     * produced; the flag keeps its effect on the frame: when it is 0, contributions with a
+     * zero real or imaginary part (or zero weight) are not splatted (integrator.h:121-124). */
+    int32_t produceDebugImage;
 } hsr_knobs;
 
 /* Fills the reference defaults of path.h:153-169 (maxDepth 10, guiding on, ...). */

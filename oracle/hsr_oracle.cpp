@@ -68,6 +68,7 @@ struct Knobs {
         clearBeforeIteration, correctPhase, filterGuiding, doFiltering, filteringSphere;
     float filteringMin, filteringMax, filteringRadius;
     int32_t primaryOrder;
+    int32_t produceDebugImage;
 };
 struct FrameCfg { int32_t chirpCount, samplesPerChirp, channelCount; };
 
@@ -690,16 +691,19 @@ struct Tree {
             }
         }
     }
-    // refine: split leaves whose share of the root mass exceeds the threshold, in node order
+    // refine: in node order (new nodes included), split every leaf whose share of the root
+    // mass exceeds the threshold; each child is assumed to carry half of its parent's share,
+    // so one refine subdivides as deep as the statistics warrant
     void refine(float splitThreshold) {
         const double total = mass.empty() ? 0.0 : mass[0];
-        const int32_t n = (int32_t)child.size();
         if (total > 0.0) {
-            for (int32_t i = 0; i < n; ++i) {
+            std::vector<double> share(child.size());
+            for (size_t i = 0; i < child.size(); ++i) share[i] = mass[i] / total;
+            for (int32_t i = 0; i < (int32_t)child.size(); ++i) {
                 if (child[i] != 0) continue;
                 if (depth[i] >= kMaxDepth) continue;
                 if ((int32_t)child.size() + 2 > kMaxNodes) break;
-                if (mass[i] / total > (double)splitThreshold) {
+                if (share[i] > (double)splitThreshold) {
                     const int32_t c = (int32_t)child.size();
                     child[i] = c;
                     p0[i] = 0.5f;
@@ -709,6 +713,7 @@ struct Tree {
                         stat.push_back(0.0);
                         mass.push_back(0.0);
                         depth.push_back(depth[i] + 1);
+                        share.push_back(share[i] * 0.5);
                     }
                 }
             }
@@ -967,8 +972,8 @@ template <class T, class Gd, class F, class P> struct Tracer {
                         }
                         guidingWeight = guidingWeight + v;  // path.h:392-394
                     }
-                    // Integrator::splat skip rule (integrator.h:121-124)
-                    if (sampleWeight == 0.0f || v.re == F(0) || v.im == F(0)) {
+                    // Integrator::splat skip rule (integrator.h:121-124), off with the debug image
+                    if (!knobs->produceDebugImage && (sampleWeight == 0.0f || v.re == F(0) || v.im == F(0))) {
                         if (rec && depth >= 1) rec->nee[(depth - 1) * C + ch] = 2;
                         continue;
                     }
