@@ -1,0 +1,204 @@
+"""Benchmark of the gfx950 radar integrator on its headline workload.
+
+Workload (one step = one radar frame): a 200,000-triangle vehicle-like clutter mesh (seed 1)
+10–30 m ahead, 1 Tx × 4 Rx at λ/2, 77 GHz / 60 MHz/µs / 10 MS/s / 256 samples, maximum depth 4,
+path guiding on: 4 training passes of 2^20 paths, then the measured frame of 2^24 paths.
+Inputs (mesh BVH, scene, guide) are resident on the device when timing starts.
+
+Prints one JSON line: paths/s of the whole step (all passes), frames/s, p50/p99 frame
+latency, the roofline of the dominant kernel (the final pass's path kernel: algorithmic BVH
+node + triangle bytes from deterministic traversal counters ÷ its HIP-event duration), the
+IF relative L2 of the frame against the FP64 oracle on a bounded sample with the same trained
+guide, and the oracle's FP32 restatement timed on the host cores (cpu_baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import hussar_amd as H  # noqa: E402
+from hussar_amd import scenes  # noqa: E402
+from hussar_amd._lib import OPT_COUNTERS, OPT_RECORD_HITS  # noqa: E402
+
+METRIC = "paths/s & frames/s at ≤1e-3 IF rel-L2 vs FP64; p50/p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES = 64, 48
+
+
+def workload(name):
+    rf = scenes.dihedral_rf()  # 77 GHz, 60 MHz/us, 10 MS/s, 256 samples (25.6 us)
+    if name == "vehicle":
+        mesh, _ = scenes.vehicle_mesh(seed=1, triangles=200_000)
+        scene = scenes.radar_array(rf, channels=4, position=(0.0, 0.75, 0.0))
+        knobs = H.Knobs(maxDepth=4, doGuiding=True)
+        fc = H.FrameConfig(1, 256, 4)
+        sched = (4, 1 << 20, 1 << 24)
+        desc = ("vehicle-like clutter 200k triangles (seed 1) at 10-30 m, 1 Tx x 4 Rx lambda/2, 77 GHz 60 MHz/us "
+                "10 MS/s 256 samples, depth 4, guiding 4 x 2^20 training paths + 2^24 frame paths")
+    elif name == "dihedral":
+        mesh = scenes.dihedral_mesh()
+        scene = scenes.dihedral_pose(0.0)
+        knobs = H.Knobs(maxDepth=3, doGuiding=False)
+        fc = H.FrameConfig(1, 256, 1)
+        sched = (0, 0, 1 << 20)
+        desc = "dihedral 2 x 10 cm plates at 1.5 m, 1 Tx/Rx, 256 samples, depth 3, 2^20 paths, no guiding"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return mesh, scene, knobs, fc, sched, desc
+
+
+def run_step(backend, pt, scene, sched):
+    passes, train, final = sched
+    pt.run_schedule(backend, scene, passes, train, final)
+
+
+def cpu_baseline(mesh, scene, knobs, fc, sched, seconds, threads):
+    """The oracle's FP32 restatement (reference arithmetic, own BVH, std::thread pool with
+    256-index batches) on a bounded sample of the same schedule, scaled down by 2^k."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bindings as ob
+    om = ob.OracleMesh.from_mesh(mesh)
+    passes, train, final = sched
+
+    def timed(scale):
+        tr = max(train // scale, 1024) if passes else 0
+        t0 = time.perf_counter()
+        ob.path_tracer_run_schedule(om, scene, knobs, fc, passes, tr, max(final // scale, 1024),
+                                    prec=ob.PREC_REF32, guide=ob.OracleGuide() if knobs.doGuiding else None,
+                                    threads=threads)
+        return time.perf_counter() - t0, passes * tr + max(final // scale, 1024)
+
+    scale = 256
+    dt, paths = timed(scale)
+    if dt < seconds / 2:  # grow the sample to about `seconds` of CPU work
+        scale = max(1, int(2 ** np.floor(np.log2(scale * dt / seconds))))
+        dt, paths = timed(scale)
+    return {"value": paths / dt, "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": f"same scene and schedule with path counts / {scale} ({paths} paths, {dt:.1f} s), "
+                      "oracle FP32 restatement (reference arithmetic, own SAH BVH2, std::thread pool, "
+                      "256-index batches)"}
+
+
+def accuracy_check(backend, pt, mesh, scene, knobs, fc, n, threads):
+    """Frame of n final-pass paths with the trained sampling tree, GPU vs FP64 oracle (same
+    tree, same Halton indices), plus bit-exact hit ids against the same-precision oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bindings as ob
+    om = ob.OracleMesh.from_mesh(mesh)
+    og = None
+    guide = pt.guiding if knobs.doGuiding else None
+    if guide is not None:
+        child, p0 = guide.sampling()
+        og = ob.OracleGuide()
+        og.set_sampling(child, p0)
+    backend.clear()
+    rec = min(n, 1 << 15)
+    backend.set_option(OPT_RECORD_HITS, rec)
+    backend.run_raw(scene, knobs, 0, n, guide=guide, is_final=True)
+    backend.set_option(OPT_RECORD_HITS, 0)
+    z, tw = backend.fetch("complex128")
+    hits, nee = backend.hit_record(rec, knobs.maxDepth, len(scene.receivers()))
+    zf, twf, _, _ = ob.render(om, scene, knobs, fc, prec=ob.PREC_FP64, count=n, is_final=True, guide=og,
+                              threads=threads)
+    zm, twm, recm, _ = ob.render(om, scene, knobs, fc, prec=ob.PREC_MIXED, count=rec, is_final=True, guide=og,
+                                 threads=threads, record_paths=rec)
+    zf = zf / twf
+    rl2 = float(np.linalg.norm(z - zf) / np.linalg.norm(zf))
+    peaks_gpu = np.abs(z).reshape(-1, fc.channelCount).argmax(0).tolist()
+    peaks_ref = np.abs(zf).reshape(-1, fc.channelCount).argmax(0).tolist()
+    return {"if_rel_l2_vs_fp64": rl2, "paths": n, "peak_bins_equal": peaks_gpu == peaks_ref,
+            "hit_ids_bitexact": bool(np.array_equal(hits, recm[0]) and np.array_equal(nee, recm[1])),
+            "hit_paths_compared": rec}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="vehicle")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the integrator has no multi-GPU path in this build "
+              "(paths shard by Halton index; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    threads = min(16, os.cpu_count() or 1)
+
+    mesh, scene, knobs, fc, sched, desc = workload(args.workload)
+    t0 = time.perf_counter()
+    pt = H.PathTracer(**{k: getattr(knobs, k) for k in knobs.__dataclass_fields__})
+    pt.configureFrame(fc)
+    backend = H.HipBackend(mesh, pt)
+    bvh_ms = backend.stats()["bvh_build_ms"]
+    setup_s = time.perf_counter() - t0
+
+    for _ in range(args.warmup):
+        run_step(backend, pt, scene, sched)
+    lat = []
+    kernel_ms = 0.0
+    launches = 0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        run_step(backend, pt, scene, sched)
+        lat.append(time.perf_counter() - t)
+        st = backend.stats()  # HIP events on the backend's stream, summed over the step's passes
+        kernel_ms += st["trace_ms"]
+        launches += st["launches"]
+    elapsed = time.perf_counter() - t_start
+    passes, train, final = sched
+    paths_per_step = passes * train + final
+    value = paths_per_step * args.steps / elapsed
+
+    # deterministic traversal counters of one step (separate, untimed kernel instance)
+    backend.set_option(OPT_COUNTERS, 1)
+    run_step(backend, pt, scene, sched)
+    cnt = backend.stats()
+    backend.set_option(OPT_COUNTERS, 0)
+    alg_bytes = (cnt["nodes_visited"] * NODE_BYTES + cnt["tris_tested"] * TRI_BYTES) / cnt["launches"]
+    kms = kernel_ms / launches  # average path-kernel launch
+    achieved = alg_bytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 traversal+fields / f64 geometry chain+phase+sums",
+        "data": "synthetic (seeded procedural mesh)",
+        "config": {"workload": desc, "triangles": int(mesh.triangle_count()), "paths_per_frame": paths_per_step,
+                   "frame_paths": final, "training": f"{passes} x {train}", "max_depth": knobs.maxDepth,
+                   "channels": fc.channelCount, "samples": fc.samplesPerChirp, "parallelism": "1 GPU"},
+        "frames_per_s": args.steps / elapsed,
+        "frame_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "kernel_paths_per_s": paths_per_step * args.steps / (kernel_ms * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_trace<false> (path kernel), average over the step's launches",
+                     "kernel_ms": kms, "launches_per_step": launches // args.steps,
+                     "algorithmic_bytes": alg_bytes,
+                     "nodes_visited": cnt["nodes_visited"], "tris_tested": cnt["tris_tested"],
+                     "rays_closest": cnt["rays_closest"], "rays_shadow": cnt["rays_shadow"], "events": cnt["events"]},
+        "bvh_build_ms": bvh_ms, "setup_s": setup_s,
+    }
+    if not args.no_check:
+        out["accuracy"] = accuracy_check(backend, pt, mesh, scene, knobs, fc, 1 << 18, threads)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mesh, scene, knobs, fc, sched, args.cpu_seconds, threads)
+        out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -54,7 +54,8 @@ template <class T> void dfree(T *&p) {
 }  // namespace
 
 struct hsr_guide {
-    hsr_backend *backend = nullptr;
+    hsr_backend *backend = nullptr;  // used only while the backend is alive (runs, steps)
+    int device = 0;                  // own copy: destroy must not touch the backend
     hsr::GuideHost host;
     int32_t *d_sChild = nullptr;
     float *d_sP0 = nullptr;
@@ -284,9 +285,11 @@ static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
     }
 }
 
-hsr_status hsr_hip_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
-                       int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
-                       const volatile int32_t *interruptFlag) {
+// One Backend::run; with `accumulate` the statistics add to those of the enclosing
+// schedule call instead of replacing them.
+static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
+                           int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
+                           const volatile int32_t *interruptFlag, bool accumulate) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured (hsr_hip_configure_frame)");
     if (budget < 0 || sampleIndexOffset < 0) return fail(HSR_ERR_INVALID, "negative budget or offset");
@@ -367,7 +370,7 @@ hsr_status hsr_hip_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *
     }
     if (b->optCounters) HSR_HIP(hipMemsetAsync(b->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
 
-    b->stats = hsr_run_stats{};
+    if (!accumulate) b->stats = hsr_run_stats{};
     b->stats.bvh_build_ms = b->bvh_ms;
     double trace_ms = 0.0, reduce_ms = 0.0;
     int64_t launches = 0;
@@ -422,27 +425,34 @@ hsr_status hsr_hip_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *
         HSR_HIP(hipEventElapsedTime(&ms2, b->ev[1], b->ev[3]));
         trace_ms += ms1;
         reduce_ms += ms2;
-        launches += 3;
+        launches += 1;
         done += n;
     }
     float total = 0.0f;
     if (budget > 0) HSR_HIP(hipEventElapsedTime(&total, b->ev[2], b->ev[3]));
     b->totalWeight += (double)budget * (double)sampleWeight;
-    b->stats.trace_ms = trace_ms;
-    b->stats.reduce_ms = reduce_ms;
-    b->stats.total_ms = total;
-    b->stats.paths = budget;
-    b->stats.launches = launches;
+    b->stats.trace_ms += trace_ms;
+    b->stats.reduce_ms += reduce_ms;
+    b->stats.total_ms += total;
+    b->stats.paths += budget;
+    b->stats.launches += launches;
     if (b->optCounters) {
         unsigned long long c[8];
         HSR_HIP(hipMemcpy(c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost));
-        b->stats.nodes_visited = (int64_t)c[0];
-        b->stats.tris_tested = (int64_t)c[1];
-        b->stats.rays_closest = (int64_t)c[2];
-        b->stats.rays_shadow = (int64_t)c[3];
-        b->stats.events = (int64_t)c[4];
+        b->stats.nodes_visited += (int64_t)c[0];
+        b->stats.tris_tested += (int64_t)c[1];
+        b->stats.rays_closest += (int64_t)c[2];
+        b->stats.rays_shadow += (int64_t)c[3];
+        b->stats.events += (int64_t)c[4];
     }
     return HSR_OK;
+}
+
+hsr_status hsr_hip_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
+                       int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
+                       const volatile int32_t *interruptFlag) {
+    return run_impl(b, scene, knobs, sampleIndexOffset, budget, sampleWeight, isFinalIteration, guide,
+                    interruptFlag, false);
 }
 
 hsr_status hsr_hip_fetch_frame(hsr_backend *b, float *out_c64, double *out_c128, double *totalWeight) {
@@ -504,6 +514,7 @@ hsr_status hsr_guide_create(hsr_backend *b, float uniformProb, float splitThresh
     HSR_HIP(hipSetDevice(b->device));
     std::unique_ptr<hsr_guide> g(new hsr_guide());
     g->backend = b;
+    g->device = b->device;
     g->host.uniformProb = uniformProb;
     g->host.splitThreshold = splitThreshold;
     g->host.secondMoment = secondMoment != 0;
@@ -520,7 +531,7 @@ hsr_status hsr_guide_create(hsr_backend *b, float uniformProb, float splitThresh
 
 void hsr_guide_destroy(hsr_guide *g) {
     if (!g) return;
-    (void)hipSetDevice(g->backend->device);
+    (void)hipSetDevice(g->device);
     dfree(g->d_sChild);
     dfree(g->d_sP0);
     dfree(g->d_tChild);
@@ -603,16 +614,17 @@ hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     hsr_status s = hsr_hip_clear_frame(b);  // setup() + clearFrame(), path.h:173-174
     if (s != HSR_OK) return s;
+    b->stats = hsr_run_stats{};
     int64_t offset = 0;
     float weight = 1.0f;
-    if (!knobs->doGuiding) return hsr_hip_run(b, scene, knobs, 0, samples, 1.0f, 0, nullptr, interruptFlag);
+    if (!knobs->doGuiding) return run_impl(b, scene, knobs, 0, samples, 1.0f, 0, nullptr, interruptFlag, true);
     if (!guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");
     if ((s = hsr_guide_reset(guide)) != HSR_OK) return s;
     bool isFinal = false;
     int64_t milestone = 16384, remaining = samples;
     while (true) {
         milestone = std::min(milestone, remaining);
-        s = hsr_hip_run(b, scene, knobs, offset, milestone, weight, isFinal, guide, interruptFlag);
+        s = run_impl(b, scene, knobs, offset, milestone, weight, isFinal, guide, interruptFlag, true);
         if (s != HSR_OK) return s;
         offset += milestone;
         remaining -= milestone;
@@ -641,16 +653,18 @@ hsr_status hsr_path_tracer_run_schedule(hsr_backend *b, const hsr_scene *scene, 
     hsr_status s;
     const bool guiding = knobs->doGuiding && guide;
     if (knobs->doGuiding && !guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");
+    b->stats = hsr_run_stats{};
     if (guiding) {
         if ((s = hsr_guide_reset(guide)) != HSR_OK) return s;
         for (int p = 0; p < trainingPasses; ++p) {
             if ((s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
-            if ((s = hsr_hip_run(b, scene, knobs, 0, trainingPaths, 1.0f, 0, guide, interruptFlag)) != HSR_OK) return s;
+            if ((s = run_impl(b, scene, knobs, 0, trainingPaths, 1.0f, 0, guide, interruptFlag, true)) != HSR_OK)
+                return s;
             if ((s = hsr_guide_step(guide)) != HSR_OK) return s;
         }
     }
     if ((s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
-    return hsr_hip_run(b, scene, knobs, 0, finalPaths, 1.0f, 1, guiding ? guide : nullptr, interruptFlag);
+    return run_impl(b, scene, knobs, 0, finalPaths, 1.0f, 1, guiding ? guide : nullptr, interruptFlag, true);
 }
 
 }  // extern "C"

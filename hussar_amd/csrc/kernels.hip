@@ -700,7 +700,10 @@ int trace_blocks_per_cu(int N, int C, bool count) {
     return e == hipSuccess ? n : 1;
 }
 
+// Each launcher first clears a stale error of an earlier, already reported runtime call, so
+// that hipGetLastError() after the launch reports this launch only.
 hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t stream) {
+    (void)hipGetLastError();
     const size_t lds = trace_lds_bytes(args.N, args.C);
     if (count)
         hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(kTraceBlock), lds, stream, args);
@@ -711,6 +714,7 @@ hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t
 
 hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum, double *stage_comp, int S,
                          double *frame_sum, double *frame_comp, hipStream_t stream) {
+    (void)hipGetLastError();
     const int bs = 256;
     hipLaunchKernelGGL(k_reduce_stage, dim3((M + bs - 1) / bs, S), dim3(bs), 0, stream, partials, W, M, S,
                        stage_sum, stage_comp);
@@ -721,6 +725,7 @@ hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum
 
 hipError_t launch_zero(double *p, int64_t n, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
+    (void)hipGetLastError();
     const int bs = 256;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, stream, p, n);
     return hipGetLastError();

@@ -168,6 +168,7 @@ hsr_status hsr_path_tracer_run_schedule(hsr_backend *backend, const hsr_scene *s
  * The guide lives on the backend's device. */
 hsr_status hsr_guide_create(hsr_backend *backend, float uniformProb, float splitThreshold,
                             int32_t secondMoment, hsr_guide **out);
+/* A guide is used only while its backend lives; destroying it after the backend is allowed. */
 void hsr_guide_destroy(hsr_guide *guide);
 /* GuidingWrapper::reset (path.h:48-51): both trees back to a single uniform leaf. */
 hsr_status hsr_guide_reset(hsr_guide *guide);
@@ -187,13 +188,15 @@ hsr_status hsr_guide_training_size(hsr_guide *guide, int32_t *nodeCount);
 hsr_status hsr_guide_get_training(hsr_guide *guide, int32_t *child, double *stat,
                                   int32_t capacity);
 
-/* Per-run statistics, measured with HIP events on the stream the kernels run on. */
+/* Statistics of the last top-level call (hsr_hip_run, or all the runs of one
+ * hsr_path_tracer_run / hsr_path_tracer_run_schedule), measured with HIP events on the
+ * stream the kernels run on. */
 typedef struct hsr_run_stats {
-    double trace_ms;        /* device time of the path-tracing launches of the last run */
+    double trace_ms;        /* device time of the path-kernel launches */
     double reduce_ms;       /* device time of the frame reduction */
     double total_ms;        /* device time of the whole last run (first to last launch) */
     int64_t paths;          /* paths traced by the last run */
-    int64_t launches;       /* kernel launches of the last run */
+    int64_t launches;       /* path-kernel launches (each followed by two reduction kernels) */
     int64_t nodes_visited;  /* BVH nodes fetched (only when counters are enabled) */
     int64_t tris_tested;    /* triangles tested (only when counters are enabled) */
     int64_t rays_closest;   /* closest-hit rays traced (counters enabled) */
