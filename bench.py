@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 import hussar_amd as H  # noqa: E402
 from hussar_amd import scenes  # noqa: E402
-from hussar_amd._lib import OPT_COUNTERS, OPT_RECORD_HITS  # noqa: E402
+from hussar_amd._lib import OPT_COUNTERS, OPT_PIPELINE, OPT_RECORD_HITS  # noqa: E402
 
 METRIC = "paths/s & frames/s at ≤1e-3 IF rel-L2 vs FP64; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -126,6 +126,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--pipeline", type=int, default=1, help="0 fused path kernel, 1 wavefront stages")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the integrator has no multi-GPU path in this build "
@@ -138,6 +139,7 @@ def main():
     pt = H.PathTracer(**{k: getattr(knobs, k) for k in knobs.__dataclass_fields__})
     pt.configureFrame(fc)
     backend = H.HipBackend(mesh, pt)
+    backend.set_option(OPT_PIPELINE, args.pipeline)
     bvh_ms = backend.stats()["bvh_build_ms"]
     setup_s = time.perf_counter() - t0
 
@@ -182,6 +184,7 @@ def main():
                    "channels": fc.channelCount, "samples": fc.samplesPerChirp, "parallelism": "1 GPU"},
         "frames_per_s": args.steps / elapsed,
         "frame_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "pipeline": ["fused path kernel", "wavefront"][args.pipeline],
         "kernel_paths_per_s": paths_per_step * args.steps / (kernel_ms * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -96,7 +96,12 @@ struct hsr_backend {
     int64_t optRecord = 0;
     int64_t optPathsPerLaunch = int64_t(1) << 24;
     int32_t optPhaseMode = 0;
+    int32_t optPipeline = 0;
     int32_t maxGrid = 0;
+    // wavefront buffers
+    void *d_wf = nullptr;
+    size_t wfBytes = 0;
+    int64_t wfCap = int64_t(1) << 22;  // paths per wavefront batch
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hsr_run_stats stats{};
@@ -198,6 +203,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_counters);
     dfree(b->d_recHits);
     dfree(b->d_recNee);
+    if (b->d_wf) (void)hipFree(b->d_wf);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -251,7 +257,8 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
             b->optPathsPerLaunch = value;
             return HSR_OK;
         case HSR_OPT_PIPELINE:
-            if (value != 0) return fail(HSR_ERR_UNSUPPORTED, "only the fused path kernel is available");
+            if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "pipeline must be 0 (fused) or 1 (wavefront)");
+            b->optPipeline = (int32_t)value;
             return HSR_OK;
         case HSR_OPT_PHASE_MODE:
             if (value < 0 || value > 1) return fail(HSR_ERR_UNSUPPORTED, "phase mode must be 0 (FP64) or 1 (FP32)");
@@ -375,7 +382,69 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     double trace_ms = 0.0, reduce_ms = 0.0;
     int64_t launches = 0;
     bool first = true;
-    for (int64_t done = 0; done < budget;) {
+    for (int64_t done = 0; done < budget && b->optPipeline == 1;) {
+        // wavefront batches: path state in HBM, all stage sizes on the device
+        if (interruptFlag && *interruptFlag) return fail(HSR_ERR_INTERRUPTED, "interrupted");
+        const int64_t n = std::min({budget - done, b->optPathsPerLaunch, b->wfCap});
+        const int64_t P = std::min(budget, std::min(b->optPathsPerLaunch, b->wfCap));
+        const size_t need = hsr::wavefront_bytes(P, C, knobs->maxDepth);
+        if (need > b->wfBytes) {
+            if (b->d_wf) (void)hipFree(b->d_wf);
+            b->d_wf = nullptr;
+            b->wfBytes = 0;
+            HSR_HIP(hipMalloc(&b->d_wf, need));
+            b->wfBytes = need;
+        }
+        hsr::WfArgs w;
+        std::memset(&w, 0, sizeof(w));
+        hsr::wavefront_layout(w, b->d_wf, P, C, knobs->maxDepth);
+        w.base = done;
+        w.n = (int32_t)n;
+        a.offset = sampleIndexOffset;
+        if (recPaths > 0) {
+            a.recHits = b->d_recHits;
+            a.recNee = b->d_recNee;
+            a.recPaths = recPaths;
+        }
+        const int gridTrav = b->cu * hsr::wavefront_trav_blocks_per_cu(b->optCounters);
+        const int gridWork = b->cu * 4;
+        const int gridGather = b->cu * 2;
+        const size_t needP = (size_t)gridGather * M;
+        if (needP > b->partialCap) {
+            dfree(b->d_partials);
+            hsr_status s;
+            if ((s = dalloc(&b->d_partials, needP)) != HSR_OK) return s;
+            b->partialCap = needP;
+        }
+        const int S = std::min(64, gridGather);
+        if ((size_t)S * M > b->stageCap) {
+            dfree(b->d_stage_sum);
+            dfree(b->d_stage_comp);
+            hsr_status s;
+            if ((s = dalloc(&b->d_stage_sum, (size_t)S * M)) != HSR_OK) return s;
+            if ((s = dalloc(&b->d_stage_comp, (size_t)S * M)) != HSR_OK) return s;
+            b->stageCap = (size_t)S * M;
+        }
+        HSR_HIP(hsr::launch_zero(b->d_partials, (int64_t)gridGather * M, b->stream));
+        HSR_HIP(hipEventRecord(b->ev[0], b->stream));
+        if (first) HSR_HIP(hipEventRecord(b->ev[2], b->stream));
+        first = false;
+        HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
+                                      b->stream));
+        HSR_HIP(hipEventRecord(b->ev[1], b->stream));
+        HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, M, b->d_stage_sum, b->d_stage_comp, S, b->d_frame_sum,
+                                   b->d_frame_comp, b->stream));
+        HSR_HIP(hipEventRecord(b->ev[3], b->stream));
+        HSR_HIP(hipEventSynchronize(b->ev[3]));
+        float ms1 = 0.0f, ms2 = 0.0f;
+        HSR_HIP(hipEventElapsedTime(&ms1, b->ev[0], b->ev[1]));
+        HSR_HIP(hipEventElapsedTime(&ms2, b->ev[1], b->ev[3]));
+        trace_ms += ms1;
+        reduce_ms += ms2;
+        launches += 1;
+        done += n;
+    }
+    for (int64_t done = 0; done < budget && b->optPipeline == 0;) {
         if (interruptFlag && *interruptFlag) return fail(HSR_ERR_INTERRUPTED, "interrupted");
         const int64_t n = std::min(budget - done, b->optPathsPerLaunch);
         int grid = (int)std::min<int64_t>(b->maxGrid, (n + 8 * hsr::kTraceBlock - 1) / (8 * hsr::kTraceBlock));

@@ -56,6 +56,47 @@ struct TraceArgs {
 };
 
 hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t stream);
+
+// ---------------------------------------------------------------- wavefront pipeline
+// Per-depth queue counters (device memory, zeroed per batch).
+struct WfCounts {
+    unsigned int n_ext, head_ext, n_shd, head_shd, n_gat, pad0, pad1, pad2;
+};
+
+// Structure-of-arrays path state and queues for a batch of up to P paths.
+//  sd: 15 double fields x P  (o.xyz, d.xyz, n_in.xyz, d_in.xyz, time, r, cosTheta)
+//  sf: 18 float fields x P   (H 6, H_in 6, guidingWeight 2, x0, x1, pdf, freq)
+//  si: 2 int fields x P      (depth, flags: bit 0 = the path ends after its pending NEE)
+//  shadow slots [P * C]: ray (o.xyz, tfar | d.xyz), NEE payload (tn, v, plane-filter guiding
+//  term), status bits (1 set-up, 2 needs a visibility query, 4 add v to the guiding weight,
+//  8 splat, 16 add the plane-filter term) and the visibility (1 occluded, 2 visible).
+struct WfArgs {
+    int64_t P;
+    int64_t base;   // index (relative to the run's first path) of slot 0
+    int32_t n;      // paths in this batch
+    double *sd;
+    float *sf;
+    int32_t *si;
+    float4 *ray_o, *ray_d;
+    int2 *hit;
+    float4 *s_o, *s_d;
+    double *s_tn;
+    float2 *s_v, *s_g1;
+    int8_t *s_st, *s_vis;
+    int32_t *q_ext0, *q_ext1, *q_shd, *q_gat;
+    WfCounts *cnt;  // [maxDepth + 2]
+};
+
+// Bytes of device memory the wavefront buffers need for P paths and C channels.
+size_t wavefront_bytes(int64_t P, int C, int maxDepth);
+// Carves the buffers out of one allocation.
+void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth);
+// One batch: generation, then per depth shadow / gather / extend / shade, with all queue
+// sizes on the device (no host synchronisation). partials: [gatherGrid][N*C], accumulated.
+hipError_t launch_wavefront(const TraceArgs &args, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
+                            double2 *partials, bool count, hipStream_t stream);
+int wavefront_trav_blocks_per_cu(bool count);
+size_t gather_lds_bytes(int N, int C);
 // partials [W][M] doubles -> stage [S][M] (sum, comp) -> frame (sum, comp) += Kahan
 hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum, double *stage_comp, int S,
                          double *frame_sum, double *frame_comp, hipStream_t stream);

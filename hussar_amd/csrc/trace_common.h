@@ -1,0 +1,650 @@
+// Device code shared by the fused path kernel (kernels.hip) and the wavefront pipeline
+// (wavefront.hip): constants, BVH traversal with the LDS-staged short stack, the guiding tree,
+// path generation, NEE set-up, the hit update, and the splat helpers.
+//
+// Precision model: FP32 traversal, triangle tests, fields and weights; FP64 geometry chain
+// (directions, hit points, segment lengths re-evaluated against the hit triangle's plane),
+// FP64 path time, IF phase and frame sums. Compiled without FP contraction.
+#pragma once
+
+#include "device_math.h"
+#include "kernels.h"
+
+namespace hsr {
+
+static constexpr double kC = 299792448.0;  // radar.h:107, 299792458f after float rounding
+static constexpr float kPiF = 3.14159265358979323846f;
+static constexpr float kInv4PiF = 0.07957747154594766788f;
+static constexpr double kPiD = 3.14159265358979323846;
+static constexpr float kEps = 5e-5f;  // hussar.h:52
+static constexpr float kOneMinusEps = 0x1.fffffep-1f;
+
+__device__ inline mat3<float> rotf(const DevAntenna &a) {
+    mat3<float> m;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m.m[i] = a.rot[i];
+    return m;
+}
+__device__ inline mat3<double> rotd(const DevAntenna &a) {
+    mat3<double> m;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m.m[i] = (double)a.rot[i];
+    return m;
+}
+__device__ inline d3 posd(const DevAntenna &a) { return {(double)a.pos[0], (double)a.pos[1], (double)a.pos[2]}; }
+
+// ------------------------------------------------------------------------ traversal stack
+struct Stack {
+    int32_t *lds;  // column of this thread: lds[i * kTraceBlock]
+    int sp;
+    int32_t spill[kStackSpill];
+    __device__ inline void push(int32_t v) {
+        if (sp < kStackShort) lds[sp * kTraceBlock] = v;
+        else spill[sp - kStackShort] = v;
+        ++sp;
+    }
+    __device__ inline int32_t pop() {
+        --sp;
+        return sp < kStackShort ? lds[sp * kTraceBlock] : spill[sp - kStackShort];
+    }
+};
+
+// BVH2 node (64 B): c0.x lo/hi, c0.y lo/hi | c1.x lo/hi, c1.y lo/hi | c0.z lo/hi, c1.z lo/hi |
+// child0, child1. A child >= 0 is an inner node; < 0 encodes a leaf: bit 31 set, bits 27-30
+// count-1, bits 0-26 first triangle slot.
+__device__ inline void leaf_decode(int32_t c, int &first, int &count) {
+    first = c & 0x07FFFFFF;
+    count = ((c >> 27) & 0xF) + 1;
+}
+
+struct SlabRay {
+    f3 o, invd;
+    float tnear;
+};
+
+__device__ inline bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const SlabRay &r,
+                            float tbest, float &tentry) {
+    const float ax = (lx - r.o.x) * r.invd.x, bx = (hx - r.o.x) * r.invd.x;
+    const float ay = (ly - r.o.y) * r.invd.y, by = (hy - r.o.y) * r.invd.y;
+    const float az = (lz - r.o.z) * r.invd.z, bz = (hz - r.o.z) * r.invd.z;
+    const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tnear));
+    float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    t1 = t1 * 1.0000004f;  // 1 + 2*gamma(3): robust traversal (Ize 2013)
+    tentry = t0;
+    return t0 <= t1 && t0 <= tbest;
+}
+
+template <bool kCount>
+__device__ inline int32_t closest_hit(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
+                                      f3 d, Stack &st, float &tout, int32_t &slot, uint32_t &cn, uint32_t &ct) {
+    SlabRay r{o, {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}, kEps};
+    float best = __builtin_inff();
+    int32_t best_id = -1, best_slot = -1;
+    st.sp = 0;
+    int32_t node = 0;
+    while (true) {
+        while (node >= 0) {
+            if (kCount) ++cn;
+            const float4 a = nodes[4 * node + 0];
+            const float4 b = nodes[4 * node + 1];
+            const float4 c = nodes[4 * node + 2];
+            const int4 ch = __builtin_bit_cast(int4, nodes[4 * node + 3]);
+            float te0, te1;
+            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, r, best, te0);
+            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, r, best, te1);
+            if (h0 && h1) {
+                const bool first0 = te0 <= te1;
+                st.push(first0 ? ch.y : ch.x);
+                node = first0 ? ch.x : ch.y;
+            } else if (h0) {
+                node = ch.x;
+            } else if (h1) {
+                node = ch.y;
+            } else {
+                if (st.sp == 0) goto done;
+                node = st.pop();
+            }
+        }
+        {
+            int first, count;
+            leaf_decode(node, first, count);
+            for (int i = first; i < first + count; ++i) {
+                if (kCount) ++ct;
+                const float4 p = tris[3 * i + 0];
+                const float4 e1 = tris[3 * i + 1];
+                const float4 e2 = tris[3 * i + 2];
+                const float t = tri_intersect({p.x, p.y, p.z}, {e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, o, d, kEps,
+                                              __builtin_inff());
+                const int32_t id = __float_as_int(p.w);
+                if (t < __builtin_inff() && (best_id < 0 || t < best || (t == best && id < best_id))) {
+                    best = t;
+                    best_id = id;
+                    best_slot = i;
+                }
+            }
+        }
+        if (st.sp == 0) break;
+        node = st.pop();
+    }
+done:
+    tout = best;
+    slot = best_slot;
+    return best_id;
+}
+
+template <bool kCount>
+__device__ inline bool occluded(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o, f3 d,
+                                float tfar, Stack &st, uint32_t &cn, uint32_t &ct) {
+    SlabRay r{o, {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}, kEps};
+    st.sp = 0;
+    int32_t node = 0;
+    while (true) {
+        while (node >= 0) {
+            if (kCount) ++cn;
+            const float4 a = nodes[4 * node + 0];
+            const float4 b = nodes[4 * node + 1];
+            const float4 c = nodes[4 * node + 2];
+            const int4 ch = __builtin_bit_cast(int4, nodes[4 * node + 3]);
+            float te0, te1;
+            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, r, tfar, te0);
+            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, r, tfar, te1);
+            if (h0 && h1) {
+                const bool first0 = te0 <= te1;
+                st.push(first0 ? ch.y : ch.x);
+                node = first0 ? ch.x : ch.y;
+            } else if (h0) {
+                node = ch.x;
+            } else if (h1) {
+                node = ch.y;
+            } else {
+                if (st.sp == 0) return false;
+                node = st.pop();
+            }
+        }
+        int first, count;
+        leaf_decode(node, first, count);
+        for (int i = first; i < first + count; ++i) {
+            if (kCount) ++ct;
+            const float4 p = tris[3 * i + 0];
+            const float4 e1 = tris[3 * i + 1];
+            const float4 e2 = tris[3 * i + 2];
+            if (tri_intersect({p.x, p.y, p.z}, {e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, o, d, kEps, tfar) <
+                __builtin_inff())
+                return true;
+        }
+        if (st.sp == 0) return false;
+        node = st.pop();
+    }
+}
+
+// ------------------------------------------------------------------------ guiding tree
+// Node 0 covers [0,1)^2; an inner node halves its cell along axis depth%2; children are
+// stored consecutively (child, child+1), child 0 the lower half.
+__device__ inline float tree_pdf(const int32_t *__restrict__ child, const float *__restrict__ p0, float x0,
+                                 float x1) {
+    float pdf = 1.0f, lo0 = 0.0f, lo1 = 0.0f, sz0 = 1.0f, sz1 = 1.0f;
+    int32_t node = 0;
+    int dep = 0;
+    int32_t c;
+    while ((c = child[node]) != 0) {
+        const float p = p0[node];
+        if ((dep & 1) == 0) {
+            const float mid = lo0 + sz0 * 0.5f;
+            if (x0 < mid) { pdf = pdf * (2.0f * p); node = c; }
+            else { pdf = pdf * (2.0f * (1.0f - p)); lo0 = mid; node = c + 1; }
+            sz0 = sz0 * 0.5f;
+        } else {
+            const float mid = lo1 + sz1 * 0.5f;
+            if (x1 < mid) { pdf = pdf * (2.0f * p); node = c; }
+            else { pdf = pdf * (2.0f * (1.0f - p)); lo1 = mid; node = c + 1; }
+            sz1 = sz1 * 0.5f;
+        }
+        ++dep;
+    }
+    return pdf;
+}
+
+__device__ inline float tree_sample(const int32_t *__restrict__ child, const float *__restrict__ p0, float &x0,
+                                    float &x1) {
+    float pdf = 1.0f, lo0 = 0.0f, lo1 = 0.0f, sz0 = 1.0f, sz1 = 1.0f;
+    int32_t node = 0;
+    int dep = 0;
+    int32_t c;
+    while ((c = child[node]) != 0) {
+        const float p = p0[node];
+        if ((dep & 1) == 0) {
+            if (x0 < p) { x0 = fminf(x0 / p, kOneMinusEps); pdf = pdf * (2.0f * p); node = c; }
+            else {
+                x0 = fminf((x0 - p) / (1.0f - p), kOneMinusEps);
+                pdf = pdf * (2.0f * (1.0f - p));
+                lo0 = lo0 + sz0 * 0.5f;
+                node = c + 1;
+            }
+            sz0 = sz0 * 0.5f;
+        } else {
+            if (x1 < p) { x1 = fminf(x1 / p, kOneMinusEps); pdf = pdf * (2.0f * p); node = c; }
+            else {
+                x1 = fminf((x1 - p) / (1.0f - p), kOneMinusEps);
+                pdf = pdf * (2.0f * (1.0f - p));
+                lo1 = lo1 + sz1 * 0.5f;
+                node = c + 1;
+            }
+            sz1 = sz1 * 0.5f;
+        }
+        ++dep;
+    }
+    x0 = lo0 + x0 * sz0;
+    x1 = lo1 + x1 * sz1;
+    return pdf;
+}
+
+__device__ inline int32_t tree_leaf(const int32_t *__restrict__ child, float x0, float x1) {
+    float lo0 = 0.0f, lo1 = 0.0f, sz0 = 1.0f, sz1 = 1.0f;
+    int32_t node = 0;
+    int dep = 0;
+    int32_t c;
+    while ((c = child[node]) != 0) {
+        if ((dep & 1) == 0) {
+            const float mid = lo0 + sz0 * 0.5f;
+            if (x0 < mid) node = c;
+            else { lo0 = mid; node = c + 1; }
+            sz0 = sz0 * 0.5f;
+        } else {
+            const float mid = lo1 + sz1 * 0.5f;
+            if (x1 < mid) node = c;
+            else { lo1 = mid; node = c + 1; }
+            sz1 = sz1 * 0.5f;
+        }
+        ++dep;
+    }
+    return node;
+}
+
+// GuidingWrapper::sample (path.h:53-82)
+__device__ inline float guide_sample(const TraceArgs &a, float &x0, float &x1) {
+    const float u = a.uniformProb;
+    if (u == 1.0f) return 1.0f;
+    float pdf = 1.0f - u;
+    if (x0 < u) {
+        x0 /= u;
+        pdf *= tree_pdf(a.gsChild, a.gsP0, x0, x1);
+    } else {
+        x0 -= u;
+        x0 /= 1.0f - u;
+        const float gpdf = tree_sample(a.gsChild, a.gsP0, x0, x1);
+        pdf *= gpdf;
+    }
+    pdf += u;
+    return pdf;
+}
+
+// ------------------------------------------------------------------------ path state
+struct PathState {
+    d3 o, d;        // FP64 chain ray
+    d3 n_in, d_in;  // normal and direction at the last hit (surface.incoming)
+    cf3 H, H_in;    // field of the ray and at the last hit
+    double time;    // path time [s]
+    double r;       // path length [m]
+    double cosTheta;
+    cf gw;          // guidingWeight
+    float x0, x1;   // primary sample after the guiding warp
+    float pdf;      // primaryPdf
+    float freq;
+    int depth;
+    int64_t local;  // index - offset
+};
+
+__device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p) {
+    const float u_f = radical_inverse_2(index);
+    const float u_a = radical_inverse_b<3>(index);
+    const float u_b = radical_inverse_b<5>(index);
+    float x0 = a.primaryOrder == 0 ? u_b : u_a;
+    float x1 = a.primaryOrder == 0 ? u_a : u_b;
+    float pdf = 1.0f;
+    if (a.doGuiding && a.guideOn) pdf = guide_sample(a, x0, x1);
+    const d3 dl = uniform_sample_sphere((double)x0, (double)x1);
+    cf3 Hl = awr_evaluate(cast3<float>(dl));
+    Hl = Hl / kInv4PiF;
+    p.o = posd(a.tx);
+    p.d = mulv(rotd(a.tx), dl);
+    p.H = (1.0f / pdf) * mulv(rotf(a.tx), Hl);
+    p.freq = a.startFreq + u_f * (a.freqSlope * a.rampTime);
+    p.time = 0.0;
+    p.r = 0.0;
+    p.cosTheta = 1.0;
+    p.gw = {0.0f, 0.0f};
+    p.x0 = x0;
+    p.x1 = x1;
+    p.pdf = pdf;
+    p.depth = 0;
+}
+
+// NEE set-up for receiver ch at the path's current vertex: everything of path.h:296-396
+// except the visibility query (rx.nee, the PO phasor of SurfaceEmitter::evaluate/connect or
+// the direct Tx evaluate/connect at depth 0, measureH, the GO density correction and the
+// filtering). Pure: the caller applies the guiding-weight updates only for visible rays.
+struct NeeSetup {
+    f3 o, d;       // shadow ray: origin and direction rounded from the FP64 chain
+    float tfar;    // distance to the receiver
+    double tn;     // path time at the receiver
+    cf v;          // measurement
+    cf gw1;        // plane-filter guiding term (filterGuiding, path.h:376-380), else 0
+    bool gwAdd;    // guidingWeight += v when visible (depth > 0, path.h:392-394)
+    bool splat;    // the splat skip rule lets v through (integrator.h:121-124)
+};
+
+__device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int ch) {
+    const float fourPi = 4.0f * kPiF;
+    NeeSetup ns;
+    const DevAntenna &rxa = a.rx[ch];
+    d3 dn = posd(rxa) - p.o;
+    const double rr = norm(dn);
+    dn = dn / rr;
+    ns.o = cast3<float>(p.o);
+    ns.d = cast3<float>(dn);
+    ns.tfar = (float)rr;
+    ns.gw1 = cf{0.0f, 0.0f};
+    ns.gwAdd = false;
+    const cf3 Hrx = mulv(rotf(rxa), awr_evaluate(cast3<float>(mulTv(rotd(rxa), -dn))));
+    const float rrF = (float)rr;
+    cf3 Hn;
+    if (p.depth == 0) {
+        // tx.evaluate + tx.connect (emitter.h:182-193)
+        Hn = mulv(rotf(a.tx), awr_evaluate(cast3<float>(mulTv(rotd(a.tx), dn))));
+        Hn = (1.0f / (fourPi * rrF)) * Hn;
+    } else {
+        // SurfaceEmitter::evaluate (emitter.h:64-70) and ::connect (emitter.h:72-81)
+        const cf3 J = 2.0f * cross(cast3<float>(p.n_in), p.H_in);
+        Hn = cross(cast3<float>(dn), J);
+        if (dot(dn, p.n_in) < 0.0) Hn = cf3{};
+        const float k0 = ((2.0f * kPiF) * p.freq) / 299792448.0f;
+        const cf green = cf{1.0f / fmaxf(rrF, 1e-3f), k0} / (fourPi * rrF);
+        Hn = Hn * green;
+    }
+    ns.tn = p.time + rr / kC;
+    cf v = cdot(Hn, Hrx);  // measureH (geometry.h:138)
+    if (p.depth > 0) {
+        // GO density correction (path.h:330-341)
+        v = v * (float)(p.r / p.cosTheta);
+        v = v / fourPi;
+        if (p.cosTheta < 1e-3) {
+            ns.v = v;
+            ns.splat = false;
+            return ns;
+        }
+        double dphase;
+        if (a.filteringSphere) {  // path.h:343-349
+            const d3 rxPos = p.o + rr * dn;
+            const double lam = fmax(0.0, dot(p.d, rxPos - p.o));
+            dphase = norm((p.o + lam * p.d) - rxPos);
+        } else {  // path.h:350-353
+            const double c = dot(normalized(dn - p.d_in), p.n_in);
+            dphase = p.r * sqrt(1.0 - c * c) / c;
+        }
+        dphase = dphase / (kC / (double)p.freq);  // path.h:355
+        if (a.doFiltering) {                      // path.h:357-387
+            if (a.filteringSphere) {
+                const double R = (double)a.filteringRadius;
+                if (dphase > R) {
+                    const double f = 1.0 - 0.20 * (dphase / R - 1.0);
+                    v = v * (float)fmax(f, 0.0);
+                }
+            } else {
+                const double fmin_ = a.filteringMin, fmax_ = a.filteringMax;
+                if (dphase > fmax_) {
+                    if (a.filterGuiding) {
+                        const double q = dphase / fmax_;
+                        ns.gw1 = v / (float)(q * q + 1.0);
+                    }
+                    v = cf{0.0f, 0.0f};
+                }
+                if (dphase > fmin_) v = v * (float)((fmax_ - dphase) / (fmax_ - fmin_));
+            }
+        }
+        ns.gwAdd = true;
+    }
+    ns.v = v;
+    // splat skip rule (integrator.h:121-124); with the debug image on, every contribution is
+    // splatted (an exactly zero one adds nothing and is skipped)
+    ns.splat = a.produceDebugImage ? !(a.sampleWeight == 0.0f || (v.re == 0.0f && v.im == 0.0f))
+                                   : !(a.sampleWeight == 0.0f || v.re == 0.0f || v.im == 0.0f);
+    return ns;
+}
+
+// Hit update and specular reflection (path.h:413-442, integrator.h:164-168,
+// intersection.h:38): geometric normal (kernel.cu:86) face-forwarded (cpu.cpp:104-107), and
+// the segment length re-evaluated in double against the hit triangle's plane, both from the
+// float edges promoted to double. Returns false when the hit is grazing (path ends).
+__device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot) {
+    const float4 P = a.tris[3 * slot + 0];
+    const float4 E1 = a.tris[3 * slot + 1];
+    const float4 E2 = a.tris[3 * slot + 2];
+    const d3 e1 = {(double)E1.x, (double)E1.y, (double)E1.z};
+    const d3 e2 = {(double)E2.x, (double)E2.y, (double)E2.z};
+    const d3 v0 = {(double)P.x, (double)P.y, (double)P.z};
+    const d3 nc = cross(e1, e2);
+    d3 n = normalized(nc);
+    if (dot(n, p.d) > 0.0) n = -n;
+    const double cosT = fabs(dot(n, -p.d));
+    if (cosT < 1e-3) return false;
+    const double tg = dot(nc, v0 - p.o) / dot(nc, p.d);
+    p.cosTheta = cosT;
+    p.r = p.r + tg;
+    p.time = p.time + tg / kC;
+    p.n_in = n;
+    p.H_in = p.H;
+    p.d_in = p.d;
+    p.o = p.o + tg * p.d;
+    p.depth++;
+    const d3 wi = -p.d;
+    p.d = (2.0 * dot(n, wi)) * n - wi;
+    const f3 nF = cast3<float>(n);
+    p.H = rscale(2.0f * nF, rdot(nF, p.H)) - p.H;
+    return true;
+}
+
+// Termination test before the extension ray (path.h:400-404).
+__device__ inline bool path_ends(const TraceArgs &a, const PathState &p) {
+    const double maxDist = ((double)a.adcRate / (double)a.freqSlope) * kC;
+    return p.depth >= a.maxDepth || p.r >= maxDist || is_zero(p.H);
+}
+
+// Interruptible traversal: a lane advances its ray by up to `steps` node visits per call, so
+// that a persistent kernel can hand a finished lane the next ray while the others go on.
+constexpr int32_t kDone = -1;  // never a valid leaf reference (a leaf holds at most 8 triangles)
+
+struct Trav {
+    f3 o, d, invd;
+    float tfar;      // any-hit: the ray's extent; closest: the best hit so far (+inf)
+    int32_t node;    // next node reference, kDone when finished
+    int32_t best_id, best_slot;
+};
+
+__device__ inline void trav_begin(Trav &t, f3 o, f3 d, float tfar, Stack &st) {
+    t.o = o;
+    t.d = d;
+    t.invd = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    t.tfar = tfar;
+    t.node = 0;
+    t.best_id = -1;
+    t.best_slot = -1;
+    st.sp = 0;
+}
+
+template <bool kAny, bool kCount>
+__device__ inline void trav_run(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, Trav &t,
+                                Stack &st, int steps, uint32_t &cn, uint32_t &ct) {
+    const SlabRay r{t.o, t.invd, kEps};
+    for (int s = 0; s < steps && t.node != kDone; ++s) {
+        if (t.node >= 0) {
+            if (kCount) ++cn;
+            const float4 a = nodes[4 * t.node + 0];
+            const float4 b = nodes[4 * t.node + 1];
+            const float4 c = nodes[4 * t.node + 2];
+            const int4 ch = __builtin_bit_cast(int4, nodes[4 * t.node + 3]);
+            const float tb = t.tfar;
+            float te0, te1;
+            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, r, tb, te0);
+            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, r, tb, te1);
+            if (h0 && h1) {
+                const bool first0 = te0 <= te1;
+                st.push(first0 ? ch.y : ch.x);
+                t.node = first0 ? ch.x : ch.y;
+            } else if (h0) {
+                t.node = ch.x;
+            } else if (h1) {
+                t.node = ch.y;
+            } else {
+                t.node = st.sp ? st.pop() : kDone;
+            }
+        } else {
+            int first, count;
+            leaf_decode(t.node, first, count);
+            bool hit = false;
+            for (int i = first; i < first + count; ++i) {
+                if (kCount) ++ct;
+                const float4 p = tris[3 * i + 0];
+                const float4 e1 = tris[3 * i + 1];
+                const float4 e2 = tris[3 * i + 2];
+                const float th = tri_intersect({p.x, p.y, p.z}, {e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, t.o, t.d,
+                                               kEps, kAny ? t.tfar : __builtin_inff());
+                if (kAny) {
+                    if (th < __builtin_inff()) { hit = true; break; }
+                } else {
+                    const int32_t id = __float_as_int(p.w);
+                    if (th < __builtin_inff() && (t.best_id < 0 || th < t.tfar || (th == t.tfar && id < t.best_id))) {
+                        t.tfar = th;
+                        t.best_id = id;
+                        t.best_slot = i;
+                    }
+                }
+            }
+            if (kAny && hit) {
+                t.best_id = 1;
+                t.node = kDone;
+            } else {
+                t.node = st.sp ? st.pop() : kDone;
+            }
+        }
+    }
+}
+
+// Wave-aggregated append to a device queue: one atomic per wave; returns this lane's position
+// (or -1). All lanes of the wave must call it.
+__device__ inline int64_t wave_append(unsigned int *counter, bool pred, int lane) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return -1;
+    const int leader = __builtin_ctzll(m);
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return pred ? (int64_t)base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// One NEE event ready for the frame: taps value·coef at bins center+k, coef = 1/(s-k), or a
+// single tap at center (delta).
+struct Event {
+    int center;
+    float s;
+    cf A;
+    int delta;
+};
+
+// Integrator::splat (integrator.h:113-135) up to the taps: fractional bin (setTime,
+// radar.h:232-241), IF phase (measureRay, integrator.h:107-110) and the splat's own
+// e^{jπs}·sin(πs)/π factor (radar.h:570-583).
+__device__ inline Event make_event(const TraceArgs &a, double tn, cf v, float weight) {
+    Event e;
+    const double dt = tn + (double)a.antennaDelay;
+    double x = 0.0;
+    if (a.freqSlope != 0.0f) {
+        const double q = (dt * (double)a.freqSlope) / (double)a.adcRate;
+        x = (double)a.N * (q - floor(q));
+    }
+    cf ph;
+    if (a.phaseMode == 1) {
+        // the reference's FP32 evaluation of 2π(f0 - Δt S/2)Δt
+        const float dtf = (float)dt;
+        const float arg = ((2.0f * kPiF) * (a.startFreq - dtf * a.freqSlope / 2.0f)) * dtf;
+        ph = {cosf(arg), sinf(arg)};
+    } else {
+        const double turns = ((double)a.startFreq - dt * (double)a.freqSlope / 2.0) * dt;
+        double s, c;
+        sincos_d((2.0 * kPiD) * (turns - floor(turns)), &s, &c);
+        ph = {(float)c, (float)s};
+    }
+    cf value = weight * (v * ph);
+    const int center = (int)round(x);
+    const double s = x - (double)center;
+    e.center = center;
+    if (fabs(s) < 1e-4) {
+        e.delta = 1;
+        e.s = 0.0f;
+        e.A = value;
+        return e;
+    }
+    double ss, cs;
+    sincos_d(kPiD * s, &ss, &cs);
+    value = value * cf{(float)cs, (float)ss};
+    const float w0 = (float)(ss / kPiD);
+    e.delta = 0;
+    e.s = (float)s;
+    e.A = value * w0;
+    return e;
+}
+
+__device__ inline int wrap_bin(int v, int N) {
+    if (N >= 33) {
+        if (v < 0) v += N;
+        else if (v >= N) v -= N;
+        return v;
+    }
+    v %= N;
+    return v < 0 ? v + N : v;
+}
+
+// Wave-cooperative splat of this lane's event (if any) into the workgroup's LDS tile.
+__device__ inline void flush_events(bool has, const Event &e, int ch, double *tile, int N, int C, int lane) {
+    uint64_t m = __ballot(has);
+    while (m) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1;
+        const int center = __builtin_amdgcn_readlane(e.center, src);
+        const float s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.s), src));
+        const float Are = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.A.re), src));
+        const float Aim = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.A.im), src));
+        const int delta = __builtin_amdgcn_readlane(e.delta, src);
+        if (delta) {
+            if (lane == 0) {
+                const int b = wrap_bin(center, N);
+                atomicAdd(&tile[2 * (b * C + ch) + 0], (double)Are);
+                atomicAdd(&tile[2 * (b * C + ch) + 1], (double)Aim);
+            }
+        } else if (lane < 33) {
+            const int k = lane - 16;
+            const int b = wrap_bin(center + k, N);
+            const float coef = 1.0f / (s - (float)k);
+            atomicAdd(&tile[2 * (b * C + ch) + 0], (double)(Are * coef));
+            atomicAdd(&tile[2 * (b * C + ch) + 1], (double)(Aim * coef));
+        }
+    }
+}
+
+// Wave-aggregated FP64 add into the training statistics: lanes with the same leaf are summed
+// with a butterfly and one lane issues the global atomic.
+__device__ inline void record_stats(bool pending, int leaf, double val, double *stat, int lane) {
+    uint64_t m = __ballot(pending);
+    while (m) {
+        const int src = __builtin_ctzll(m);
+        const int L = __builtin_amdgcn_readlane(leaf, src);
+        const bool mine = pending && leaf == L;
+        double v = mine ? val : 0.0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == src) atomicAdd(&stat[L], v);
+        m &= ~__ballot(mine);
+        pending = pending && !mine;
+    }
+}
+
+}  // namespace hsr

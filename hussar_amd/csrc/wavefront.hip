@@ -1,0 +1,412 @@
+// Wavefront pipeline of the radar integrator (PathTracer::sample, path.h:236-451, split into
+// stages over a batch of paths whose state lives in HBM):
+//
+//   k_wf_gen      Halton sample, guiding warp, Tx sample; NEE set-up at depth 0 when
+//                 onlyIndirect is off; the first extension queue.
+//   per depth k:
+//   k_wf_shadow   any-hit BVH traversal of the depth-k shadow rays (persistent, lane refill)
+//   k_wf_gather   per path: guiding weight of the visible NEE terms, splat of the visible
+//                 events into the workgroup's LDS frame tile, guiding record of paths that end
+//   k_wf_extend   closest-hit traversal of the extension rays (persistent, lane refill)
+//   k_wf_shade    hit update and reflection, NEE set-up of depth k+1 for every receiver, the
+//                 next extension queue; guiding record of paths that miss or graze
+//
+// Queues are built with one atomic per wave and their sizes stay on the device, so a batch
+// is enqueued without host synchronisation. The traversal kernels hold only the ray and
+// stack, which keeps their register use low and their occupancy high; a lane whose ray is
+// done takes the next one from the queue while the others continue.
+#include "trace_common.h"
+
+namespace hsr {
+
+enum : int { D_OX = 0, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_NX, D_NY, D_NZ, D_IX, D_IY, D_IZ, D_TIME, D_R, D_COS, D_N };
+enum : int { F_H = 0, F_HIN = 6, F_GWR = 12, F_GWI, F_X0, F_X1, F_PDF, F_FREQ, F_N };
+enum : int { I_DEPTH = 0, I_FLAGS, I_N };
+enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
+
+constexpr int kWfBlock = 256;
+constexpr int kTravSteps = 16;   // node visits between refills of idle lanes
+constexpr int kRefillMin = 16;   // idle lanes that trigger a refill
+
+__device__ inline void load_state(const WfArgs &w, int64_t i, PathState &p) {
+    const double *d = w.sd;
+    const float *f = w.sf;
+    const int64_t P = w.P;
+    p.o = {d[D_OX * P + i], d[D_OY * P + i], d[D_OZ * P + i]};
+    p.d = {d[D_DX * P + i], d[D_DY * P + i], d[D_DZ * P + i]};
+    p.n_in = {d[D_NX * P + i], d[D_NY * P + i], d[D_NZ * P + i]};
+    p.d_in = {d[D_IX * P + i], d[D_IY * P + i], d[D_IZ * P + i]};
+    p.time = d[D_TIME * P + i];
+    p.r = d[D_R * P + i];
+    p.cosTheta = d[D_COS * P + i];
+    p.H = {{f[(F_H + 0) * P + i], f[(F_H + 1) * P + i]},
+           {f[(F_H + 2) * P + i], f[(F_H + 3) * P + i]},
+           {f[(F_H + 4) * P + i], f[(F_H + 5) * P + i]}};
+    p.H_in = {{f[(F_HIN + 0) * P + i], f[(F_HIN + 1) * P + i]},
+              {f[(F_HIN + 2) * P + i], f[(F_HIN + 3) * P + i]},
+              {f[(F_HIN + 4) * P + i], f[(F_HIN + 5) * P + i]}};
+    p.gw = {f[F_GWR * P + i], f[F_GWI * P + i]};
+    p.x0 = f[F_X0 * P + i];
+    p.x1 = f[F_X1 * P + i];
+    p.pdf = f[F_PDF * P + i];
+    p.freq = f[F_FREQ * P + i];
+    p.depth = w.si[I_DEPTH * P + i];
+    p.local = w.base + i;
+}
+
+__device__ inline void store_state(const WfArgs &w, int64_t i, const PathState &p) {
+    double *d = w.sd;
+    float *f = w.sf;
+    const int64_t P = w.P;
+    d[D_OX * P + i] = p.o.x; d[D_OY * P + i] = p.o.y; d[D_OZ * P + i] = p.o.z;
+    d[D_DX * P + i] = p.d.x; d[D_DY * P + i] = p.d.y; d[D_DZ * P + i] = p.d.z;
+    d[D_NX * P + i] = p.n_in.x; d[D_NY * P + i] = p.n_in.y; d[D_NZ * P + i] = p.n_in.z;
+    d[D_IX * P + i] = p.d_in.x; d[D_IY * P + i] = p.d_in.y; d[D_IZ * P + i] = p.d_in.z;
+    d[D_TIME * P + i] = p.time;
+    d[D_R * P + i] = p.r;
+    d[D_COS * P + i] = p.cosTheta;
+    f[(F_H + 0) * P + i] = p.H.x.re; f[(F_H + 1) * P + i] = p.H.x.im;
+    f[(F_H + 2) * P + i] = p.H.y.re; f[(F_H + 3) * P + i] = p.H.y.im;
+    f[(F_H + 4) * P + i] = p.H.z.re; f[(F_H + 5) * P + i] = p.H.z.im;
+    f[(F_HIN + 0) * P + i] = p.H_in.x.re; f[(F_HIN + 1) * P + i] = p.H_in.x.im;
+    f[(F_HIN + 2) * P + i] = p.H_in.y.re; f[(F_HIN + 3) * P + i] = p.H_in.y.im;
+    f[(F_HIN + 4) * P + i] = p.H_in.z.re; f[(F_HIN + 5) * P + i] = p.H_in.z.im;
+    f[F_GWR * P + i] = p.gw.re;
+    f[F_GWI * P + i] = p.gw.im;
+    f[F_X0 * P + i] = p.x0;
+    f[F_X1 * P + i] = p.x1;
+    f[F_PDF * P + i] = p.pdf;
+    f[F_FREQ * P + i] = p.freq;
+    w.si[I_DEPTH * P + i] = p.depth;
+}
+
+// Guiding record of a finished path (path.h:448-450), wave-aggregated.
+__device__ inline void finish_record(const TraceArgs &a, bool finishing, const PathState &p, int lane) {
+    if (!(a.doGuiding && a.guideOn && !a.isFinal)) return;  // uniform
+    bool pend = false;
+    int32_t leaf = 0;
+    double val = 0.0;
+    if (finishing && p.pdf > 0.0f) {
+        const float density = cabsf_(p.gw) * p.pdf;
+        const float weight = 1.0f / p.pdf;
+        leaf = tree_leaf(a.gtChild, p.x0, p.x1);
+        const double dd = (double)density;
+        val = (a.secondMoment ? dd * dd : dd) * (double)weight;
+        pend = true;
+    }
+    record_stats(pend, leaf, val, a.gtStat, lane);
+}
+
+// NEE set-up of every receiver at the path's current vertex into its shadow slots, and the
+// shadow / gather queue entries (all lanes of the wave call it).
+__device__ inline void emit_nee(const TraceArgs &a, const WfArgs &w, bool on, int64_t i, const PathState &p,
+                                WfCounts *cnt, int lane) {
+    const int nrx = min(a.rxCount, a.C);
+    for (int ch = 0; ch < nrx; ++ch) {
+        bool test = false;
+        if (on) {
+            const NeeSetup ns = nee_setup(a, p, ch);
+            const int64_t j = i * a.C + ch;
+            w.s_o[j] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.tfar);
+            w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, 0.0f);
+            w.s_tn[j] = ns.tn;
+            w.s_v[j] = make_float2(ns.v.re, ns.v.im);
+            w.s_g1[j] = make_float2(ns.gw1.re, ns.gw1.im);
+            test = !a.poDiffraction;
+            const int st = S_SETUP | (test ? S_TEST : 0) | (ns.gwAdd ? S_GW : 0) | (ns.splat ? S_SPLAT : 0) |
+                           ((ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) ? S_GW1 : 0);
+            w.s_st[j] = (int8_t)st;
+            w.s_vis[j] = 0;
+        }
+        const int64_t q = wave_append(&cnt->n_shd, test, lane);
+        if (test) w.q_shd[q] = (int32_t)(i * a.C + ch);
+    }
+    const int64_t g = wave_append(&cnt->n_gat, on, lane);
+    if (on) w.q_gat[g] = (int32_t)i;
+}
+
+// Queue the extension ray of a path that goes on, or flag it to end after its NEE.
+__device__ inline void emit_extend(const WfArgs &w, bool alive, bool ends, int64_t i, const PathState &p,
+                                   int32_t *qnext, WfCounts *cnt, int lane) {
+    const bool ext = alive && !ends;
+    const int64_t q = wave_append(&cnt->n_ext, ext, lane);
+    if (ext) {
+        qnext[q] = (int32_t)i;
+        const f3 o = cast3<float>(p.o), d = cast3<float>(p.d);
+        w.ray_o[i] = make_float4(o.x, o.y, o.z, 0.0f);
+        w.ray_d[i] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
+    if (alive) w.si[I_FLAGS * w.P + i] = ends ? 1 : 0;
+}
+
+// ------------------------------------------------------------------------ generation
+__global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * kWfBlock + threadIdx.x;
+    const bool valid = i < w.n;
+    PathState p;
+    if (valid) {
+        p.local = w.base + i;
+        generate(a, (uint64_t)(a.offset + w.base + i), p);
+    }
+    const bool nee = valid && !a.onlyIndirect;
+    emit_nee(a, w, nee, i, p, &w.cnt[0], lane);
+    const bool ends = valid && path_ends(a, p);
+    emit_extend(w, valid, ends, i, p, w.q_ext0, &w.cnt[0], lane);
+    if (valid) store_state(w, i, p);
+    finish_record(a, valid && ends && !nee, p, lane);
+}
+
+// ------------------------------------------------------------------------ traversal
+// Persistent: each wave keeps its lanes on rays, refilling idle lanes from the queue.
+template <bool kAny, bool kCount>
+__global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int depth) {
+    __shared__ int32_t lds_stack[kStackShort * kWfBlock];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    Stack st;
+    st.lds = lds_stack + tid;
+    st.sp = 0;
+    WfCounts *cnt = &w.cnt[depth];
+    const unsigned int n = kAny ? cnt->n_shd : cnt->n_ext;
+    unsigned int *head = kAny ? &cnt->head_shd : &cnt->head_ext;
+    const int32_t *queue = kAny ? w.q_shd : ((depth & 1) ? w.q_ext1 : w.q_ext0);
+    uint32_t cn = 0, ct = 0, cr = 0;
+    Trav t;
+    int32_t item = -1;
+    bool has = false, exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        if (!exhausted && __popcll(idle) >= kRefillMin) {
+            const int leader = __builtin_ctzll(idle);
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
+            base = __shfl(base, leader, 64);
+            if (base + (unsigned int)__popcll(idle) >= n) exhausted = true;
+            const unsigned int my = base + (unsigned int)__popcll(idle & ((1ull << lane) - 1ull));
+            if (!has && my < n) {
+                item = queue[my];
+                const float4 o = kAny ? w.s_o[item] : w.ray_o[item];
+                const float4 d = kAny ? w.s_d[item] : w.ray_d[item];
+                trav_begin(t, {o.x, o.y, o.z}, {d.x, d.y, d.z}, kAny ? o.w : __builtin_inff(), st);
+                has = true;
+                if (kCount) ++cr;
+            }
+        }
+        if (!__any(has)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has) {
+            trav_run<kAny, kCount>(a.nodes, a.tris, t, st, kTravSteps, cn, ct);
+            if (t.node == kDone) {
+                if (kAny) w.s_vis[item] = t.best_id >= 0 ? 1 : 2;
+                else w.hit[item] = make_int2(t.best_slot, __float_as_int(t.tfar));
+                has = false;
+            }
+        }
+    }
+    if (kCount) {
+        atomicAdd(&a.counters[0], (unsigned long long)cn);
+        atomicAdd(&a.counters[1], (unsigned long long)ct);
+        atomicAdd(&a.counters[kAny ? 3 : 2], (unsigned long long)cr);
+    }
+}
+
+// ------------------------------------------------------------------------ shade
+__global__ __launch_bounds__(kWfBlock) void k_wf_shade(TraceArgs a, WfArgs w, int depth) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
+    const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
+    const unsigned int n = w.cnt[depth].n_ext;
+    const int32_t *q = (depth & 1) ? w.q_ext1 : w.q_ext0;
+    int32_t *qnext = (depth & 1) ? w.q_ext0 : w.q_ext1;
+    WfCounts *next = &w.cnt[depth + 1];
+    for (int64_t b = wave * 64; b < n; b += waves * 64) {
+        const int64_t k = b + lane;
+        const bool valid = k < n;
+        const int64_t i = valid ? q[k] : 0;
+        PathState p;
+        bool alive = false;
+        if (valid) {
+            load_state(w, i, p);
+            const int2 h = w.hit[i];
+            const int32_t id = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
+            if (a.recHits && p.local < a.recPaths) a.recHits[p.local * a.maxDepth + p.depth] = id;
+            alive = h.x >= 0 && apply_hit(a, p, h.x);
+        }
+        emit_nee(a, w, alive, i, p, next, lane);
+        const bool ends = alive && path_ends(a, p);
+        emit_extend(w, alive, ends, i, p, qnext, next, lane);
+        if (alive) store_state(w, i, p);
+        finish_record(a, valid && !alive, p, lane);
+    }
+}
+
+// ------------------------------------------------------------------------ gather + splat
+__global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, int depth, double2 *partials,
+                                                        bool count) {
+    extern __shared__ double tile[];  // [N*C] complex double
+    const int NC = a.N * a.C;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    for (int j = tid; j < 2 * NC; j += kWfBlock) tile[j] = 0.0;
+    __syncthreads();
+    const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + tid) >> 6;
+    const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
+    const unsigned int n = w.cnt[depth].n_gat;
+    const int nrx = min(a.rxCount, a.C);
+    const int64_t P = w.P;
+    uint32_t cev = 0;
+    for (int64_t b = wave * 64; b < n; b += waves * 64) {
+        const int64_t k = b + lane;
+        const bool valid = k < n;
+        const int64_t i = valid ? w.q_gat[k] : 0;
+        PathState p;
+        if (valid) {
+            p.gw = {w.sf[F_GWR * P + i], w.sf[F_GWI * P + i]};
+            p.x0 = w.sf[F_X0 * P + i];
+            p.x1 = w.sf[F_X1 * P + i];
+            p.pdf = w.sf[F_PDF * P + i];
+            p.depth = w.si[I_DEPTH * P + i];
+            p.local = w.base + i;
+        }
+        const bool rec = valid && a.recNee && p.local < a.recPaths && p.depth >= 1;
+        for (int ch = 0; ch < nrx; ++ch) {
+            bool has = false;
+            Event ev{0, 0.0f, {0.0f, 0.0f}, 0};
+            const int64_t j = i * a.C + ch;
+            const int stt = valid ? (int)w.s_st[j] : 0;
+            if (stt & S_SETUP) {
+                const bool vis = (stt & S_TEST) ? w.s_vis[j] == 2 : true;
+                int8_t flag = 1;
+                if (vis) {
+                    flag = 2;
+                    const float2 v2 = w.s_v[j];
+                    const cf v = {v2.x, v2.y};
+                    if (stt & S_GW1) {
+                        const float2 g = w.s_g1[j];
+                        p.gw = p.gw + cf{g.x, g.y};
+                    }
+                    if (stt & S_GW) p.gw = p.gw + v;
+                    if (stt & S_SPLAT) {
+                        ev = make_event(a, w.s_tn[j], v, a.sampleWeight);
+                        has = true;
+                        flag = 3;
+                        if (count) ++cev;
+                    }
+                }
+                if (rec) a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + ch] = flag;
+            }
+            flush_events(has, ev, ch, tile, a.N, a.C, lane);
+        }
+        if (valid) {
+            w.sf[F_GWR * P + i] = p.gw.re;
+            w.sf[F_GWI * P + i] = p.gw.im;
+        }
+        const bool finishing = valid && (w.si[I_FLAGS * P + i] & 1);
+        finish_record(a, finishing, p, lane);
+    }
+    __syncthreads();
+    double2 *dst = partials + (size_t)blockIdx.x * NC;
+    for (int j = tid; j < NC; j += kWfBlock) {
+        double2 v = dst[j];
+        v.x += tile[2 * j];
+        v.y += tile[2 * j + 1];
+        dst[j] = v;
+    }
+    if (count) atomicAdd(&a.counters[4], (unsigned long long)cev);
+}
+
+__global__ void k_wf_zero_counts(WfCounts *c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = WfCounts{0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+// ------------------------------------------------------------------------ host side
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
+    size_t b = 0;
+    b += align256(sizeof(double) * D_N * P);
+    b += align256(sizeof(float) * F_N * P);
+    b += align256(sizeof(int32_t) * I_N * P);
+    b += 2 * align256(sizeof(float4) * P);
+    b += align256(sizeof(int2) * P);
+    const int64_t S = P * C;
+    b += 2 * align256(sizeof(float4) * S);
+    b += align256(sizeof(double) * S);
+    b += 2 * align256(sizeof(float2) * S);
+    b += 2 * align256(sizeof(int8_t) * S);
+    b += 3 * align256(sizeof(int32_t) * P) + align256(sizeof(int32_t) * S);
+    b += align256(sizeof(WfCounts) * (maxDepth + 2));
+    return b;
+}
+
+void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
+    char *p = (char *)base;
+    auto take = [&](size_t bytes) {
+        char *r = p;
+        p += align256(bytes);
+        return (void *)r;
+    };
+    const int64_t S = P * C;
+    w.P = P;
+    w.sd = (double *)take(sizeof(double) * D_N * P);
+    w.sf = (float *)take(sizeof(float) * F_N * P);
+    w.si = (int32_t *)take(sizeof(int32_t) * I_N * P);
+    w.ray_o = (float4 *)take(sizeof(float4) * P);
+    w.ray_d = (float4 *)take(sizeof(float4) * P);
+    w.hit = (int2 *)take(sizeof(int2) * P);
+    w.s_o = (float4 *)take(sizeof(float4) * S);
+    w.s_d = (float4 *)take(sizeof(float4) * S);
+    w.s_tn = (double *)take(sizeof(double) * S);
+    w.s_v = (float2 *)take(sizeof(float2) * S);
+    w.s_g1 = (float2 *)take(sizeof(float2) * S);
+    w.s_st = (int8_t *)take(sizeof(int8_t) * S);
+    w.s_vis = (int8_t *)take(sizeof(int8_t) * S);
+    w.q_ext0 = (int32_t *)take(sizeof(int32_t) * P);
+    w.q_ext1 = (int32_t *)take(sizeof(int32_t) * P);
+    w.q_gat = (int32_t *)take(sizeof(int32_t) * P);
+    w.q_shd = (int32_t *)take(sizeof(int32_t) * S);
+    w.cnt = (WfCounts *)take(sizeof(WfCounts) * (maxDepth + 2));
+}
+
+size_t gather_lds_bytes(int N, int C) { return (size_t)N * C * 2 * sizeof(double); }
+
+int wavefront_trav_blocks_per_cu(bool count) {
+    int n = 0;
+    hipError_t e = count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trav<false, true>, kWfBlock, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trav<false, false>, kWfBlock, 0);
+    return e == hipSuccess && n > 0 ? n : 1;
+}
+
+hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
+                            double2 *partials, bool count, hipStream_t s) {
+    (void)hipGetLastError();
+    const int maxDepth = a.maxDepth;
+    hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 63) / 64), dim3(64), 0, s, w.cnt, maxDepth + 2);
+    hipLaunchKernelGGL(k_wf_gen, dim3((unsigned)((w.n + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s, a, w);
+    const size_t glds = gather_lds_bytes(a.N, a.C);
+    for (int k = 0; k <= maxDepth; ++k) {
+        if (k >= 1 || !a.onlyIndirect) {
+            if (!a.poDiffraction) {
+                if (count)
+                    hipLaunchKernelGGL((k_wf_trav<true, true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                else
+                    hipLaunchKernelGGL((k_wf_trav<true, false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+            }
+            hipLaunchKernelGGL(k_wf_gather, dim3(gridGather), dim3(kWfBlock), glds, s, a, w, k, partials, count);
+        }
+        if (k < maxDepth) {
+            if (count)
+                hipLaunchKernelGGL((k_wf_trav<false, true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+            else
+                hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+            hipLaunchKernelGGL(k_wf_shade, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k);
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hsr

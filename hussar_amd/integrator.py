@@ -167,6 +167,7 @@ class HipBackend:
                                    I.ctypes.data_as(C.POINTER(C.c_int32)), I.shape[0], device, C.byref(h)))
         self._h = h
         self.device = device
+        self.pipeline = 0
         self.integrator = integrator
         self.frame_config: Optional[FrameConfig] = None
         if integrator is not None:
@@ -210,6 +211,8 @@ class HipBackend:
 
     def set_option(self, option: int, value: int) -> None:
         check(lib().hsr_hip_set_option(self._h, int(option), int(value)))
+        if option == _lib.OPT_PIPELINE:
+            self.pipeline = int(value)
 
     def stats(self) -> dict:
         s = c_run_stats()

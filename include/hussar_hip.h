@@ -214,7 +214,8 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        id hit at every depth and the shadow-ray outcome of every NEE;
  *                        read back with hsr_hip_get_hit_record.
  *  HSR_OPT_PATHS_PER_LAUNCH  maximum paths per kernel launch (interrupt granularity).
- *  HSR_OPT_PIPELINE      0 = fused path kernel (default), 1 = wavefront stages.
+ *  HSR_OPT_PIPELINE      0 = fused path kernel (default), 1 = wavefront stages (path
+ *                        state in device memory, persistent traversal kernels).
  *  HSR_OPT_PHASE_MODE    0 = FP64 phase (default), 1 = FP32 phase as in the reference,
  *                        2 = double-float (two-float) phase. */
 #define HSR_OPT_COUNTERS 1

@@ -14,13 +14,24 @@ import pytest
 
 import hussar_amd as H
 from hussar_amd import scenes
-from hussar_amd._lib import HsrError, OPT_COUNTERS, OPT_PATHS_PER_LAUNCH, OPT_RECORD_HITS
+from hussar_amd._lib import HsrError, OPT_COUNTERS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE, OPT_RECORD_HITS
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
 THREADS = min(16, os.cpu_count() or 1)
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+PIPELINES = [0, 1]  # fused path kernel, wavefront stages
+
+
+def make_backend(mesh, fc, pipeline, integrator=None):
+    b = H.HipBackend(mesh, integrator)
+    b.set_option(OPT_PIPELINE, pipeline)
+    if fc is not None and integrator is None:
+        b._configure(fc)
+    return b
 
 
 def gpu_run(backend, scene, knobs, n, offset=0, record=0, guide=None, is_final=False, clear=True):
@@ -52,13 +63,11 @@ def check_parity(oracle, om, backend, scene, knobs, fc, n, record, guide=None, o
     return z
 
 
-@pytest.fixture(scope="module")
-def dihedral():
+@pytest.fixture(scope="module", params=PIPELINES, ids=["fused", "wavefront"])
+def dihedral(request):
     mesh = scenes.dihedral_mesh()
-    b = H.HipBackend(mesh)
     fc = H.FrameConfig(1, 256, 1)
-    b._configure(fc)
-    return mesh, b, fc
+    return mesh, make_backend(mesh, fc, request.param), fc
 
 
 @pytest.mark.parametrize("yaw", [-45.0, -17.0, 0.0, 30.0, 45.0])
@@ -82,11 +91,15 @@ def test_dihedral_full_pose_size_vs_fp64(oracle, dihedral):
 
 
 @pytest.fixture(scope="module")
-def vehicle():
-    mesh, centre = scenes.vehicle_mesh(seed=1, triangles=200_000)
-    b = H.HipBackend(mesh)
+def vehicle_mesh():
+    return scenes.vehicle_mesh(seed=1, triangles=200_000)
+
+
+@pytest.fixture(scope="module", params=PIPELINES, ids=["fused", "wavefront"])
+def vehicle(request, vehicle_mesh):
+    mesh, centre = vehicle_mesh
     fc = H.FrameConfig(1, 256, 4)
-    b._configure(fc)
+    b = make_backend(mesh, fc, request.param)
     rf = scenes.dihedral_rf()
     sc = scenes.radar_array(rf, channels=4)
     return mesh, b, fc, sc, centre
@@ -151,7 +164,7 @@ def test_path_tracer_run_doubling_schedule(oracle, dihedral):
     om = oracle.OracleMesh.from_mesh(mesh)
     pt = H.PathTracer(maxDepth=3, doGuiding=True)
     pt.configureFrame(fc)
-    b2 = H.HipBackend(mesh, pt)
+    b2 = make_backend(mesh, None, b.pipeline if hasattr(b, "pipeline") else 0, pt)
     sc = scenes.dihedral_pose(10.0)
     pt.run(b2, sc, 1 << 20)
     z = pt.fetchFrame("complex128")
@@ -190,9 +203,10 @@ def test_repeatable(dihedral):
     np.testing.assert_allclose(a, c, rtol=1e-13, atol=1e-15 * np.abs(a).max())
 
 
+@pytest.mark.parametrize("pipeline", PIPELINES, ids=["fused", "wavefront"])
 @pytest.mark.parametrize("case", ["direct", "po_diffraction", "plane_filter", "no_filter", "clang_order",
                                   "small_frame", "odd_budget"])
-def test_knob_and_shape_variants(oracle, case):
+def test_knob_and_shape_variants(oracle, case, pipeline):
     mesh = scenes.dihedral_mesh()
     sc = scenes.dihedral_pose(5.0)
     kn = H.Knobs(maxDepth=3, doGuiding=False)
@@ -215,16 +229,15 @@ def test_knob_and_shape_variants(oracle, case):
         fc = H.FrameConfig(1, 24, 1)
     elif case == "odd_budget":
         n = 100_003
-    b = H.HipBackend(mesh)
-    b._configure(fc)
+    b = make_backend(mesh, fc, pipeline)
     om = oracle.OracleMesh.from_mesh(mesh)
     check_parity(oracle, om, b, sc, kn, fc, n, record=min(n, 1 << 16), fp64=True)
 
 
-def test_empty_mesh_and_zero_depth():
-    b = H.HipBackend(H.TriangleMesh())
+@pytest.mark.parametrize("pipeline", PIPELINES, ids=["fused", "wavefront"])
+def test_empty_mesh_and_zero_depth(pipeline):
     fc = H.FrameConfig(1, 256, 1)
-    b._configure(fc)
+    b = make_backend(H.TriangleMesh(), fc, pipeline)
     sc = scenes.dihedral_pose(0.0)
     z, tw, rec = gpu_run(b, sc, H.Knobs(maxDepth=3, doGuiding=False), 4096, record=4096)
     assert tw == 4096 and np.all(z == 0) and np.all(rec[0] == -1)
@@ -269,8 +282,7 @@ def test_reference_dihedral_sweep_fixture_on_gpu(size_cm):
     and peak magnitude within 10 % at in-beam poses."""
     from test_oracle_integrator import mesh_1dihedral, scene_1dihedral
     fx = json.load(open(os.path.join(GOLDEN, "dihedral_fixture.json")))["sizes_cm"][str(size_cm)]
-    b = H.HipBackend(mesh_1dihedral(size_cm / 100))
-    b._configure(H.FrameConfig(1, 256, 1))
+    b = make_backend(mesh_1dihedral(size_cm / 100), H.FrameConfig(1, 256, 1), 1)
     kn = H.Knobs(maxDepth=10, doGuiding=False)
     for yaw in np.arange(-30.0, 30.01, 2.5):
         i = int(round((yaw + 75.0) / 0.25))
