@@ -8,8 +8,11 @@
 //   k_wf_gather   per path: guiding weight of the visible NEE terms, splat of the visible
 //                 events into the workgroup's LDS frame tile, guiding record of paths that end
 //   k_wf_extend   closest-hit traversal of the extension rays (persistent, lane refill)
-//   k_wf_shade    hit update and reflection, NEE set-up of depth k+1 for every receiver, the
-//                 next extension queue; guiding record of paths that miss or graze
+//   k_wf_hit      hit update and reflection, the next extension queue and the list of paths
+//                 with an NEE at depth k+1; guiding record of paths that miss or graze
+//   k_wf_nee      one thread per (path, receiver): NEE set-up of depth k+1 into the shadow
+//                 slot; the shadow queue stays path-major, so neighbouring lanes trace the
+//                 nearly identical rays towards receivers a few millimetres apart
 //
 // Queues are built with one atomic per wave and their sizes stay on the device, so a batch
 // is enqueued without host synchronisation. The traversal kernels hold only the ray and
@@ -80,6 +83,65 @@ __device__ inline void store_state(const WfArgs &w, int64_t i, const PathState &
     w.si[I_DEPTH * P + i] = p.depth;
 }
 
+// The fields a stage needs, loaded separately to keep register use and traffic down.
+__device__ inline void load_chain(const WfArgs &w, int64_t i, PathState &p) {
+    const double *d = w.sd;
+    const int64_t P = w.P;
+    p.o = {d[D_OX * P + i], d[D_OY * P + i], d[D_OZ * P + i]};
+    p.d = {d[D_DX * P + i], d[D_DY * P + i], d[D_DZ * P + i]};
+    p.time = d[D_TIME * P + i];
+    p.r = d[D_R * P + i];
+    p.depth = w.si[I_DEPTH * P + i];
+    p.local = w.base + i;
+}
+__device__ inline void load_field(const WfArgs &w, int64_t i, PathState &p) {
+    const float *f = w.sf;
+    const int64_t P = w.P;
+    p.H = {{f[(F_H + 0) * P + i], f[(F_H + 1) * P + i]},
+           {f[(F_H + 2) * P + i], f[(F_H + 3) * P + i]},
+           {f[(F_H + 4) * P + i], f[(F_H + 5) * P + i]}};
+}
+__device__ inline void load_record(const WfArgs &w, int64_t i, PathState &p) {
+    const float *f = w.sf;
+    const int64_t P = w.P;
+    p.gw = {f[F_GWR * P + i], f[F_GWI * P + i]};
+    p.x0 = f[F_X0 * P + i];
+    p.x1 = f[F_X1 * P + i];
+    p.pdf = f[F_PDF * P + i];
+}
+__device__ inline void load_vertex(const WfArgs &w, int64_t i, PathState &p) {
+    const double *d = w.sd;
+    const float *f = w.sf;
+    const int64_t P = w.P;
+    p.n_in = {d[D_NX * P + i], d[D_NY * P + i], d[D_NZ * P + i]};
+    p.d_in = {d[D_IX * P + i], d[D_IY * P + i], d[D_IZ * P + i]};
+    p.cosTheta = d[D_COS * P + i];
+    p.H_in = {{f[(F_HIN + 0) * P + i], f[(F_HIN + 1) * P + i]},
+              {f[(F_HIN + 2) * P + i], f[(F_HIN + 3) * P + i]},
+              {f[(F_HIN + 4) * P + i], f[(F_HIN + 5) * P + i]}};
+    p.freq = f[F_FREQ * P + i];
+}
+// after a hit: everything apply_hit changed
+__device__ inline void store_hit(const WfArgs &w, int64_t i, const PathState &p) {
+    double *d = w.sd;
+    float *f = w.sf;
+    const int64_t P = w.P;
+    d[D_OX * P + i] = p.o.x; d[D_OY * P + i] = p.o.y; d[D_OZ * P + i] = p.o.z;
+    d[D_DX * P + i] = p.d.x; d[D_DY * P + i] = p.d.y; d[D_DZ * P + i] = p.d.z;
+    d[D_NX * P + i] = p.n_in.x; d[D_NY * P + i] = p.n_in.y; d[D_NZ * P + i] = p.n_in.z;
+    d[D_IX * P + i] = p.d_in.x; d[D_IY * P + i] = p.d_in.y; d[D_IZ * P + i] = p.d_in.z;
+    d[D_TIME * P + i] = p.time;
+    d[D_R * P + i] = p.r;
+    d[D_COS * P + i] = p.cosTheta;
+    f[(F_H + 0) * P + i] = p.H.x.re; f[(F_H + 1) * P + i] = p.H.x.im;
+    f[(F_H + 2) * P + i] = p.H.y.re; f[(F_H + 3) * P + i] = p.H.y.im;
+    f[(F_H + 4) * P + i] = p.H.z.re; f[(F_H + 5) * P + i] = p.H.z.im;
+    f[(F_HIN + 0) * P + i] = p.H_in.x.re; f[(F_HIN + 1) * P + i] = p.H_in.x.im;
+    f[(F_HIN + 2) * P + i] = p.H_in.y.re; f[(F_HIN + 3) * P + i] = p.H_in.y.im;
+    f[(F_HIN + 4) * P + i] = p.H_in.z.re; f[(F_HIN + 5) * P + i] = p.H_in.z.im;
+    w.si[I_DEPTH * P + i] = p.depth;
+}
+
 // Guiding record of a finished path (path.h:448-450), wave-aggregated.
 __device__ inline void finish_record(const TraceArgs &a, bool finishing, const PathState &p, int lane) {
     if (!(a.doGuiding && a.guideOn && !a.isFinal)) return;  // uniform
@@ -97,30 +159,9 @@ __device__ inline void finish_record(const TraceArgs &a, bool finishing, const P
     record_stats(pend, leaf, val, a.gtStat, lane);
 }
 
-// NEE set-up of every receiver at the path's current vertex into its shadow slots, and the
-// shadow / gather queue entries (all lanes of the wave call it).
-__device__ inline void emit_nee(const TraceArgs &a, const WfArgs &w, bool on, int64_t i, const PathState &p,
-                                WfCounts *cnt, int lane) {
-    const int nrx = min(a.rxCount, a.C);
-    for (int ch = 0; ch < nrx; ++ch) {
-        bool test = false;
-        if (on) {
-            const NeeSetup ns = nee_setup(a, p, ch);
-            const int64_t j = i * a.C + ch;
-            w.s_o[j] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.tfar);
-            w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, 0.0f);
-            w.s_tn[j] = ns.tn;
-            w.s_v[j] = make_float2(ns.v.re, ns.v.im);
-            w.s_g1[j] = make_float2(ns.gw1.re, ns.gw1.im);
-            test = !a.poDiffraction;
-            const int st = S_SETUP | (test ? S_TEST : 0) | (ns.gwAdd ? S_GW : 0) | (ns.splat ? S_SPLAT : 0) |
-                           ((ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) ? S_GW1 : 0);
-            w.s_st[j] = (int8_t)st;
-            w.s_vis[j] = 0;
-        }
-        const int64_t q = wave_append(&cnt->n_shd, test, lane);
-        if (test) w.q_shd[q] = (int32_t)(i * a.C + ch);
-    }
+// Paths with an NEE at their current vertex go to the gather list; k_wf_nee then sets up one
+// shadow slot per (path, receiver) (all lanes of the wave call it).
+__device__ inline void emit_gather(const WfArgs &w, bool on, int64_t i, WfCounts *cnt, int lane) {
     const int64_t g = wave_append(&cnt->n_gat, on, lane);
     if (on) w.q_gat[g] = (int32_t)i;
 }
@@ -150,7 +191,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
         generate(a, (uint64_t)(a.offset + w.base + i), p);
     }
     const bool nee = valid && !a.onlyIndirect;
-    emit_nee(a, w, nee, i, p, &w.cnt[0], lane);
+    emit_gather(w, nee, i, &w.cnt[0], lane);
     const bool ends = valid && path_ends(a, p);
     emit_extend(w, valid, ends, i, p, w.q_ext0, &w.cnt[0], lane);
     if (valid) store_state(w, i, p);
@@ -213,8 +254,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int
     }
 }
 
-// ------------------------------------------------------------------------ shade
-__global__ __launch_bounds__(kWfBlock) void k_wf_shade(TraceArgs a, WfArgs w, int depth) {
+// ------------------------------------------------------------------------ hit
+__global__ __launch_bounds__(kWfBlock) void k_wf_hit(TraceArgs a, WfArgs w, int depth) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
@@ -229,17 +270,58 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(TraceArgs a, WfArgs w, in
         PathState p;
         bool alive = false;
         if (valid) {
-            load_state(w, i, p);
+            load_chain(w, i, p);
+            load_field(w, i, p);
             const int2 h = w.hit[i];
             const int32_t id = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
             if (a.recHits && p.local < a.recPaths) a.recHits[p.local * a.maxDepth + p.depth] = id;
             alive = h.x >= 0 && apply_hit(a, p, h.x);
+            if (!alive) load_record(w, i, p);
         }
-        emit_nee(a, w, alive, i, p, next, lane);
+        emit_gather(w, alive, i, next, lane);
         const bool ends = alive && path_ends(a, p);
         emit_extend(w, alive, ends, i, p, qnext, next, lane);
-        if (alive) store_state(w, i, p);
+        if (alive) store_hit(w, i, p);
         finish_record(a, valid && !alive, p, lane);
+    }
+}
+
+// ------------------------------------------------------------------------ NEE set-up
+// Items j < n_gat * nrx: path q_gat[j / nrx], receiver j % nrx.
+__global__ __launch_bounds__(kWfBlock) void k_wf_nee(TraceArgs a, WfArgs w, int depth) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
+    const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
+    const int nrx = min(a.rxCount, a.C);
+    WfCounts *cnt = &w.cnt[depth];
+    const int64_t n = (int64_t)cnt->n_gat * nrx;
+    for (int64_t b = wave * 64; b < n; b += waves * 64) {
+        const int64_t k = b + lane;
+        const bool valid = k < n;
+        bool test = false;
+        int64_t j = 0;
+        if (valid) {
+            const int64_t i = w.q_gat[k / nrx];
+            const int ch = (int)(k % nrx);
+            PathState p;
+            load_chain(w, i, p);
+            if (p.depth > 0) load_vertex(w, i, p);
+            else p.freq = w.sf[F_FREQ * w.P + i];
+            const NeeSetup ns = nee_setup(a, p, ch);
+            j = i * a.C + ch;
+            w.s_o[j] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.tfar);
+            w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, 0.0f);
+            w.s_tn[j] = ns.tn;
+            w.s_v[j] = make_float2(ns.v.re, ns.v.im);
+            w.s_g1[j] = make_float2(ns.gw1.re, ns.gw1.im);
+            test = !a.poDiffraction;
+            const int st = S_SETUP | (test ? S_TEST : 0) | (ns.gwAdd ? S_GW : 0) | (ns.splat ? S_SPLAT : 0) |
+                           ((ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) ? S_GW1 : 0);
+            w.s_st[j] = (int8_t)st;
+            w.s_vis[j] = 0;
+        }
+        const int64_t q = wave_append(&cnt->n_shd, test, lane);
+        if (test) w.q_shd[q] = (int32_t)j;
     }
 }
 
@@ -388,6 +470,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 63) / 64), dim3(64), 0, s, w.cnt, maxDepth + 2);
     hipLaunchKernelGGL(k_wf_gen, dim3((unsigned)((w.n + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s, a, w);
     const size_t glds = gather_lds_bytes(a.N, a.C);
+    if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, 0);
     for (int k = 0; k <= maxDepth; ++k) {
         if (k >= 1 || !a.onlyIndirect) {
             if (!a.poDiffraction) {
@@ -403,7 +486,8 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
                 hipLaunchKernelGGL((k_wf_trav<false, true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
             else
                 hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
-            hipLaunchKernelGGL(k_wf_shade, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k);
+            hipLaunchKernelGGL(k_wf_hit, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k);
+            hipLaunchKernelGGL(k_wf_nee, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k + 1);
         }
     }
     return hipGetLastError();

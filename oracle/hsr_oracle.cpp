@@ -10,8 +10,9 @@
 // scalar):
 //   prec 0 "ref32"  G=float,  P=float   — the reference's own arithmetic (hussar.h:37,
 //                                         radar.h:36): FP32 everywhere.
-//   prec 1 "mixed"  G=float,  P=double  — the product's precision model: FP32 traversal,
-//                                         geometry and fields; every segment length is
+//   prec 1 "mixed"  G=float,  P=double  — the product's precision model: FP32 traversal
+//                                         and fields (antenna lobes included); every segment
+//                                         length is
 //                                         recomputed in FP64 against the hit triangle's
 //                                         plane and chained through FP64 hit points, so the
 //                                         path time, phase and frame sums are FP64.
@@ -252,14 +253,23 @@ template <class G> inline V3<G> uniform_sample_sphere(G u0, G u1) {
 }
 
 // AWRAngularDistribution::evaluate (emitter.h:126-136). H = e_y × d, scaled by the two
-// polynomial lobes; the lobes are evaluated in double (the double literals promote) and
-// then applied as a float (G) factor.
-template <class G> inline CV3<G> awr_evaluate(V3<G> d) {
+// polynomial lobes. kPromote: the lobes in double as the reference's double literals promote
+// them, then applied as a G factor (reference arithmetic); otherwise in G throughout, in the
+// order ((a2·a2)·a2) with a2 = a·a (the product's FP32 field arithmetic).
+template <class G, bool kPromote = true> inline CV3<G> awr_evaluate(V3<G> d) {
     CV3<G> H{{d.z, G(0)}, {G(0), G(0)}, {-d.x, G(0)}};  // (0,1,0) × d
     const G cos_h = std::sqrt(G(1) - d.x * d.x);
     const G cos_e = std::sqrt(G(1) - d.y * d.y);
-    const G f1 = G(2.622 / std::pow((double)cos_h - 1.8, 6));
-    const G f2 = G(0.625 / std::pow((double)cos_e - 1.5, 4));
+    G f1, f2;
+    if (kPromote) {
+        f1 = G(2.622 / std::pow((double)cos_h - 1.8, 6));
+        f2 = G(0.625 / std::pow((double)cos_e - 1.5, 4));
+    } else {
+        const G a = cos_h - G(1.8), b = cos_e - G(1.5);
+        const G a2 = a * a, b2 = b * b;
+        f1 = G(2.622) / ((a2 * a2) * a2);
+        f2 = G(0.625) / (b2 * b2);
+    }
     H = {H.x * f1, H.y * f1, H.z * f1};
     H = {H.x * f2, H.y * f2, H.z * f2};
     return H;
@@ -825,7 +835,7 @@ template <class A> inline M33<A> mcast(const float *m) {
     return r;
 }
 
-template <class T, class Gd, class F, class P> struct Tracer {
+template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
     static constexpr bool refine = !std::is_same<T, Gd>::value;
     const Mesh *mesh;
     const Accel<T> *acc;
@@ -892,7 +902,7 @@ template <class T, class Gd, class F, class P> struct Tracer {
         // depth 0: tx.sample (emitter.h:174-180) after the guiding warp (path.h:270-275)
         if (knobs->doGuiding && guide) primaryPdf = guide->sample(primary);
         const V3<Gd> dl = uniform_sample_sphere<Gd>(Gd(primary[0]), Gd(primary[1]));
-        const CV3<F> Hl = awr_evaluate<F>(vcast<F>(dl)) / K<F>::inv4pi();  // evaluate(d) / pdf(d)
+        const CV3<F> Hl = awr_evaluate<F, kPromote>(vcast<F>(dl)) / K<F>::inv4pi();  // evaluate(d) / pdf(d)
         V3<Gd> o = vcast3<Gd>(scene->tx.pos);
         V3<Gd> d = txRg.mul(dl);
         CV3<F> H = txRf.mul(Hl);
@@ -914,7 +924,7 @@ template <class T, class Gd, class F, class P> struct Tracer {
                     V3<Gd> dn = vcast3<Gd>(scene->rx[ch].pos) - o;
                     const Gd rr = norm(dn);
                     dn = dn / rr;
-                    const CV3<F> Hrx = rxRf.mul(awr_evaluate<F>(vcast<F>(rxRg.mulT(-dn))));
+                    const CV3<F> Hrx = rxRf.mul(awr_evaluate<F, kPromote>(vcast<F>(rxRg.mulT(-dn))));
                     if (!knobs->poDiffraction && !trace_visible(vcast<T>(o), vcast<T>(dn), T(rr))) {
                         if (rec && depth >= 1) rec->nee[(depth - 1) * C + ch] = 1;
                         continue;
@@ -923,7 +933,7 @@ template <class T, class Gd, class F, class P> struct Tracer {
                     CV3<F> Hn;
                     if (depth == 0) {
                         // tx.evaluate + tx.connect (emitter.h:182-193)
-                        Hn = txRf.mul(awr_evaluate<F>(vcast<F>(txRg.mulT(dn))));
+                        Hn = txRf.mul(awr_evaluate<F, kPromote>(vcast<F>(txRg.mulT(dn))));
                         Hn = (F(1) / ((F(4) * piF) * rrF)) * Hn;
                     } else {
                         // SurfaceEmitter::evaluate (emitter.h:64-70)
@@ -1020,7 +1030,7 @@ template <class T, class Gd, class F, class P> struct Tracer {
 // ---------------------------------------------------------------------------------------
 // Backend::run (cpu.h:21-41): a std::thread pool grabbing batches of 256 consecutive
 // indices under a mutex; one accumulation frame per worker, summed after the join.
-template <class T, class Gd, class F, class P>
+template <class T, class Gd, class F, class P, bool kPromote>
 void render(const Mesh &mesh, const SceneC &scene, const Knobs &knobs, const FrameCfg &fc, int64_t offset,
             int64_t count, float sampleWeight, bool isFinal, Guide *guide, int nthreads, double *frame_out,
             double *totalWeight, int32_t *hit_ids, int8_t *nee, int64_t record_paths, Counters *counters) {
@@ -1034,7 +1044,7 @@ void render(const Mesh &mesh, const SceneC &scene, const Knobs &knobs, const Fra
     std::mutex mtx;
     int64_t next = 0;
     auto worker = [&](int tid) {
-        Tracer<T, Gd, F, P> tr;
+        Tracer<T, Gd, F, P, kPromote> tr;
         tr.mesh = &mesh;
         tr.acc = &accel_of<T>(mesh);
         tr.scene = &scene;
@@ -1096,9 +1106,9 @@ typedef void (*RenderFn)(const Mesh &, const SceneC &, const Knobs &, const Fram
                          bool, Guide *, int, double *, double *, int32_t *, int8_t *, int64_t, Counters *);
 static RenderFn render_for(int prec) {
     switch (prec) {
-        case 0: return &render<float, float, float, float>;
-        case 1: return &render<float, double, float, double>;
-        default: return &render<double, double, double, double>;
+        case 0: return &render<float, float, float, float, true>;
+        case 1: return &render<float, double, float, double, false>;
+        default: return &render<double, double, double, double, true>;
     }
 }
 

@@ -155,7 +155,7 @@ def test_guiding_record_and_step_match_oracle(oracle, dihedral):
         sc_, sp_ = g.sampling()
         oc_, op_, _ = og.get(0)
         np.testing.assert_array_equal(sc_, oc_)
-        np.testing.assert_array_equal(sp_, op_)
+        np.testing.assert_allclose(sp_, op_, rtol=1e-6)
 
 
 def test_path_tracer_run_doubling_schedule(oracle, dihedral):
