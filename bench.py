@@ -6,8 +6,9 @@ path guiding on: 4 training passes of 2^20 paths, then the measured frame of 2^2
 Inputs (mesh BVH, scene, guide) are resident on the device when timing starts.
 
 Prints one JSON line: paths/s of the whole step (all passes), frames/s, p50/p99 frame
-latency, the roofline of the dominant kernel (the final pass's path kernel: algorithmic BVH
-node + triangle bytes from deterministic traversal counters ÷ its HIP-event duration), the
+latency, the roofline of the dominant kernel (the closest-hit traversal of the wavefront
+pipeline: algorithmic BVH node + triangle bytes from deterministic traversal counters ÷ its
+HIP-event duration), the
 IF relative L2 of the frame against the FP64 oracle on a bounded sample with the same trained
 guide, and the oracle's FP32 restatement timed on the host cores (cpu_baseline).
 """
@@ -51,6 +52,42 @@ def workload(name):
     else:
         raise SystemExit(f"unknown workload {name}")
     return mesh, scene, knobs, fc, sched, desc
+
+
+def roofline(pipeline, acc, cnt, steps):
+    """Roofline of the dominant kernel. Algorithmic bytes = BVH node bytes + triangle bytes the
+    traversal must read (deterministic counters of one step) per launch; its duration is the
+    HIP-event time around its launches on the backend's stream, averaged over the timed steps."""
+    if pipeline == 1:
+        # wavefront: the closest-hit traversal (one launch per bounce) dominates
+        n = cnt["nodes_visited"] - cnt["nodes_shadow"]
+        t = cnt["tris_tested"] - cnt["tris_shadow"]
+        launches = cnt["closest_launches"]
+        kms = acc["closest_ms"] / acc["closest_launches"]
+        kernel, sym = "k_wf_trav<false,false> (closest-hit traversal, one launch per bounce)", "k_wf_trav_closest"
+        rays = cnt["rays_closest"]
+        sh_bytes = (cnt["nodes_shadow"] * NODE_BYTES + cnt["tris_shadow"] * TRI_BYTES) / cnt["shadow_launches"]
+        sh_ms = acc["shadow_ms"] / acc["shadow_launches"]
+        second = {"kernel": "k_wf_shadow<false> (packet any-hit traversal)", "kernel_ms": sh_ms,
+                  "algorithmic_bytes": sh_bytes, "achieved": sh_bytes / (sh_ms * 1e-3) / 1e9,
+                  "rays_shadow": cnt["rays_shadow"], "nodes_visited": cnt["nodes_shadow"],
+                  "tris_tested": cnt["tris_shadow"]}
+    else:
+        n, t = cnt["nodes_visited"], cnt["tris_tested"]
+        launches = cnt["launches"]
+        kms = acc["trace_ms"] / acc["launches"]
+        kernel, sym = "k_trace<false> (fused path kernel)", "k_trace"
+        rays = cnt["rays_closest"] + cnt["rays_shadow"]
+        second = None
+    alg = (n * NODE_BYTES + t * TRI_BYTES) / launches
+    achieved = alg / (kms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "kernel_symbol": sym,
+           "kernel_ms": kms, "launches_per_step": launches, "algorithmic_bytes": alg,
+           "nodes_visited": n, "tris_tested": t, "rays": rays, "events": cnt["events"]}
+    if second:
+        out["shadow_kernel"] = second
+    return out
 
 
 def run_step(backend, pt, scene, sched):
@@ -117,6 +154,21 @@ def accuracy_check(backend, pt, mesh, scene, knobs, fc, n, threads):
             "hit_paths_compared": rec}
 
 
+def copy_probe(mib):
+    """A plain copy kernel of known size: MIB MiB read + MIB MiB written per dispatch. Under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it gives the factor between the counters' KiB and
+    the bytes moved, which scales the path kernels' counter readings into HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_dispatch": n * 4, "dispatches": 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,7 +179,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--pipeline", type=int, default=1, help="0 fused path kernel, 1 wavefront stages")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 4 plain device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the integrator has no multi-GPU path in this build "
               "(paths shard by Halton index; see the design notes). Run with --gpus 1.", file=sys.stderr)
@@ -146,33 +203,30 @@ def main():
     for _ in range(args.warmup):
         run_step(backend, pt, scene, sched)
     lat = []
-    kernel_ms = 0.0
-    launches = 0
+    acc = {"trace_ms": 0.0, "launches": 0, "closest_ms": 0.0, "closest_launches": 0, "shadow_ms": 0.0,
+           "shadow_launches": 0}
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t = time.perf_counter()
         run_step(backend, pt, scene, sched)
         lat.append(time.perf_counter() - t)
         st = backend.stats()  # HIP events on the backend's stream, summed over the step's passes
-        kernel_ms += st["trace_ms"]
-        launches += st["launches"]
+        for k in acc:
+            acc[k] += st[k]
     elapsed = time.perf_counter() - t_start
     passes, train, final = sched
     paths_per_step = passes * train + final
     value = paths_per_step * args.steps / elapsed
 
-    # deterministic traversal counters of one step (separate, untimed kernel instance)
+    # deterministic traversal counters of one step (separate, untimed instance)
     backend.set_option(OPT_COUNTERS, 1)
     run_step(backend, pt, scene, sched)
     cnt = backend.stats()
     backend.set_option(OPT_COUNTERS, 0)
-    alg_bytes = (cnt["nodes_visited"] * NODE_BYTES + cnt["tris_tested"] * TRI_BYTES) / cnt["launches"]
-    kms = kernel_ms / launches  # average path-kernel launch
-    achieved = alg_bytes / (kms * 1e-3) / 1e9
-    traffic = None
+    roof = roofline(args.pipeline, acc, cnt, args.steps)
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        roof["traffic"] = json.load(open(tpath)).get(roof["kernel_symbol"])
 
     out = {
         "metric": METRIC, "value": value, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
@@ -185,14 +239,8 @@ def main():
         "frames_per_s": args.steps / elapsed,
         "frame_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "pipeline": ["fused path kernel", "wavefront"][args.pipeline],
-        "kernel_paths_per_s": paths_per_step * args.steps / (kernel_ms * 1e-3),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_trace<false> (path kernel), average over the step's launches",
-                     "kernel_ms": kms, "launches_per_step": launches // args.steps,
-                     "algorithmic_bytes": alg_bytes,
-                     "nodes_visited": cnt["nodes_visited"], "tris_tested": cnt["tris_tested"],
-                     "rays_closest": cnt["rays_closest"], "rays_shadow": cnt["rays_shadow"], "events": cnt["events"]},
+        "kernel_paths_per_s": paths_per_step * args.steps / (acc["trace_ms"] * 1e-3),
+        "roofline": roof,
         "bvh_build_ms": bvh_ms, "setup_s": setup_s,
     }
     if not args.no_check:

@@ -96,7 +96,7 @@ struct hsr_backend {
     int64_t optRecord = 0;
     int64_t optPathsPerLaunch = int64_t(1) << 24;
     int32_t optPhaseMode = 0;
-    int32_t optPipeline = 0;
+    int32_t optPipeline = 1;
     int32_t maxGrid = 0;
     // wavefront buffers
     void *d_wf = nullptr;
@@ -104,6 +104,7 @@ struct hsr_backend {
     int64_t wfCap = int64_t(1) << 22;  // paths per wavefront batch
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> evTrav;  // pairs around traversal launches (wavefront)
     hsr_run_stats stats{};
 };
 
@@ -206,6 +207,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     if (b->d_wf) (void)hipFree(b->d_wf);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : b->evTrav) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -429,8 +431,16 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         HSR_HIP(hipEventRecord(b->ev[0], b->stream));
         if (first) HSR_HIP(hipEventRecord(b->ev[2], b->stream));
         first = false;
+        const size_t nev = 4 * (size_t)(knobs->maxDepth + 1);
+        while (b->evTrav.size() < nev) {
+            hipEvent_t e;
+            HSR_HIP(hipEventCreate(&e));
+            b->evTrav.push_back(e);
+        }
+        int nClosest = 0, nShadow = 0;
+        hipEvent_t *evC = b->evTrav.data(), *evS = b->evTrav.data() + nev / 2;
         HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
-                                      b->stream));
+                                      b->stream, evC, &nClosest, evS, &nShadow));
         HSR_HIP(hipEventRecord(b->ev[1], b->stream));
         HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, M, b->d_stage_sum, b->d_stage_comp, S, b->d_frame_sum,
                                    b->d_frame_comp, b->stream));
@@ -442,6 +452,18 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         trace_ms += ms1;
         reduce_ms += ms2;
         launches += 1;
+        for (int q = 0; q < nClosest; ++q) {
+            float m = 0.0f;
+            HSR_HIP(hipEventElapsedTime(&m, evC[2 * q], evC[2 * q + 1]));
+            b->stats.closest_ms += m;
+        }
+        for (int q = 0; q < nShadow; ++q) {
+            float m = 0.0f;
+            HSR_HIP(hipEventElapsedTime(&m, evS[2 * q], evS[2 * q + 1]));
+            b->stats.shadow_ms += m;
+        }
+        b->stats.closest_launches += nClosest;
+        b->stats.shadow_launches += nShadow;
         done += n;
     }
     for (int64_t done = 0; done < budget && b->optPipeline == 0;) {
@@ -508,8 +530,10 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     if (b->optCounters) {
         unsigned long long c[8];
         HSR_HIP(hipMemcpy(c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost));
-        b->stats.nodes_visited += (int64_t)c[0];
-        b->stats.tris_tested += (int64_t)c[1];
+        b->stats.nodes_visited += (int64_t)(c[0] + c[5]);
+        b->stats.tris_tested += (int64_t)(c[1] + c[6]);
+        b->stats.nodes_shadow += (int64_t)c[5];
+        b->stats.tris_shadow += (int64_t)c[6];
         b->stats.rays_closest += (int64_t)c[2];
         b->stats.rays_shadow += (int64_t)c[3];
         b->stats.events += (int64_t)c[4];

@@ -30,6 +30,7 @@ enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
 constexpr int kWfBlock = 256;
 constexpr int kTravSteps = 16;   // node visits between refills of idle lanes
 constexpr int kRefillMin = 16;   // idle lanes that trigger a refill
+constexpr int kPacket = 4;       // shadow rays (receivers of one path) traced together per lane
 
 __device__ inline void load_state(const WfArgs &w, int64_t i, PathState &p) {
     const double *d = w.sd;
@@ -254,6 +255,159 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int
     }
 }
 
+// Any-hit traversal of up to kPacket shadow rays that share their origin (one path's vertex,
+// several receivers): a node is entered when any still-unoccluded ray hits one of its child
+// boxes and every leaf triangle is tested against each such ray, so each ray's outcome is its
+// own exact any-hit result while the node fetches are shared.
+struct Packet {
+    f3 o;
+    f3 d[kPacket], invd[kPacket];
+    float tfar[kPacket];
+    uint32_t alive;  // rays not yet occluded
+    int32_t node;
+};
+
+__device__ inline bool slab_diff(float dlx, float dhx, float dly, float dhy, float dlz, float dhz, f3 invd,
+                                 float tfar, float &tentry) {
+    const float ax = dlx * invd.x, bx = dhx * invd.x;
+    const float ay = dly * invd.y, by = dhy * invd.y;
+    const float az = dlz * invd.z, bz = dhz * invd.z;
+    const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), kEps));
+    float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    t1 = t1 * 1.0000004f;
+    tentry = t0;
+    return t0 <= t1 && t0 <= tfar;
+}
+
+template <bool kCount>
+__device__ inline void packet_run(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, Packet &pk,
+                                  Stack &st, int steps, uint32_t &cn, uint32_t &ct) {
+    for (int s = 0; s < steps && pk.node != kDone; ++s) {
+        if (pk.node >= 0) {
+            if (kCount) ++cn;
+            const float4 a = nodes[4 * pk.node + 0];
+            const float4 b = nodes[4 * pk.node + 1];
+            const float4 c = nodes[4 * pk.node + 2];
+            const int4 ch = __builtin_bit_cast(int4, nodes[4 * pk.node + 3]);
+            const float a0 = a.x - pk.o.x, a1 = a.y - pk.o.x, a2 = a.z - pk.o.y, a3 = a.w - pk.o.y;
+            const float b0 = b.x - pk.o.x, b1 = b.y - pk.o.x, b2 = b.z - pk.o.y, b3 = b.w - pk.o.y;
+            const float c0 = c.x - pk.o.z, c1 = c.y - pk.o.z, c2 = c.z - pk.o.z, c3 = c.w - pk.o.z;
+            bool h0 = false, h1 = false;
+            float te0 = __builtin_inff(), te1 = __builtin_inff();
+#pragma unroll
+            for (int r = 0; r < kPacket; ++r) {
+                if (pk.alive & (1u << r)) {
+                    float e0, e1;
+                    if (slab_diff(a0, a1, a2, a3, c0, c1, pk.invd[r], pk.tfar[r], e0)) { h0 = true; te0 = fminf(te0, e0); }
+                    if (slab_diff(b0, b1, b2, b3, c2, c3, pk.invd[r], pk.tfar[r], e1)) { h1 = true; te1 = fminf(te1, e1); }
+                }
+            }
+            if (h0 && h1) {
+                const bool first0 = te0 <= te1;
+                st.push(first0 ? ch.y : ch.x);
+                pk.node = first0 ? ch.x : ch.y;
+            } else if (h0) {
+                pk.node = ch.x;
+            } else if (h1) {
+                pk.node = ch.y;
+            } else {
+                pk.node = st.sp ? st.pop() : kDone;
+            }
+        } else {
+            int first, count;
+            leaf_decode(pk.node, first, count);
+            for (int i = first; i < first + count; ++i) {
+                if (kCount) ++ct;
+                const float4 p = tris[3 * i + 0];
+                const float4 e1 = tris[3 * i + 1];
+                const float4 e2 = tris[3 * i + 2];
+#pragma unroll
+                for (int r = 0; r < kPacket; ++r) {
+                    if ((pk.alive & (1u << r)) &&
+                        tri_intersect({p.x, p.y, p.z}, {e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, pk.o, pk.d[r], kEps,
+                                      pk.tfar[r]) < __builtin_inff())
+                        pk.alive &= ~(1u << r);
+                }
+            }
+            pk.node = (pk.alive && st.sp) ? st.pop() : kDone;
+        }
+    }
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, int depth) {
+    __shared__ int32_t lds_stack[kStackShort * kWfBlock];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    Stack st;
+    st.lds = lds_stack + tid;
+    st.sp = 0;
+    WfCounts *cnt = &w.cnt[depth];
+    const unsigned int n = cnt->n_shd;
+    unsigned int *head = &cnt->head_shd;
+    const int nrx = min(a.rxCount, a.C);
+    uint32_t cn = 0, ct = 0, cr = 0;
+    Packet pk;
+    int64_t slot0 = 0;
+    int rays = 0;
+    bool has = false, exhausted = false;
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        if (!exhausted && __popcll(idle) >= kRefillMin) {
+            const int leader = __builtin_ctzll(idle);
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
+            base = __shfl(base, leader, 64);
+            if (base + (unsigned int)__popcll(idle) >= n) exhausted = true;
+            const unsigned int my = base + (unsigned int)__popcll(idle & ((1ull << lane) - 1ull));
+            if (!has && my < n) {
+                const int32_t e = w.q_shd[my];
+                const int64_t i = e >> 2;
+                const int c0 = (e & 3) * kPacket;
+                rays = min(kPacket, nrx - c0);
+                slot0 = i * a.C + c0;
+                pk.alive = 0;
+#pragma unroll
+                for (int r = 0; r < kPacket; ++r) {
+                    if (r < rays) {
+                        const float4 o = w.s_o[slot0 + r];
+                        const float4 d = w.s_d[slot0 + r];
+                        pk.o = {o.x, o.y, o.z};
+                        pk.d[r] = {d.x, d.y, d.z};
+                        pk.invd[r] = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+                        pk.tfar[r] = o.w;
+                        pk.alive |= 1u << r;
+                    } else {
+                        pk.d[r] = {0.0f, 0.0f, 1.0f};
+                        pk.invd[r] = {0.0f, 0.0f, 1.0f};
+                        pk.tfar[r] = 0.0f;
+                    }
+                }
+                pk.node = 0;
+                st.sp = 0;
+                has = true;
+                if (kCount) cr += rays;
+            }
+        }
+        if (!__any(has)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (has) {
+            packet_run<kCount>(a.nodes, a.tris, pk, st, kTravSteps, cn, ct);
+            if (pk.node == kDone) {
+                for (int r = 0; r < rays; ++r) w.s_vis[slot0 + r] = (pk.alive & (1u << r)) ? 2 : 1;
+                has = false;
+            }
+        }
+    }
+    if (kCount) {
+        atomicAdd(&a.counters[5], (unsigned long long)cn);
+        atomicAdd(&a.counters[6], (unsigned long long)ct);
+        atomicAdd(&a.counters[3], (unsigned long long)cr);
+    }
+}
+
 // ------------------------------------------------------------------------ hit
 __global__ __launch_bounds__(kWfBlock) void k_wf_hit(TraceArgs a, WfArgs w, int depth) {
     const int lane = threadIdx.x & 63;
@@ -320,8 +474,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_nee(TraceArgs a, WfArgs w, int 
             w.s_st[j] = (int8_t)st;
             w.s_vis[j] = 0;
         }
-        const int64_t q = wave_append(&cnt->n_shd, test, lane);
-        if (test) w.q_shd[q] = (int32_t)j;
+        // one shadow packet per four receivers of a path: entry = path * 4 + group
+        const bool lead = test && (k % nrx) % kPacket == 0;
+        const int64_t q = wave_append(&cnt->n_shd, lead, lane);
+        if (lead) w.q_shd[q] = (int32_t)((j / a.C) * 4 + (int32_t)((k % nrx) / kPacket));
     }
 }
 
@@ -464,7 +620,10 @@ int wavefront_trav_blocks_per_cu(bool count) {
 }
 
 hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
-                            double2 *partials, bool count, hipStream_t s) {
+                            double2 *partials, bool count, hipStream_t s, hipEvent_t *evClosest, int *nClosest,
+                            hipEvent_t *evShadow, int *nShadow) {
+    *nClosest = 0;
+    *nShadow = 0;
     (void)hipGetLastError();
     const int maxDepth = a.maxDepth;
     hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 63) / 64), dim3(64), 0, s, w.cnt, maxDepth + 2);
@@ -474,18 +633,24 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     for (int k = 0; k <= maxDepth; ++k) {
         if (k >= 1 || !a.onlyIndirect) {
             if (!a.poDiffraction) {
+                (void)hipEventRecord(evShadow[2 * *nShadow], s);
                 if (count)
-                    hipLaunchKernelGGL((k_wf_trav<true, true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                    hipLaunchKernelGGL((k_wf_shadow<true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
                 else
-                    hipLaunchKernelGGL((k_wf_trav<true, false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                    hipLaunchKernelGGL((k_wf_shadow<false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                (void)hipEventRecord(evShadow[2 * *nShadow + 1], s);
+                ++*nShadow;
             }
             hipLaunchKernelGGL(k_wf_gather, dim3(gridGather), dim3(kWfBlock), glds, s, a, w, k, partials, count);
         }
         if (k < maxDepth) {
+            (void)hipEventRecord(evClosest[2 * *nClosest], s);
             if (count)
                 hipLaunchKernelGGL((k_wf_trav<false, true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
             else
                 hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+            (void)hipEventRecord(evClosest[2 * *nClosest + 1], s);
+            ++*nClosest;
             hipLaunchKernelGGL(k_wf_hit, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k);
             hipLaunchKernelGGL(k_wf_nee, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k + 1);
         }

@@ -167,7 +167,7 @@ class HipBackend:
                                    I.ctypes.data_as(C.POINTER(C.c_int32)), I.shape[0], device, C.byref(h)))
         self._h = h
         self.device = device
-        self.pipeline = 0
+        self.pipeline = 1  # the library default: wavefront stages
         self.integrator = integrator
         self.frame_config: Optional[FrameConfig] = None
         if integrator is not None:

@@ -65,7 +65,9 @@ class c_run_stats(C.Structure):
     _fields_ = [("trace_ms", C.c_double), ("reduce_ms", C.c_double), ("total_ms", C.c_double),
                 ("paths", C.c_int64), ("launches", C.c_int64), ("nodes_visited", C.c_int64),
                 ("tris_tested", C.c_int64), ("rays_closest", C.c_int64),
-                ("rays_shadow", C.c_int64), ("events", C.c_int64), ("bvh_build_ms", C.c_double)]
+                ("rays_shadow", C.c_int64), ("events", C.c_int64), ("bvh_build_ms", C.c_double),
+                ("closest_ms", C.c_double), ("shadow_ms", C.c_double), ("closest_launches", C.c_int64),
+                ("shadow_launches", C.c_int64), ("nodes_shadow", C.c_int64), ("tris_shadow", C.c_int64)]
 
 
 class c_device_info(C.Structure):

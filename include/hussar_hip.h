@@ -203,6 +203,14 @@ typedef struct hsr_run_stats {
     int64_t rays_shadow;    /* shadow rays traced (counters enabled) */
     int64_t events;         /* frame splat events (counters enabled) */
     double bvh_build_ms;    /* host BVH build time at create */
+    /* wavefront pipeline: HIP-event time and launches of the two traversal kernels, and the
+     * share of nodes / triangles of the shadow (any-hit) traversal (counters enabled) */
+    double closest_ms;
+    double shadow_ms;
+    int64_t closest_launches;
+    int64_t shadow_launches;
+    int64_t nodes_shadow;
+    int64_t tris_shadow;
 } hsr_run_stats;
 
 hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
@@ -214,10 +222,10 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        id hit at every depth and the shadow-ray outcome of every NEE;
  *                        read back with hsr_hip_get_hit_record.
  *  HSR_OPT_PATHS_PER_LAUNCH  maximum paths per kernel launch (interrupt granularity).
- *  HSR_OPT_PIPELINE      0 = fused path kernel (default), 1 = wavefront stages (path
- *                        state in device memory, persistent traversal kernels).
- *  HSR_OPT_PHASE_MODE    0 = FP64 phase (default), 1 = FP32 phase as in the reference,
- *                        2 = double-float (two-float) phase. */
+ *  HSR_OPT_PIPELINE      0 = fused path kernel, 1 = wavefront stages (default: path
+ *                        state in device memory, persistent traversal kernels). Both
+ *                        give the same hits, NEE outcomes and frame up to summation order.
+ *  HSR_OPT_PHASE_MODE    0 = FP64 phase (default), 1 = FP32 phase as in the reference. */
 #define HSR_OPT_COUNTERS 1
 #define HSR_OPT_RECORD_HITS 2
 #define HSR_OPT_PATHS_PER_LAUNCH 3
