@@ -27,11 +27,11 @@ SIGNATURES = {
     "hsr_hip_configure_frame": (C.c_int, [_vp, _P(c_frame_config)]),
     "hsr_hip_clear_frame": (C.c_int, [_vp]),
     "hsr_hip_run": (C.c_int, [_vp, _P(c_scene), _P(c_knobs), C.c_int64, C.c_int64, C.c_float, C.c_int32, _vp,
-                              _P(C.c_int32)]),
+                              _P(C.c_uint8)]),
     "hsr_hip_fetch_frame": (C.c_int, [_vp, _P(C.c_float), _P(C.c_double), _P(C.c_double)]),
-    "hsr_path_tracer_run": (C.c_int, [_vp, _P(c_scene), _P(c_knobs), C.c_int64, _vp, _P(C.c_int32)]),
+    "hsr_path_tracer_run": (C.c_int, [_vp, _P(c_scene), _P(c_knobs), C.c_int64, _vp, _P(C.c_uint8)]),
     "hsr_path_tracer_run_schedule": (C.c_int, [_vp, _P(c_scene), _P(c_knobs), C.c_int32, C.c_int64, C.c_int64,
-                                               _vp, _P(C.c_int32)]),
+                                               _vp, _P(C.c_uint8)]),
     "hsr_guide_create": (C.c_int, [_vp, C.c_float, C.c_float, C.c_int32, _P(_vp)]),
     "hsr_guide_destroy": (None, [_vp]),
     "hsr_guide_reset": (C.c_int, [_vp]),

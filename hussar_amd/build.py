@@ -45,7 +45,26 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    build_cpp_front_end()
     return LIB
+
+
+CPP_DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "drop_in_dihedral.cpp")
+CPP_DEMO = os.path.join(ROOT, "tests", "cpp", "drop_in_dihedral")
+
+
+def build_cpp_front_end() -> str:
+    """Compiles the dihedral sweep written against include/hussar_hip.hpp (the header-only
+    C++ front end) with g++ and links it to the library through an $ORIGIN-relative rpath,
+    so that the binary finds the in-tree library wherever the tree is copied."""
+    tmp = CPP_DEMO + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+           CPP_DEMO_SRC, "-L", HERE, "-lhussar_hip", "-Wl,-rpath,$ORIGIN/../../hussar_amd", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed on the C++ front end:\n" + r.stderr)
+    os.replace(tmp, CPP_DEMO)
+    return CPP_DEMO
 
 
 if __name__ == "__main__":

@@ -298,7 +298,7 @@ static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
 // schedule call instead of replacing them.
 static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
                            int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
-                           const volatile int32_t *interruptFlag, bool accumulate) {
+                           const volatile uint8_t *interruptFlag, bool accumulate) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured (hsr_hip_configure_frame)");
     if (budget < 0 || sampleIndexOffset < 0) return fail(HSR_ERR_INVALID, "negative budget or offset");
@@ -543,7 +543,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
 
 hsr_status hsr_hip_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
                        int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
-                       const volatile int32_t *interruptFlag) {
+                       const volatile uint8_t *interruptFlag) {
     return run_impl(b, scene, knobs, sampleIndexOffset, budget, sampleWeight, isFinalIteration, guide,
                     interruptFlag, false);
 }
@@ -703,7 +703,7 @@ hsr_status hsr_guide_get_training(hsr_guide *g, int32_t *child, double *stat, in
 
 // ---------------------------------------------------------------- PathTracer::run
 hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t samples,
-                               hsr_guide *guide, const volatile int32_t *interruptFlag) {
+                               hsr_guide *guide, const volatile uint8_t *interruptFlag) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     hsr_status s = hsr_hip_clear_frame(b);  // setup() + clearFrame(), path.h:173-174
     if (s != HSR_OK) return s;
@@ -740,7 +740,7 @@ hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr
 
 hsr_status hsr_path_tracer_run_schedule(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs,
                                         int32_t trainingPasses, int64_t trainingPaths, int64_t finalPaths,
-                                        hsr_guide *guide, const volatile int32_t *interruptFlag) {
+                                        hsr_guide *guide, const volatile uint8_t *interruptFlag) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (trainingPasses < 0 || trainingPaths < 0 || finalPaths < 0) return fail(HSR_ERR_INVALID, "negative schedule");
     hsr_status s;

@@ -151,9 +151,11 @@ class PathTracer(Knobs):
 def _flag_ptr(flag):
     if flag is None:
         return None
-    if isinstance(flag, C.c_int32):
+    if isinstance(flag, C.c_uint8):
         return C.pointer(flag)
-    raise TypeError("interruptFlag must be a ctypes.c_int32 (set .value = 1 to interrupt)")
+    if isinstance(flag, C.c_bool):
+        return C.cast(C.pointer(flag), C.POINTER(C.c_uint8))
+    raise TypeError("interruptFlag must be a ctypes.c_bool or c_uint8 (set .value = 1 to interrupt)")
 
 
 class HipBackend:

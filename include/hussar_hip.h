@@ -132,12 +132,13 @@ hsr_status hsr_hip_clear_frame(hsr_backend *backend);
  * work of PathTracer::sample (path.h:236-451) for Halton indices
  * sampleIndexOffset .. sampleIndexOffset+budget-1, accumulated into the frame.
  * Blocking, like the reference. guide may be NULL when knobs->doGuiding is 0.
- * interruptFlag (may be NULL) is polled between device launches; when it reads non-zero
+ * interruptFlag (may be NULL) is the reference's `bool *` (one byte), polled between
+ * device launches; when it reads non-zero
  * the call returns HSR_ERR_INTERRUPTED, with a partial frame (cpu.h:25 semantics). */
 hsr_status hsr_hip_run(hsr_backend *backend, const hsr_scene *scene, const hsr_knobs *knobs,
                        int64_t sampleIndexOffset, int64_t budget, float sampleWeight,
                        int32_t isFinalIteration, hsr_guide *guide,
-                       const volatile int32_t *interruptFlag);
+                       const volatile uint8_t *interruptFlag);
 
 /* PathTracer::fetchFrame (path.h:453-455): frame / totalWeight.
  * out_c64: float[2*sampleCount] (complex64, the reference's output type) or NULL.
@@ -151,7 +152,7 @@ hsr_status hsr_hip_fetch_frame(hsr_backend *backend, float *out_c64, double *out
  * have been configured. guide may be NULL when knobs->doGuiding is 0. */
 hsr_status hsr_path_tracer_run(hsr_backend *backend, const hsr_scene *scene,
                                const hsr_knobs *knobs, int64_t samples, hsr_guide *guide,
-                               const volatile int32_t *interruptFlag);
+                               const volatile uint8_t *interruptFlag);
 
 /* An explicit training schedule (an extension): trainingPasses passes of trainingPaths
  * paths each (each pass restarts at Halton index 0, records into the guide and is
@@ -161,7 +162,7 @@ hsr_status hsr_path_tracer_run_schedule(hsr_backend *backend, const hsr_scene *s
                                         const hsr_knobs *knobs, int32_t trainingPasses,
                                         int64_t trainingPaths, int64_t finalPaths,
                                         hsr_guide *guide,
-                                        const volatile int32_t *interruptFlag);
+                                        const volatile uint8_t *interruptFlag);
 
 /* GuidingWrapper<BTree<2,...>> (path.h:17-131) with the settings PathTracer::setup
  * assigns (path.h:229-232: uniformProb 0.1, splitThreshold 0.005, secondMoment true).

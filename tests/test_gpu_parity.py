@@ -252,7 +252,7 @@ def test_errors_and_interrupt(dihedral):
         b.run_raw(sc, H.Knobs(correctPhase=True, doGuiding=False), 0, 10)
     with pytest.raises(HsrError, match="INVALID"):
         b.run_raw(sc, H.Knobs(doGuiding=True), 0, 10)  # guiding without a guide
-    flag = C.c_int32(1)
+    flag = C.c_bool(True)
     with pytest.raises(HsrError, match="INTERRUPTED"):
         b.run_raw(sc, H.Knobs(maxDepth=3, doGuiding=False), 0, 1 << 20, interruptFlag=flag)
     bad = H.Scene(rfConfig=sc.rfConfig, tx=sc.tx, rx=sc.rx, rxs=[sc.rx, sc.rx])
