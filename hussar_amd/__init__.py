@@ -7,8 +7,8 @@ kernels behind the C ABI of include/hussar_hip.h (hussar_amd/libhussar_hip.so).
 from .types import (FrameConfig, Knobs, NFAntenna, RFConfig, Scene, TriangleMesh, SPEED_OF_LIGHT,
                     GHz, MHz, kHz, Hz, ms, us, ns, mm, cm, rotation_y)
 from .integrator import Guide, HipBackend, PathTracer, device_info, knobs_default
-from . import scenes
+from . import io, scenes
 
 __all__ = ["FrameConfig", "Knobs", "NFAntenna", "RFConfig", "Scene", "TriangleMesh", "SPEED_OF_LIGHT",
-           "Guide", "HipBackend", "PathTracer", "device_info", "knobs_default", "scenes", "rotation_y",
+           "Guide", "HipBackend", "PathTracer", "device_info", "knobs_default", "io", "scenes", "rotation_y",
            "GHz", "MHz", "kHz", "Hz", "ms", "us", "ns", "mm", "cm"]

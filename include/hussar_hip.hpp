@@ -33,7 +33,10 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <fstream>
+#include <iostream>
 #include <ostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -206,6 +209,69 @@ inline void writeFrameToFile(std::ostream &file, const Frame &frame) {
                (std::streamsize)(frame.data.size() * sizeof(std::complex<float>)));
     file.flush();
 }
+
+inline void saveFrame(const Frame &frame, const std::string &path) {
+    std::ofstream file(path + ".SIM", std::ios::binary);
+    writeFrameToFile(file, frame);
+}
+
+// hussar::WavefrontFile (io/wavefront.h:12-64): `v x y z` and `f a b c` lines only; face
+// indices are 1-based and relative to this file's vertices; a face uses its first three
+// indices; `#` comment lines, `s` lines and empty lines are skipped, other commands are
+// reported and skipped. A face token that is not a plain integer (v/vt/vn syntax) throws
+// instead of stalling the reader. Works with any mesh whose vertexBuffer holds 3 floats
+// (operator[]) and whose indexBuffer holds {v0, v1, v2}.
+class WavefrontFile {
+public:
+    explicit WavefrontFile(const std::string &path) : m_path(path) {}
+
+    template <class Mesh>
+    void read(Mesh &mesh) {
+        std::ifstream in(m_path);
+        if (!in) throw Error(HSR_ERR_INVALID, "cannot open wavefront file " + m_path);
+        const int offset = (int)mesh.vertexBuffer.size();
+        std::string line;
+        for (int lineno = 1; std::getline(in, line); ++lineno) {
+            if (!line.empty() && line[0] == '#') continue;
+            std::istringstream ls(line);
+            std::string cmd;
+            if (!(ls >> cmd)) continue;
+            if (cmd == "v") {
+                typename std::decay<decltype(mesh.vertexBuffer[0])>::type v{};
+                for (int i = 0; i < 3; ++i) {
+                    float x = 0.0f;
+                    if (!(ls >> x)) throw Error(HSR_ERR_INVALID, where(lineno) + "a vertex needs three numbers");
+                    v[i] = x;
+                }
+                mesh.vertexBuffer.push_back(v);
+            } else if (cmd == "f") {
+                int idx[3];
+                for (int i = 0; i < 3; ++i) {
+                    std::string tok;
+                    if (!(ls >> tok)) throw Error(HSR_ERR_INVALID, where(lineno) + "a face needs three indices");
+                    size_t used = 0;
+                    try {
+                        idx[i] = std::stoi(tok, &used);
+                    } catch (const std::exception &) {
+                        used = 0;
+                    }
+                    if (used == 0 || used != tok.size())
+                        throw Error(HSR_ERR_INVALID,
+                                    where(lineno) + "face indices must be plain integers (no v/vt/vn syntax)");
+                    idx[i] += offset - 1;
+                }
+                mesh.indexBuffer.push_back({idx[0], idx[1], idx[2]});
+            } else if (cmd == "s") {
+            } else {
+                std::cout << "unsupported wavefront command: " << cmd << std::endl;
+            }
+        }
+    }
+
+private:
+    std::string where(int lineno) const { return m_path + ":" + std::to_string(lineno) + ": "; }
+    std::string m_path;
+};
 
 namespace detail {
 template <class V>

@@ -1,7 +1,8 @@
 // Compile check: the front end's Backend and PathTracer accept types shaped like the
 // reference's (Eigen-style vectors with operator[], matrices with operator()(r, c), a mesh of
 // packed float3 / int3, a Scene with rfConfig / rx / tx). Also prints the vertex and index
-// buffers of addQuad / addBox of the front end's own TriangleMesh for a comparison.
+// buffers of addQuad / addBox of the front end's own TriangleMesh, or, given file names, of
+// the Wavefront OBJ files read one after another into one mesh, for a comparison.
 #include "hussar_hip.hpp"
 
 #include <cstdio>
@@ -46,8 +47,19 @@ void instantiate(const refshape::Mesh &mesh, const refshape::Scene &scene) {
     backend.run(scene, 16, &stop);
 }
 
-int main() {
+int main(int argc, char **argv) {
     hussar::hip::TriangleMesh m;
+    if (argc > 1) {
+        try {
+            for (int a = 1; a < argc; ++a) hussar::hip::WavefrontFile(argv[a]).read(m);
+        } catch (const hussar::hip::Error &e) {
+            std::printf("error %s\n", e.what());
+            return 1;
+        }
+        for (const auto &v : m.vertexBuffer) std::printf("v %a %a %a\n", v[0], v[1], v[2]);
+        for (const auto &t : m.indexBuffer) std::printf("f %d %d %d\n", t.v0, t.v1, t.v2);
+        return 0;
+    }
     m.addQuad({0.5f, -1, 2}, {0, 0.25f, 0}, {0.125f, 0, 0});
     m.addBox({-1, 0, 0.5f}, {2, 0.75f, 3});
     for (const auto &v : m.vertexBuffer) std::printf("v %a %a %a\n", v[0], v[1], v[2]);

@@ -3,12 +3,14 @@
 the .SIM writer of examples/src/utils.h:29-40) over the C ABI.
 
 CPU: the header compiles with types shaped like the reference's Eigen-based ones, its mesh
-builders produce the same buffers as the Python types (mesh.h:30-54), and the demo binary
-built by hussar_amd/build.py exists and checks its arguments.
+builders produce the same buffers as the Python types (mesh.h:30-54), the C++ and Python
+Wavefront OBJ readers agree and follow io/wavefront.h:12-64, .SIM files round-trip, and the
+demo binary built by hussar_amd/build.py exists and checks its arguments.
 GPU: the demo's guided dihedral sweep (the reference PathTracer::run loop written in C++ and
 issued as hsr_hip_run / hsr_guide_step calls) gives the frames of the library's own
 schedule entry point (hsr_path_tracer_run) for the same scenes.
 """
+import io
 import os
 import subprocess
 
@@ -16,6 +18,7 @@ import numpy as np
 import pytest
 
 import hussar_amd as H
+from hussar_amd import io as hio
 from hussar_amd import scenes
 
 from conftest import ROOT
@@ -26,18 +29,70 @@ DEMO = os.path.join(ROOT, "tests", "cpp", "drop_in_dihedral")
 SHAPED = os.path.join(ROOT, "tests", "cpp", "reference_shaped_types.cpp")
 
 
-def test_header_with_reference_shaped_types(tmp_path):
-    exe = str(tmp_path / "shaped")
+@pytest.fixture(scope="module")
+def shaped(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("cpp") / "shaped")
     subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I", INCLUDE, SHAPED,
                     "-L", LIBDIR, "-lhussar_hip", f"-Wl,-rpath,{LIBDIR}", "-o", exe], check=True)
-    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines()
+    return exe
+
+
+def parse_mesh(out):
     verts = np.array([[float.fromhex(x) for x in l.split()[1:]] for l in out if l.startswith("v ")], np.float32)
     faces = np.array([[int(x) for x in l.split()[1:]] for l in out if l.startswith("f ")], np.int32)
+    return verts.reshape(-1, 3), faces.reshape(-1, 3)
+
+
+def test_header_with_reference_shaped_types(shaped):
+    out = subprocess.run([shaped], check=True, capture_output=True, text=True).stdout.splitlines()
+    verts, faces = parse_mesh(out)
     m = H.TriangleMesh()
     m.addQuad([0.5, -1, 2], [0, 0.25, 0], [0.125, 0, 0])
     m.addBox([-1, 0, 0.5], [2, 0.75, 3])
     assert np.array_equal(verts, m.vertexBuffer) and np.array_equal(faces, m.indexBuffer)
     assert out[-1] == "rf 1 2 3 4 5 6 rx 1"
+
+
+def test_obj_readers_agree_and_follow_the_reference_rules(shaped, tmp_path):
+    obj = os.path.join(ROOT, "tests", "golden", "two_plates.obj")
+    # two reads into one mesh: the second file's indices are offset by the first's vertices
+    r = subprocess.run([shaped, obj, obj], check=True, capture_output=True, text=True)
+    verts, faces = parse_mesh(r.stdout.splitlines())
+    assert r.stdout.count("unsupported wavefront command: o") == 2
+    log = io.StringIO()
+    m = hio.read_obj(obj, log=log)
+    hio.read_obj(obj, m, log=log)
+    assert log.getvalue().count("unsupported wavefront command: o") == 2
+    assert np.array_equal(verts, m.vertexBuffer) and np.array_equal(faces, m.indexBuffer)
+    assert m.triangle_count() == 8 and m.vertexBuffer.shape[0] == 16
+    assert faces[4].tolist() == [8, 9, 10]  # first face of the second read: 1 2 3 (+8 - 1)
+    # the triangles of one read are the dihedral's two plates (same corners, same planes)
+    d = scenes.dihedral_mesh()
+    tri = lambda V, I: sorted(tuple(sorted(map(tuple, V[t].tolist()))) for t in I)
+    assert tri(m.vertexBuffer, m.indexBuffer[:4]) == tri(d.vertexBuffer, d.indexBuffer)
+    bad = tmp_path / "bad.obj"
+    bad.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1/1 2/2 3/3\n")
+    with pytest.raises(ValueError, match="plain integers"):
+        hio.read_obj(str(bad))
+    r = subprocess.run([shaped, str(bad)], capture_output=True, text=True)
+    assert r.returncode == 1 and "plain integers" in r.stdout
+
+
+def test_sim_files_round_trip(tmp_path):
+    fc = H.FrameConfig(2, 8, 3)
+    rng = np.random.default_rng(5)
+    frames = [(rng.normal(size=fc.sampleCount()) + 1j * rng.normal(size=fc.sampleCount())).astype(np.complex64)
+              for _ in range(3)]
+    path = str(tmp_path / "sweep.SIM")
+    with open(path, "wb") as f:
+        for z in frames:
+            hio.write_frame(f, z)
+    back = hio.read_sim(path, fc)
+    assert back.shape == (3, 2, 8, 3)
+    assert np.array_equal(back.reshape(3, -1), np.stack(frames))
+    assert os.path.getsize(path) == 3 * fc.sampleCount() * 8
+    hio.save_frame(frames[0], str(tmp_path / "one"))
+    assert np.array_equal(np.fromfile(str(tmp_path / "one.SIM"), np.complex64), frames[0])
 
 
 def test_demo_binary_checks_arguments():
