@@ -24,6 +24,7 @@ SIGNATURES = {
     "hsr_knobs_default": (None, [_P(c_knobs)]),
     "hsr_hip_create": (C.c_int, [_P(C.c_float), C.c_int64, _P(C.c_int32), C.c_int64, C.c_int32, _P(_vp)]),
     "hsr_hip_destroy": (None, [_vp]),
+    "hsr_hip_set_velocities": (C.c_int, [_vp, _P(C.c_float), C.c_int64]),
     "hsr_hip_configure_frame": (C.c_int, [_vp, _P(c_frame_config)]),
     "hsr_hip_clear_frame": (C.c_int, [_vp]),
     "hsr_hip_run": (C.c_int, [_vp, _P(c_scene), _P(c_knobs), C.c_int64, C.c_int64, C.c_float, C.c_int32, _vp,
@@ -58,6 +59,9 @@ class HsrError(RuntimeError):
         self.status = status
 
 
+ABI_VERSION = 2  # HSR_ABI_VERSION of include/hussar_hip.h
+
+
 def lib():
     """Load the backend library (raises if it is missing: there is no CPU fallback)."""
     global _lib
@@ -70,6 +74,9 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        if L.hsr_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.hsr_abi_version()}, the bindings expect "
+                               f"{ABI_VERSION}: rebuild it with hussar_amd.build.build()")
         _lib = L
     return _lib
 

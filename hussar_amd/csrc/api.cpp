@@ -75,6 +75,8 @@ struct hsr_backend {
     int32_t nodeCount = 0, slotCount = 0, bvhDepth = 0;
     int64_t triangles = 0;
     double bvh_ms = 0.0;
+    std::vector<int32_t> slotIds;  // triangle id of every slot (leaf order)
+    float4 *d_vel = nullptr;       // per slot velocity, null for a static mesh
     // frame
     bool configured = false;
     hsr_frame_config fc{1, 0, 0};
@@ -184,6 +186,8 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     if ((s = dalloc(&b->d_tris, (size_t)bvh.slotCount * 3)) != HSR_OK) return s;
     HSR_HIP(hipMemcpy(b->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(float), hipMemcpyHostToDevice));
     HSR_HIP(hipMemcpy(b->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(float), hipMemcpyHostToDevice));
+    b->slotIds.resize(bvh.slotCount);
+    for (int32_t s2 = 0; s2 < bvh.slotCount; ++s2) std::memcpy(&b->slotIds[s2], &bvh.tris[12 * (size_t)s2 + 3], 4);
     if ((s = dalloc(&b->d_counters, 8)) != HSR_OK) return s;
     b->stats.bvh_build_ms = b->bvh_ms;
     *out = b.release();
@@ -204,6 +208,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_counters);
     dfree(b->d_recHits);
     dfree(b->d_recNee);
+    dfree(b->d_vel);
     if (b->d_wf) (void)hipFree(b->d_wf);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -212,13 +217,40 @@ void hsr_hip_destroy(hsr_backend *b) {
     delete b;
 }
 
+hsr_status hsr_hip_set_velocities(hsr_backend *b, const float *vel, int64_t triangleCount) {
+    if (!b) return fail(HSR_ERR_INVALID, "null backend");
+    HSR_HIP(hipSetDevice(b->device));
+    HSR_HIP(hipStreamSynchronize(b->stream));
+    if (!vel || triangleCount == 0) {
+        dfree(b->d_vel);
+        return HSR_OK;
+    }
+    if (triangleCount != b->triangles) return fail(HSR_ERR_INVALID, "one velocity per mesh triangle expected");
+    std::vector<float4> v(b->slotIds.size());
+    for (size_t s = 0; s < v.size(); ++s) {
+        const int32_t id = b->slotIds[s];
+        v[s] = (id >= 0 && id < b->triangles)
+                   ? make_float4(vel[3 * (int64_t)id], vel[3 * (int64_t)id + 1], vel[3 * (int64_t)id + 2], 0.0f)
+                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (!std::isfinite(v[s].x) || !std::isfinite(v[s].y) || !std::isfinite(v[s].z))
+            return fail(HSR_ERR_INVALID, "velocities must be finite");
+    }
+    if (!b->d_vel) {
+        hsr_status s;
+        if ((s = dalloc(&b->d_vel, v.size())) != HSR_OK) return s;
+    }
+    HSR_HIP(hipMemcpy(b->d_vel, v.data(), v.size() * sizeof(float4), hipMemcpyHostToDevice));
+    return HSR_OK;
+}
+
 hsr_status hsr_hip_configure_frame(hsr_backend *b, const hsr_frame_config *c) {
     if (!b || !c) return fail(HSR_ERR_INVALID, "null argument");
     if (c->chirpCount < 1 || c->samplesPerChirp < 1 || c->channelCount < 1 || c->channelCount > HSR_MAX_CHANNELS)
         return fail(HSR_ERR_INVALID, "frame dimensions out of range");
-    if (c->chirpCount != 1)
-        return fail(HSR_ERR_UNSUPPORTED, "only chirpCount == 1: the integrator has no motion (integrator.h:130)");
-    if (hsr::trace_lds_bytes(c->samplesPerChirp, c->channelCount) > 65536)
+    if ((int64_t)c->chirpCount * c->samplesPerChirp * c->channelCount > (int64_t(1) << 28))
+        return fail(HSR_ERR_INVALID, "frame too large");
+    // one chirp: the frame is accumulated in LDS tiles; more chirps: in device memory
+    if (c->chirpCount == 1 && hsr::trace_lds_bytes(c->samplesPerChirp, c->channelCount) > 65536)
         return fail(HSR_ERR_UNSUPPORTED, "frame tile exceeds 64 KiB of LDS (samples x channels too large)");
     HSR_HIP(hipSetDevice(b->device));
     const int64_t nd = 2LL * c->chirpCount * c->samplesPerChirp * c->channelCount;
@@ -288,6 +320,8 @@ static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
     a.rxCount = sc->rxCount;
     std::memcpy(a.tx.pos, sc->tx.position, sizeof(a.tx.pos));
     std::memcpy(a.tx.rot, sc->tx.rotation, sizeof(a.tx.rot));
+    for (int k = 0; k < 3; ++k) a.ego[k] = sc->velocity[k];
+    a.chirpFreq = 1.0 / ((double)rf.idleTime + (double)rf.rampTime);
     for (int i = 0; i < HSR_MAX_CHANNELS; ++i) {
         std::memcpy(a.rx[i].pos, sc->rx[i].position, sizeof(a.rx[i].pos));
         std::memcpy(a.rx[i].rot, sc->rx[i].rotation, sizeof(a.rx[i].rot));
@@ -311,6 +345,17 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     if (knobs->doGuiding && !guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");
     if (guide && guide->backend != b) return fail(HSR_ERR_INVALID, "guide belongs to another backend");
     if (!(scene->rfConfig.adcRate > 0.0f)) return fail(HSR_ERR_INVALID, "adcRate must be positive");
+    const int T = scene->txCount < 1 ? 1 : scene->txCount;
+    if (T > HSR_MAX_TX) return fail(HSR_ERR_INVALID, "txCount out of range");
+    if (T * scene->rxCount > b->fc.channelCount)
+        return fail(HSR_ERR_INVALID, "txCount * rxCount exceeds the frame's channels");
+    const bool motion = b->d_vel != nullptr || scene->velocity[0] != 0.0f || scene->velocity[1] != 0.0f ||
+                        scene->velocity[2] != 0.0f;
+    const int chirps = b->fc.chirpCount;
+    if ((motion || chirps > 1) && b->optPipeline == 0)
+        return fail(HSR_ERR_UNSUPPORTED, "motion and multi-chirp frames need the wavefront pipeline");
+    if (motion && !(scene->rfConfig.idleTime + scene->rfConfig.rampTime > 0.0f))
+        return fail(HSR_ERR_INVALID, "motion needs a positive chirp period (idleTime + rampTime)");
     HSR_HIP(hipSetDevice(b->device));
     if (guide && guide->dirty) {
         hsr_status s = guide_upload(guide);
@@ -318,7 +363,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     }
 
     const int N = b->fc.samplesPerChirp, C = b->fc.channelCount;
-    const int M = 2 * N * C;
+    const int MD = 2 * N * C;  // doubles of one chirp's frame (one LDS tile / partial)
     hsr::TraceArgs a;
     std::memset(&a, 0, sizeof(a));
     fill_scene(a, scene);
@@ -351,6 +396,10 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         a.guideOn = 1;
     }
     a.counters = b->d_counters;
+    a.M = chirps;
+    a.motion = motion;
+    a.vel = b->d_vel;
+    a.frameAcc = chirps > 1 ? b->d_frame_sum : nullptr;
 
     // grid: as many workgroups as fit at once, fewer for small budgets (>= 8 paths per lane)
     if (b->maxGrid == 0) {
@@ -384,6 +433,12 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     double trace_ms = 0.0, reduce_ms = 0.0;
     int64_t launches = 0;
     bool first = true;
+    for (int tx = 0; tx < T; ++tx) {
+    // MIMO: transmitter tx emits the same paths into virtual channels tx*rxCount + r
+    const hsr_antenna &txa = tx == 0 ? scene->tx : scene->txMore[tx - 1];
+    std::memcpy(a.tx.pos, txa.position, sizeof(a.tx.pos));
+    std::memcpy(a.tx.rot, txa.rotation, sizeof(a.tx.rot));
+    a.chBase = tx * scene->rxCount;
     for (int64_t done = 0; done < budget && b->optPipeline == 1;) {
         // wavefront batches: path state in HBM, all stage sizes on the device
         if (interruptFlag && *interruptFlag) return fail(HSR_ERR_INTERRUPTED, "interrupted");
@@ -411,7 +466,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         const int gridTrav = b->cu * hsr::wavefront_trav_blocks_per_cu(b->optCounters);
         const int gridWork = b->cu * 4;
         const int gridGather = b->cu * 2;
-        const size_t needP = (size_t)gridGather * M;
+        const size_t needP = (size_t)gridGather * MD;
         if (needP > b->partialCap) {
             dfree(b->d_partials);
             hsr_status s;
@@ -419,15 +474,15 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             b->partialCap = needP;
         }
         const int S = std::min(64, gridGather);
-        if ((size_t)S * M > b->stageCap) {
+        if ((size_t)S * MD > b->stageCap) {
             dfree(b->d_stage_sum);
             dfree(b->d_stage_comp);
             hsr_status s;
-            if ((s = dalloc(&b->d_stage_sum, (size_t)S * M)) != HSR_OK) return s;
-            if ((s = dalloc(&b->d_stage_comp, (size_t)S * M)) != HSR_OK) return s;
-            b->stageCap = (size_t)S * M;
+            if ((s = dalloc(&b->d_stage_sum, (size_t)S * MD)) != HSR_OK) return s;
+            if ((s = dalloc(&b->d_stage_comp, (size_t)S * MD)) != HSR_OK) return s;
+            b->stageCap = (size_t)S * MD;
         }
-        HSR_HIP(hsr::launch_zero(b->d_partials, (int64_t)gridGather * M, b->stream));
+        if (chirps == 1) HSR_HIP(hsr::launch_zero(b->d_partials, (int64_t)gridGather * MD, b->stream));
         HSR_HIP(hipEventRecord(b->ev[0], b->stream));
         if (first) HSR_HIP(hipEventRecord(b->ev[2], b->stream));
         first = false;
@@ -442,8 +497,11 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
                                       b->stream, evC, &nClosest, evS, &nShadow));
         HSR_HIP(hipEventRecord(b->ev[1], b->stream));
-        HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, M, b->d_stage_sum, b->d_stage_comp, S, b->d_frame_sum,
-                                   b->d_frame_comp, b->stream));
+        // one chirp: the workgroup tiles are reduced into the frame; more chirps: the gather
+        // kernel has added into the frame directly
+        if (chirps == 1)
+            HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, MD, b->d_stage_sum, b->d_stage_comp, S,
+                                       b->d_frame_sum, b->d_frame_comp, b->stream));
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
         HSR_HIP(hipEventSynchronize(b->ev[3]));
         float ms1 = 0.0f, ms2 = 0.0f;
@@ -485,7 +543,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             a.recNee = nullptr;
             a.recPaths = 0;
         }
-        const size_t needP = (size_t)grid * M;
+        const size_t needP = (size_t)grid * MD;
         if (needP > b->partialCap) {
             dfree(b->d_partials);
             hsr_status s;
@@ -493,13 +551,13 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             b->partialCap = needP;
         }
         const int S = std::min(64, grid);
-        if ((size_t)S * M > b->stageCap) {
+        if ((size_t)S * MD > b->stageCap) {
             dfree(b->d_stage_sum);
             dfree(b->d_stage_comp);
             hsr_status s;
-            if ((s = dalloc(&b->d_stage_sum, (size_t)S * M)) != HSR_OK) return s;
-            if ((s = dalloc(&b->d_stage_comp, (size_t)S * M)) != HSR_OK) return s;
-            b->stageCap = (size_t)S * M;
+            if ((s = dalloc(&b->d_stage_sum, (size_t)S * MD)) != HSR_OK) return s;
+            if ((s = dalloc(&b->d_stage_comp, (size_t)S * MD)) != HSR_OK) return s;
+            b->stageCap = (size_t)S * MD;
         }
         a.partials = (double2 *)b->d_partials;
         HSR_HIP(hipEventRecord(b->ev[0], b->stream));
@@ -507,7 +565,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         first = false;
         HSR_HIP(hsr::launch_trace(a, grid, b->optCounters, b->stream));
         HSR_HIP(hipEventRecord(b->ev[1], b->stream));
-        HSR_HIP(hsr::launch_reduce(b->d_partials, grid, M, b->d_stage_sum, b->d_stage_comp, S, b->d_frame_sum,
+        HSR_HIP(hsr::launch_reduce(b->d_partials, grid, MD, b->d_stage_sum, b->d_stage_comp, S, b->d_frame_sum,
                                    b->d_frame_comp, b->stream));
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
         HSR_HIP(hipEventSynchronize(b->ev[3]));
@@ -519,6 +577,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         launches += 1;
         done += n;
     }
+    }  // transmitters
     float total = 0.0f;
     if (budget > 0) HSR_HIP(hipEventElapsedTime(&total, b->ev[2], b->ev[3]));
     b->totalWeight += (double)budget * (double)sampleWeight;

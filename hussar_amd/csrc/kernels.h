@@ -53,6 +53,14 @@ struct TraceArgs {
     int64_t recPaths;
     // counters (kCount instance): nodes, tris, closest rays, shadow rays, events
     unsigned long long *counters;
+    // MIMO and motion
+    int32_t chBase;     // frame channel of receiver 0 (transmitter index * rxCount)
+    int32_t M;          // chirps per frame
+    int32_t motion;     // 1: every contribution carries its radial velocity (chirp index)
+    float ego[3];       // platform velocity of all antennas [m/s]
+    double chirpFreq;   // RFConfig::chirpFrequency, 1 / (idleTime + rampTime) [Hz]
+    const float4 *vel;  // per triangle slot velocity (xyz) or null
+    double *frameAcc;   // M > 1: the frame sum [M][N][C] complex, accumulated with atomics
 };
 
 hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t stream);
@@ -81,6 +89,7 @@ struct WfArgs {
     int2 *hit;
     float4 *s_o, *s_d;
     double *s_tn;
+    double *s_dv;  // radial velocity of the contribution (motion only)
     float2 *s_v, *s_g1;
     int8_t *s_st, *s_vis;
     int32_t *q_ext0, *q_ext1, *q_shd, *q_gat;

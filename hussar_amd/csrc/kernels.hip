@@ -80,15 +80,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                     if (ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) p.gw = p.gw + ns.gw1;
                     if (ns.gwAdd) p.gw = p.gw + ns.v;
                     if (ns.splat) {
-                        ev = make_event(a, ns.tn, ns.v, a.sampleWeight);
+                        ev = make_event(a, ns.tn, ns.v, a.sampleWeight, ns.dv);
                         has = true;
                         flag = 3;
                         if (kCount) ++cev;
                     }
                 }
-                if (rec && p.depth >= 1) a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + ch] = flag;
+                if (rec && p.depth >= 1) a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + a.chBase + ch] = flag;
             }
-            flush_events(has, ev, ch, tile, a.N, a.C, lane);
+            flush_events(has, ev, a.chBase + ch, tile, a.N, a.C, lane);
         }
 
         // ---- extend (path.h:399-442)

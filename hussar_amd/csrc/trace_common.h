@@ -292,6 +292,8 @@ struct PathState {
     float freq;
     int depth;
     int64_t local;  // index - offset
+    double rate;    // motion: d/dt of the path length up to the current vertex [m/s]
+    f3 vprev;       // motion: velocity of the current vertex (the Tx at depth 0)
 };
 
 __device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p) {
@@ -317,6 +319,8 @@ __device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p
     p.x1 = x1;
     p.pdf = pdf;
     p.depth = 0;
+    p.rate = 0.0;
+    p.vprev = {a.ego[0], a.ego[1], a.ego[2]};
 }
 
 // NEE set-up for receiver ch at the path's current vertex: everything of path.h:296-396
@@ -331,6 +335,7 @@ struct NeeSetup {
     cf gw1;        // plane-filter guiding term (filterGuiding, path.h:376-380), else 0
     bool gwAdd;    // guidingWeight += v when visible (depth > 0, path.h:392-394)
     bool splat;    // the splat skip rule lets v through (integrator.h:121-124)
+    double dv;     // motion: radial velocity -1/2 d/dt(path length) [m/s], else 0
 };
 
 __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int ch) {
@@ -343,6 +348,14 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
     ns.o = cast3<float>(p.o);
     ns.d = cast3<float>(dn);
     ns.tfar = (float)rr;
+    ns.dv = 0.0;
+    if (a.motion) {
+        // the leg to the receiver: d|rx - p|/dt = dn . (v_rx - v_p)
+        const d3 rel = {(double)a.ego[0] - (double)p.vprev.x, (double)a.ego[1] - (double)p.vprev.y,
+                        (double)a.ego[2] - (double)p.vprev.z};
+        const double rn = p.rate + dot(dn, rel);
+        ns.dv = -(rn / 2.0);
+    }
     ns.gw1 = cf{0.0f, 0.0f};
     ns.gwAdd = false;
     const cf3 Hrx = mulv(rotf(rxa), awr_evaluate(cast3<float>(mulTv(rotd(rxa), -dn))));
@@ -427,6 +440,14 @@ __device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot)
     if (dot(n, p.d) > 0.0) n = -n;
     const double cosT = fabs(dot(n, -p.d));
     if (cosT < 1e-3) return false;
+    if (a.motion) {
+        // the incoming leg: d|p - o|/dt = d . (v_p - v_o), the vertex moving with its triangle
+        const float4 vh = a.vel ? a.vel[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const d3 rel = {(double)vh.x - (double)p.vprev.x, (double)vh.y - (double)p.vprev.y,
+                        (double)vh.z - (double)p.vprev.z};
+        p.rate = p.rate + dot(p.d, rel);
+        p.vprev = {vh.x, vh.y, vh.z};
+    }
     const double tg = dot(nc, v0 - p.o) / dot(nc, p.d);
     p.cosTheta = cosT;
     p.r = p.r + tg;
@@ -541,20 +562,52 @@ __device__ inline int64_t wave_append(unsigned int *counter, bool pred, int lane
     return pred ? (int64_t)base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
-// One NEE event ready for the frame: taps value·coef at bins center+k, coef = 1/(s-k), or a
-// single tap at center (delta).
+// One NEE event ready for the frame: taps A·1/(s-k) at sample bins center+k (or one tap at
+// center when delta), and with motion and M > 1 chirps the same along the chirp axis
+// (A·1/(sc-l)·1/(s-k) at chirp cc+l), the separable kernel of Frame::splat (radar.h:564-627).
 struct Event {
     int center;
     float s;
     cf A;
     int delta;
+    int cc;
+    float sc;
+    int deltaC;
 };
 
 // Integrator::splat (integrator.h:113-135) up to the taps: fractional bin (setTime,
-// radar.h:232-241), IF phase (measureRay, integrator.h:107-110) and the splat's own
-// e^{jπs}·sin(πs)/π factor (radar.h:570-583).
-__device__ inline Event make_event(const TraceArgs &a, double tn, cf v, float weight) {
+// radar.h:232-241), fractional chirp from the radial velocity (setVelocity, radar.h:248-251),
+// IF phase (measureRay, integrator.h:107-110) and the splat's own e^{jπs}·sin(πs)/π factor
+// per spread axis (radar.h:570-583). With one chirp the chirp taps all land in chirp 0 and
+// are folded into A.
+__device__ inline Event make_event(const TraceArgs &a, double tn, cf v, float weight, double dv) {
     Event e;
+    e.cc = 0;
+    e.sc = 0.0f;
+    e.deltaC = 1;
+    double wc = 1.0, pc_s = 0.0, pc_c = 1.0;
+    if (a.motion) {
+        const double dp = ((2.0 * (double)a.startFreq) * dv) / kC;
+        const double q = dp / a.chirpFreq;
+        const double xc = (double)a.M * (q - floor(q));
+        const int cc = (int)round(xc);
+        const double sc = xc - (double)cc;
+        e.cc = cc;
+        if (fabs(sc) >= 1e-4) {
+            double ss;
+            sincos_d(kPiD * sc, &ss, &pc_c);
+            pc_s = ss;
+            wc = ss / kPiD;
+            e.sc = (float)sc;
+            e.deltaC = 0;
+            if (a.M == 1) {
+                double f = 0.0;
+                for (int l = -16; l <= 16; ++l) f += 1.0 / (sc - (double)l);
+                wc = wc * f;
+                e.deltaC = 1;
+            }
+        }
+    }
     const double dt = tn + (double)a.antennaDelay;
     double x = 0.0;
     if (a.freqSlope != 0.0f) {
@@ -574,6 +627,7 @@ __device__ inline Event make_event(const TraceArgs &a, double tn, cf v, float we
         ph = {(float)c, (float)s};
     }
     cf value = weight * (v * ph);
+    if (pc_s != 0.0 || wc != 1.0) value = (value * cf{(float)pc_c, (float)pc_s}) * (float)wc;
     const int center = (int)round(x);
     const double s = x - (double)center;
     e.center = center;
@@ -626,6 +680,38 @@ __device__ inline void flush_events(bool has, const Event &e, int ch, double *ti
             const float coef = 1.0f / (s - (float)k);
             atomicAdd(&tile[2 * (b * C + ch) + 0], (double)(Are * coef));
             atomicAdd(&tile[2 * (b * C + ch) + 1], (double)(Aim * coef));
+        }
+    }
+}
+
+// Wave-cooperative splat of this lane's event (if any) into the frame cube in device memory
+// [M][N][C] with FP64 atomics: up to 33 x 33 taps per event, spread over the wave's lanes.
+__device__ inline void flush_events_cube(bool has, const Event &e, int ch, double *frame, int M, int N, int C,
+                                         int lane) {
+    uint64_t m = __ballot(has);
+    while (m) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1;
+        const int center = __builtin_amdgcn_readlane(e.center, src);
+        const float s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.s), src));
+        const float Are = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.A.re), src));
+        const float Aim = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.A.im), src));
+        const int delta = __builtin_amdgcn_readlane(e.delta, src);
+        const int cc = __builtin_amdgcn_readlane(e.cc, src);
+        const float sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.sc), src));
+        const int deltaC = __builtin_amdgcn_readlane(e.deltaC, src);
+        const int ns = delta ? 1 : 33, nc = deltaC ? 1 : 33;
+        for (int t = lane; t < ns * nc; t += 64) {
+            const int l = deltaC ? 0 : t / ns - 16;
+            const int k = delta ? 0 : t % ns - 16;
+            float coef = 1.0f;
+            if (!deltaC) coef = 1.0f / (sc - (float)l);
+            if (!delta) coef = coef * (1.0f / (s - (float)k));
+            const int bc = wrap_bin(cc + l, M);
+            const int bs = wrap_bin(center + k, N);
+            double *dst = frame + 2 * (((int64_t)bc * N + bs) * C + ch);
+            unsafeAtomicAdd(dst + 0, (double)(Are * coef));
+            unsafeAtomicAdd(dst + 1, (double)(Aim * coef));
         }
     }
 }

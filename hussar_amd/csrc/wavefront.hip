@@ -22,8 +22,8 @@
 
 namespace hsr {
 
-enum : int { D_OX = 0, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_NX, D_NY, D_NZ, D_IX, D_IY, D_IZ, D_TIME, D_R, D_COS, D_N };
-enum : int { F_H = 0, F_HIN = 6, F_GWR = 12, F_GWI, F_X0, F_X1, F_PDF, F_FREQ, F_N };
+enum : int { D_OX = 0, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_NX, D_NY, D_NZ, D_IX, D_IY, D_IZ, D_TIME, D_R, D_COS, D_RATE, D_N };
+enum : int { F_H = 0, F_HIN = 6, F_GWR = 12, F_GWI, F_X0, F_X1, F_PDF, F_FREQ, F_VX, F_VY, F_VZ, F_N };
 enum : int { I_DEPTH = 0, I_FLAGS, I_N };
 enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
 
@@ -82,6 +82,20 @@ __device__ inline void store_state(const WfArgs &w, int64_t i, const PathState &
     f[F_PDF * P + i] = p.pdf;
     f[F_FREQ * P + i] = p.freq;
     w.si[I_DEPTH * P + i] = p.depth;
+}
+
+// motion: the path-length rate and the current vertex's velocity
+__device__ inline void load_motion(const WfArgs &w, int64_t i, PathState &p) {
+    const int64_t P = w.P;
+    p.rate = w.sd[D_RATE * P + i];
+    p.vprev = {w.sf[F_VX * P + i], w.sf[F_VY * P + i], w.sf[F_VZ * P + i]};
+}
+__device__ inline void store_motion(const WfArgs &w, int64_t i, const PathState &p) {
+    const int64_t P = w.P;
+    w.sd[D_RATE * P + i] = p.rate;
+    w.sf[F_VX * P + i] = p.vprev.x;
+    w.sf[F_VY * P + i] = p.vprev.y;
+    w.sf[F_VZ * P + i] = p.vprev.z;
 }
 
 // The fields a stage needs, loaded separately to keep register use and traffic down.
@@ -196,6 +210,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
     const bool ends = valid && path_ends(a, p);
     emit_extend(w, valid, ends, i, p, w.q_ext0, &w.cnt[0], lane);
     if (valid) store_state(w, i, p);
+    if (valid && a.motion) store_motion(w, i, p);
     finish_record(a, valid && ends && !nee, p, lane);
 }
 
@@ -426,6 +441,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_hit(TraceArgs a, WfArgs w, int 
         if (valid) {
             load_chain(w, i, p);
             load_field(w, i, p);
+            if (a.motion) load_motion(w, i, p);
             const int2 h = w.hit[i];
             const int32_t id = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
             if (a.recHits && p.local < a.recPaths) a.recHits[p.local * a.maxDepth + p.depth] = id;
@@ -436,6 +452,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_hit(TraceArgs a, WfArgs w, int 
         const bool ends = alive && path_ends(a, p);
         emit_extend(w, alive, ends, i, p, qnext, next, lane);
         if (alive) store_hit(w, i, p);
+        if (alive && a.motion) store_motion(w, i, p);
         finish_record(a, valid && !alive, p, lane);
     }
 }
@@ -461,8 +478,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_nee(TraceArgs a, WfArgs w, int 
             load_chain(w, i, p);
             if (p.depth > 0) load_vertex(w, i, p);
             else p.freq = w.sf[F_FREQ * w.P + i];
+            if (a.motion) load_motion(w, i, p);
             const NeeSetup ns = nee_setup(a, p, ch);
             j = i * a.C + ch;
+            if (a.motion) w.s_dv[j] = ns.dv;
             w.s_o[j] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.tfar);
             w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, 0.0f);
             w.s_tn[j] = ns.tn;
@@ -484,12 +503,15 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_nee(TraceArgs a, WfArgs w, int 
 // ------------------------------------------------------------------------ gather + splat
 __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, int depth, double2 *partials,
                                                         bool count) {
-    extern __shared__ double tile[];  // [N*C] complex double
+    extern __shared__ double tile[];  // [N*C] complex double (one chirp); unused with M > 1
     const int NC = a.N * a.C;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    for (int j = tid; j < 2 * NC; j += kWfBlock) tile[j] = 0.0;
-    __syncthreads();
+    const bool cube = a.M > 1;  // uniform: splat into the frame cube in device memory
+    if (!cube) {
+        for (int j = tid; j < 2 * NC; j += kWfBlock) tile[j] = 0.0;
+        __syncthreads();
+    }
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + tid) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
     const unsigned int n = w.cnt[depth].n_gat;
@@ -528,15 +550,18 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
                     }
                     if (stt & S_GW) p.gw = p.gw + v;
                     if (stt & S_SPLAT) {
-                        ev = make_event(a, w.s_tn[j], v, a.sampleWeight);
+                        ev = make_event(a, w.s_tn[j], v, a.sampleWeight, a.motion ? w.s_dv[j] : 0.0);
                         has = true;
                         flag = 3;
                         if (count) ++cev;
                     }
                 }
-                if (rec) a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + ch] = flag;
+                if (rec) a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + a.chBase + ch] = flag;
             }
-            flush_events(has, ev, ch, tile, a.N, a.C, lane);
+            if (cube)
+                flush_events_cube(has, ev, a.chBase + ch, a.frameAcc, a.M, a.N, a.C, lane);
+            else
+                flush_events(has, ev, a.chBase + ch, tile, a.N, a.C, lane);
         }
         if (valid) {
             w.sf[F_GWR * P + i] = p.gw.re;
@@ -545,6 +570,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
         const bool finishing = valid && (w.si[I_FLAGS * P + i] & 1);
         finish_record(a, finishing, p, lane);
     }
+    if (count) atomicAdd(&a.counters[4], (unsigned long long)cev);
+    if (cube) return;
     __syncthreads();
     double2 *dst = partials + (size_t)blockIdx.x * NC;
     for (int j = tid; j < NC; j += kWfBlock) {
@@ -553,7 +580,6 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
         v.y += tile[2 * j + 1];
         dst[j] = v;
     }
-    if (count) atomicAdd(&a.counters[4], (unsigned long long)cev);
 }
 
 __global__ void k_wf_zero_counts(WfCounts *c, int n) {
@@ -573,7 +599,7 @@ size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
     b += align256(sizeof(int2) * P);
     const int64_t S = P * C;
     b += 2 * align256(sizeof(float4) * S);
-    b += align256(sizeof(double) * S);
+    b += 2 * align256(sizeof(double) * S);
     b += 2 * align256(sizeof(float2) * S);
     b += 2 * align256(sizeof(int8_t) * S);
     b += 3 * align256(sizeof(int32_t) * P) + align256(sizeof(int32_t) * S);
@@ -599,6 +625,7 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.s_o = (float4 *)take(sizeof(float4) * S);
     w.s_d = (float4 *)take(sizeof(float4) * S);
     w.s_tn = (double *)take(sizeof(double) * S);
+    w.s_dv = (double *)take(sizeof(double) * S);
     w.s_v = (float2 *)take(sizeof(float2) * S);
     w.s_g1 = (float2 *)take(sizeof(float2) * S);
     w.s_st = (int8_t *)take(sizeof(int8_t) * S);
@@ -628,7 +655,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     const int maxDepth = a.maxDepth;
     hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 63) / 64), dim3(64), 0, s, w.cnt, maxDepth + 2);
     hipLaunchKernelGGL(k_wf_gen, dim3((unsigned)((w.n + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s, a, w);
-    const size_t glds = gather_lds_bytes(a.N, a.C);
+    const size_t glds = a.M > 1 ? 0 : gather_lds_bytes(a.N, a.C);
     if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, 0);
     for (int k = 0; k <= maxDepth; ++k) {
         if (k >= 1 || !a.onlyIndirect) {

@@ -175,6 +175,14 @@ class HipBackend:
         if integrator is not None:
             integrator._bind(self)
 
+    def set_velocities(self, velocities: Optional[np.ndarray]) -> None:
+        """Per-triangle velocities [m/s] (triangle order of the mesh), or None for a static mesh."""
+        if velocities is None:
+            check(lib().hsr_hip_set_velocities(self._h, None, 0))
+            return
+        v = np.ascontiguousarray(velocities, np.float32).reshape(-1, 3)
+        check(lib().hsr_hip_set_velocities(self._h, v.ctypes.data_as(C.POINTER(C.c_float)), v.shape[0]))
+
     def _configure(self, config: FrameConfig) -> None:
         check(lib().hsr_hip_configure_frame(self._h, C.byref(config.to_c())))
         self.frame_config = config

@@ -199,6 +199,15 @@ def radar_array(rf: RFConfig, channels: int = 4, position=(0.0, 0.75, 0.0),
     return Scene(rfConfig=rf, tx=tx, rx=rxs[0], rxs=rxs)
 
 
+def plate_mesh(size: float = 0.5, distance: float = 5.0) -> TriangleMesh:
+    """A size×size plate in the plane z = distance facing a radar at the origin looking down
+    +z (2 triangles)."""
+    m = TriangleMesh()
+    h = np.float32(size / 2)
+    m.addQuad([-h, -h, np.float32(distance)], [np.float32(size), 0, 0], [0, np.float32(size), 0])
+    return m
+
+
 # --------------------------------------------------------------------------------------
 # Precision study: plates and trihedrals at 5 / 50 / 150 m over random-height ground
 

@@ -22,6 +22,7 @@ from typing import List, Sequence
 import numpy as np
 
 MAX_CHANNELS = 16
+MAX_TX = 4
 
 # Units as in libradar/include/radar/units.h:8-37 (values in SI).
 GHz, MHz, kHz, Hz = 1e9, 1e6, 1e3, 1.0
@@ -47,7 +48,8 @@ class c_antenna(C.Structure):
 
 class c_scene(C.Structure):
     _fields_ = [("rfConfig", c_rf_config), ("tx", c_antenna), ("rx", c_antenna * MAX_CHANNELS),
-                ("rxCount", C.c_int32), ("reserved0", C.c_int32)]
+                ("rxCount", C.c_int32), ("txCount", C.c_int32), ("txMore", c_antenna * (MAX_TX - 1)),
+                ("velocity", C.c_float * 3), ("reserved0", C.c_int32)]
 
 
 class c_knobs(C.Structure):
@@ -196,27 +198,42 @@ class NFAntenna:
 
 @dataclass
 class Scene:
-    """hussar::Scene (scene.h:10-16): rfConfig, tx and rx. ``rxs`` lists extra receivers
-    (the multi-receiver extension); when it is empty the scene has the reference's one
-    receiver ``rx``."""
+    """hussar::Scene (scene.h:10-16): rfConfig, tx and rx, with the extensions of the C ABI:
+    ``rxs`` lists the receivers (when empty, the reference's one receiver ``rx``), ``txs``
+    the MIMO transmitters (when empty, ``tx``; pair (t, r) feeds channel t*len(rxs) + r),
+    and ``velocity`` is the platform velocity of all antennas [m/s]."""
     rfConfig: RFConfig = field(default_factory=RFConfig)
     tx: NFAntenna = field(default_factory=NFAntenna)
     rx: NFAntenna = field(default_factory=NFAntenna)
     rxs: List[NFAntenna] = field(default_factory=list)
+    txs: List[NFAntenna] = field(default_factory=list)
+    velocity: Sequence[float] = (0.0, 0.0, 0.0)
 
     def receivers(self) -> List[NFAntenna]:
         return list(self.rxs) if self.rxs else [self.rx]
 
+    def transmitters(self) -> List[NFAntenna]:
+        return list(self.txs) if self.txs else [self.tx]
+
     def to_c(self) -> c_scene:
         sc = c_scene()
         sc.rfConfig = self.rfConfig.to_c()
-        sc.tx = self.tx.to_c()
         rx = self.receivers()
+        tx = self.transmitters()
         if len(rx) > MAX_CHANNELS:
             raise ValueError(f"at most {MAX_CHANNELS} receivers")
+        if len(tx) > MAX_TX:
+            raise ValueError(f"at most {MAX_TX} transmitters")
         for k, a in enumerate(rx):
             sc.rx[k] = a.to_c()
         sc.rxCount = len(rx)
+        sc.tx = tx[0].to_c()
+        for k, a in enumerate(tx[1:]):
+            sc.txMore[k] = a.to_c()
+        sc.txCount = len(tx)
+        v = np.asarray(self.velocity, np.float32).reshape(3)
+        for k in range(3):
+            sc.velocity[k] = float(v[k])
         return sc
 
 

@@ -25,8 +25,9 @@
 extern "C" {
 #endif
 
-#define HSR_ABI_VERSION 1
+#define HSR_ABI_VERSION 2
 #define HSR_MAX_CHANNELS 16
+#define HSR_MAX_TX 4
 
 typedef enum hsr_status {
     HSR_OK = 0,
@@ -49,7 +50,14 @@ typedef struct hsr_rf_config {
 } hsr_rf_config;
 
 /* radar::FrameConfig (radar.h:141-164). The frame is laid out [chirp][sample][channel]
- * row-major (Frame::makeIndex, radar.h:487-494), complex interleaved (re, im). */
+ * row-major (Frame::makeIndex, radar.h:487-494), complex interleaved (re, im). Like the
+ * reference's, it is the Fourier-domain cube: a contribution lands at the fractional sample
+ * index of its delay (PIndex::setTime, radar.h:232-241) and the fractional chirp index of its
+ * radial velocity (PIndex::setVelocity, radar.h:248-251), spread by the separable truncated
+ * Dirichlet kernel (Frame::splat, radar.h:564-627). The reference integrator splats at
+ * velocity 0 (integrator.h:130); here a contribution's velocity comes from the motion of the
+ * scene (hsr_scene.velocity, hsr_hip_set_velocities). chirpCount > 1 needs the wavefront
+ * pipeline (HSR_OPT_PIPELINE = 1). */
 typedef struct hsr_frame_config {
     int32_t chirpCount;
     int32_t samplesPerChirp;
@@ -65,14 +73,22 @@ typedef struct hsr_antenna {
 } hsr_antenna;
 
 /* hussar::Scene (libhussar/include/hussar/core/scene.h:10-16): one RF config, one Tx and
- * rxCount receivers. The reference has exactly one receiver and fills channel 0 only
- * (path.h:291-292); rx[1..rxCount-1] are the multi-receiver extension and feed frame
- * channels 1..rxCount-1. rxCount must not exceed the frame's channelCount. */
+ * rxCount receivers. The reference has exactly one transmitter and one receiver and fills
+ * channel 0 only (path.h:291-292). Extensions:
+ *  - rx[1..rxCount-1] feed frame channels 1..rxCount-1;
+ *  - MIMO: txCount transmitters (0 or 1 = `tx` only; up to HSR_MAX_TX), tx 0 = `tx`, tx t =
+ *    txMore[t-1]. Every transmitter emits the same Halton paths; the pair (tx t, rx r) feeds
+ *    virtual channel t*rxCount + r, so txCount*rxCount must not exceed channelCount. The
+ *    total weight counts each path index once;
+ *  - velocity: the platform velocity [m/s] shared by all antennas (ego motion). */
 typedef struct hsr_scene {
     hsr_rf_config rfConfig;
     hsr_antenna tx;
     hsr_antenna rx[HSR_MAX_CHANNELS];
     int32_t rxCount;
+    int32_t txCount;
+    hsr_antenna txMore[HSR_MAX_TX - 1];
+    float velocity[3];
     int32_t reserved0;
 } hsr_scene;
 
@@ -119,6 +135,14 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount,
                           const int32_t *indices, int64_t triangleCount,
                           int32_t device, hsr_backend **out);
 void hsr_hip_destroy(hsr_backend *backend);
+
+/* Scene motion (no reference counterpart: the reference splats at velocity 0,
+ * integrator.h:130). velocities: float[3*triangleCount], the velocity [m/s] of every mesh
+ * triangle in the order given to hsr_hip_create; NULL (or triangleCount 0) makes the mesh
+ * static again. A contribution's radial velocity is -1/2 of the rate of change of its path
+ * length, with every path vertex moving with its triangle and the antennas with the scene's
+ * platform velocity. */
+hsr_status hsr_hip_set_velocities(hsr_backend *backend, const float *velocities, int64_t triangleCount);
 
 /* Integrator::configureFrame (libhussar/include/hussar/core/integrator.h:60-62). Allocates
  * the device-resident accumulation frame and clears it. */

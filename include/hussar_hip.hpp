@@ -169,13 +169,17 @@ private:
     Matrix33f m_rotation;
 };
 
-// scene.h:10-16. rxArray is the multi-receiver extension: when it is not empty its antennas
-// feed frame channels 0..n-1 and `rx` is ignored; otherwise `rx` feeds channel 0.
+// scene.h:10-16, with the extensions of hsr_scene: when rxArray is not empty its antennas
+// are the receivers (otherwise `rx`); when txArray is not empty its antennas are the MIMO
+// transmitters (otherwise `tx`), the pair (t, r) feeding channel t * receivers + r;
+// velocity is the platform velocity of all antennas [m/s].
 struct Scene {
     RFConfig rfConfig;
     NFAntenna rx;
     NFAntenna tx;
     std::vector<NFAntenna> rxArray;
+    std::vector<NFAntenna> txArray;
+    Vector3f velocity;
 };
 
 // radar.h:169-... : [chirp][sample][channel] row-major complex64 (makeIndex, radar.h:487-494).
@@ -290,6 +294,14 @@ template <class S, class = void>
 struct has_rx_array : std::false_type {};
 template <class S>
 struct has_rx_array<S, std::void_t<decltype(std::declval<const S &>().rxArray.size())>> : std::true_type {};
+template <class S, class = void>
+struct has_tx_array : std::false_type {};
+template <class S>
+struct has_tx_array<S, std::void_t<decltype(std::declval<const S &>().txArray.size())>> : std::true_type {};
+template <class S, class = void>
+struct has_velocity : std::false_type {};
+template <class S>
+struct has_velocity<S, std::void_t<decltype(std::declval<const S &>().velocity[0])>> : std::true_type {};
 
 template <class S>
 inline hsr_scene scene(const S &s) {
@@ -307,6 +319,16 @@ inline hsr_scene scene(const S &s) {
             r.rxCount = (int32_t)s.rxArray.size();
         }
     }
+    r.txCount = 1;
+    if constexpr (has_tx_array<S>::value) {
+        if (!s.txArray.empty()) {
+            if (s.txArray.size() > HSR_MAX_TX) throw Error(HSR_ERR_INVALID, "more transmitters than supported");
+            r.tx = antenna(s.txArray[0]);
+            for (size_t i = 1; i < s.txArray.size(); ++i) r.txMore[i - 1] = antenna(s.txArray[i]);
+            r.txCount = (int32_t)s.txArray.size();
+        }
+    }
+    if constexpr (has_velocity<S>::value) put3(r.velocity, s.velocity);
     return r;
 }
 }  // namespace detail
@@ -450,6 +472,12 @@ public:
         return s;
     }
     void setOption(int32_t option, int64_t value) { check(hsr_hip_set_option(m_handle, option, value)); }
+    // per-triangle velocities [m/s] in the mesh's triangle order; an empty vector: static mesh
+    void setVelocities(const std::vector<Vector3f> &velocities) {
+        check(hsr_hip_set_velocities(m_handle,
+                                     velocities.empty() ? nullptr : reinterpret_cast<const float *>(velocities.data()),
+                                     (int64_t)velocities.size()));
+    }
     hsr_backend *handle() const { return m_handle; }
 
 private:

@@ -57,13 +57,23 @@
 namespace orc {
 
 static constexpr int kMaxCh = 16;
+static constexpr int kMaxTx = 4;
 
 // ---------------------------------------------------------------------------------------
 // POD mirrors of the C-ABI structs (same layout as include/hussar_hip.h, restated here so
 // that the oracle does not depend on the product).
 struct Rf { float startFreq, freqSlope, adcRate, idleTime, rampTime, antennaDelay; };
 struct Antenna { float pos[3]; float rot[9]; };
-struct SceneC { Rf rf; Antenna tx; Antenna rx[kMaxCh]; int32_t rxCount; int32_t reserved0; };
+struct SceneC {
+    Rf rf;
+    Antenna tx;
+    Antenna rx[kMaxCh];
+    int32_t rxCount;
+    int32_t txCount;             // MIMO: 0 or 1 = tx only; tx t = txMore[t-1]
+    Antenna txMore[kMaxTx - 1];
+    float velocity[3];           // platform velocity of all antennas
+    int32_t reserved0;
+};
 struct Knobs {
     int32_t onlyIndirect, maxDepth, useGeometricalOptics, poDiffraction, doGuiding,
         clearBeforeIteration, correctPhase, filterGuiding, doFiltering, filteringSphere;
@@ -587,7 +597,8 @@ struct Mesh {
     Accel<float> f;
     Accel<double> d;
     int64_t nt = 0;
-    int brute = 0;  // 1: closest/occluded by brute force instead of the BVH
+    int brute = 0;           // 1: closest/occluded by brute force instead of the BVH
+    std::vector<float> vel;  // motion: 3 floats per triangle id, empty for a static mesh
 };
 template <class G> const Accel<G> &accel_of(const Mesh &m);
 template <> const Accel<float> &accel_of<float>(const Mesh &m) { return m.f; }
@@ -769,13 +780,30 @@ struct Guide {
 // ---------------------------------------------------------------------------------------
 // Frame splat (radar::Frame::splat<16>, radar.h:564-627) of one NEE event through
 // Integrator::splat (integrator.h:113-135): fractional sample index via setTime
-// (radar.h:232-241), IF phase via measureRay (integrator.h:107-110), chirp = 0 (no motion,
-// integrator.h:130), 33 taps of value·w/(s-k). The centre bin of a delta peak is wrapped
-// modulo N (the reference would index one past the end when round(x) == N).
+// (radar.h:232-241), IF phase via measureRay (integrator.h:107-110), fractional chirp index
+// via setVelocity (radar.h:248-251) from the contribution's radial velocity delta_v (the
+// reference passes 0, integrator.h:130; `motion` off keeps chirp 0 exactly), and the
+// separable kernel: per spread axis value·e^{jπs}, weight·sin(πs)/π and taps
+// weight/(s_chirp-l)/(s_sample-k) over ±16 bins, an axis with |s| < 1e-4 being a delta.
+// Bins wrap modulo the axis length (the reference would index one past the end when
+// round(x) equals the length). chirpFreq = RFConfig::chirpFrequency in double.
 template <class P>
-inline void splat_event(P *frame, int N, int C, int ch, P delta_t, Cx<P> v, P weight, const Rf &rf,
-                        Counters *cnt) {
+inline void splat_event(P *frame, int M, int N, int C, int ch, P delta_t, Cx<P> v, P weight, const Rf &rf,
+                        Counters *cnt, bool motion = false, P delta_v = P(0)) {
     delta_t = delta_t + P(rf.antennaDelay);
+    // chirp axis (setVelocity)
+    int cc = 0;
+    P sc = P(0);
+    bool deltaC = true;
+    if (motion) {
+        const P chirpFreq = P(1.0 / ((double)rf.idleTime + (double)rf.rampTime));
+        const P dp = ((P(2) * P(rf.startFreq)) * delta_v) / P(kSpeedOfLight);
+        const P q = dp / chirpFreq;
+        const P xc = P(M) * (q - std::floor(q));
+        cc = (int)std::round(xc);
+        sc = xc - P(cc);
+        deltaC = std::abs(sc) < P(1e-4);
+    }
     P x;
     if (rf.freqSlope == 0.0f) {
         x = P(0);
@@ -791,23 +819,29 @@ inline void splat_event(P *frame, int N, int C, int ch, P delta_t, Cx<P> v, P we
     const int center = (int)std::round(x);
     P s = x - P(center);
     if (cnt) cnt->events++;
-    if (std::abs(s) < P(1e-4)) {
-        const int b = ((center % N) + N) % N;
-        P *dst = frame + 2 * ((int64_t)b * C + ch);
-        dst[0] += value.re;
-        dst[1] += value.im;
-        return;
+    const bool deltaS = std::abs(s) < P(1e-4);
+    P w = P(1);
+    if (!deltaC) {
+        const P shiftPi = P(M_PI * (double)sc);
+        value = value * Cx<P>{std::cos(shiftPi), std::sin(shiftPi)};
+        w = w * P(std::sin(shiftPi) / M_PI);
     }
-    const P shiftPi = P(M_PI * (double)s);
-    value = value * Cx<P>{std::cos(shiftPi), std::sin(shiftPi)};
-    const P w = P(std::sin(shiftPi) / M_PI);
-    for (int k = -16; k <= 16; ++k) {
-        int b = (center + k) % N;
-        if (b < 0) b += N;
-        const P wk = w / (s - P(k));
-        P *dst = frame + 2 * ((int64_t)b * C + ch);
-        dst[0] += wk * value.re;
-        dst[1] += wk * value.im;
+    if (!deltaS) {
+        const P shiftPi = P(M_PI * (double)s);
+        value = value * Cx<P>{std::cos(shiftPi), std::sin(shiftPi)};
+        w = w * P(std::sin(shiftPi) / M_PI);
+    }
+    const int lc = deltaC ? 0 : 16, ls = deltaS ? 0 : 16;
+    for (int l = -lc; l <= lc; ++l) {
+        const P wl = deltaC ? w : w / (sc - P(l));
+        const int bc = (((cc + l) % M) + M) % M;
+        for (int k = -ls; k <= ls; ++k) {
+            const P wk = deltaS ? wl : wl / (s - P(k));
+            const int b = (((center + k) % N) + N) % N;
+            P *dst = frame + 2 * (((int64_t)bc * N + b) * C + ch);
+            dst[0] += wk * value.re;
+            dst[1] += wk * value.im;
+        }
     }
 }
 
@@ -848,6 +882,11 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
     bool isFinal;
     float sampleWeight;
     Counters *cnt;
+    // MIMO and motion: the transmitter of this pass, the channel of its receiver 0, and
+    // whether contributions carry a radial velocity (mesh velocities or platform motion)
+    const Antenna *txa = nullptr;
+    int chBase = 0;
+    bool motion = false;
 
     int32_t trace_closest(V3<T> o, V3<T> d, T *t) const {
         if (cnt) cnt->rays_closest++;
@@ -893,8 +932,13 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
         const F freq = F(rf.startFreq) + F(u_f) * (F(rf.freqSlope) * F(rf.rampTime));  // path.h:257
         const F k0 = ((F(2) * piF) * freq) / F(kSpeedOfLight);        // geometry.h:97-99
         const Gd lambda = cG / Gd(freq);                                // geometry.h:102-104
-        const M33<Gd> txRg = mcast<Gd>(scene->tx.rot);
-        const M33<F> txRf = mcast<F>(scene->tx.rot);
+        const Antenna &txAnt = txa ? *txa : scene->tx;
+        const M33<Gd> txRg = mcast<Gd>(txAnt.rot);
+        const M33<F> txRf = mcast<F>(txAnt.rot);
+        // motion: d/dt of the path length up to the current vertex, and that vertex's velocity
+        const V3<Gd> ego = vcast3<Gd>(scene->velocity);
+        Gd rate = Gd(0);
+        V3<Gd> vprev = ego;
 
         float primaryPdf = 1.0f;
         Cx<F> guidingWeight{F(0), F(0)};
@@ -903,7 +947,7 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
         if (knobs->doGuiding && guide) primaryPdf = guide->sample(primary);
         const V3<Gd> dl = uniform_sample_sphere<Gd>(Gd(primary[0]), Gd(primary[1]));
         const CV3<F> Hl = awr_evaluate<F, kPromote>(vcast<F>(dl)) / K<F>::inv4pi();  // evaluate(d) / pdf(d)
-        V3<Gd> o = vcast3<Gd>(scene->tx.pos);
+        V3<Gd> o = vcast3<Gd>(txAnt.pos);
         V3<Gd> d = txRg.mul(dl);
         CV3<F> H = txRf.mul(Hl);
         H = (F(1) / F(primaryPdf)) * H;  // ray.weightBy(1 / primaryPdf)
@@ -926,14 +970,16 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
                     dn = dn / rr;
                     const CV3<F> Hrx = rxRf.mul(awr_evaluate<F, kPromote>(vcast<F>(rxRg.mulT(-dn))));
                     if (!knobs->poDiffraction && !trace_visible(vcast<T>(o), vcast<T>(dn), T(rr))) {
-                        if (rec && depth >= 1) rec->nee[(depth - 1) * C + ch] = 1;
+                        if (rec && depth >= 1) rec->nee[(depth - 1) * C + chBase + ch] = 1;
                         continue;
                     }
+                    // the leg to the receiver: d|rx - p|/dt = dn . (v_rx - v_p)
+                    const Gd dvel = motion ? -((rate + dot(dn, ego - vprev)) / Gd(2)) : Gd(0);
                     const F rrF = F(rr);
                     CV3<F> Hn;
                     if (depth == 0) {
                         // tx.evaluate + tx.connect (emitter.h:182-193)
-                        Hn = txRf.mul(awr_evaluate<F, kPromote>(vcast<F>(txRg.mulT(dn))));
+                        Hn = txRf.mul(awr_evaluate<F, kPromote>(vcast<F>(txRg.mulT(dn))));  // tx.evaluate
                         Hn = (F(1) / ((F(4) * piF) * rrF)) * Hn;
                     } else {
                         // SurfaceEmitter::evaluate (emitter.h:64-70)
@@ -984,11 +1030,12 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
                     }
                     // Integrator::splat skip rule (integrator.h:121-124), off with the debug image
                     if (!knobs->produceDebugImage && (sampleWeight == 0.0f || v.re == F(0) || v.im == F(0))) {
-                        if (rec && depth >= 1) rec->nee[(depth - 1) * C + ch] = 2;
+                        if (rec && depth >= 1) rec->nee[(depth - 1) * C + chBase + ch] = 2;
                         continue;
                     }
-                    splat_event<P>(frame, N, C, ch, tn, Cx<P>{P(v.re), P(v.im)}, P(sampleWeight), rf, cnt);
-                    if (rec && depth >= 1) rec->nee[(depth - 1) * C + ch] = 3;
+                    splat_event<P>(frame, chirps, N, C, chBase + ch, tn, Cx<P>{P(v.re), P(v.im)}, P(sampleWeight),
+                                   rf, cnt, motion, P(dvel));
+                    if (rec && depth >= 1) rec->nee[(depth - 1) * C + chBase + ch] = 3;
                 }
             }
             // random walk (path.h:399-442)
@@ -1002,6 +1049,12 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
             if (dot(n, d) > Gd(0)) n = -n;  // face-forward (cpu.cpp:104-107)
             cosTheta = std::abs(dot(n, -d));
             if (cosTheta < Gd(1e-3)) break;
+            if (motion) {
+                // the incoming leg: d|p - o|/dt = d . (v_p - v_o), the vertex moving with its triangle
+                const V3<Gd> vh = mesh->vel.empty() ? V3<Gd>{Gd(0), Gd(0), Gd(0)} : vcast3<Gd>(&mesh->vel[3 * (size_t)id]);
+                rate = rate + dot(d, vh - vprev);
+                vprev = vh;
+            }
             const Gd tg = refine ? plane_distance(id, o, d) : Gd(t);
             r = r + tg;
             time = time + P(tg) / P(kSpeedOfLight);  // addDistance, geometry.h:110
@@ -1058,6 +1111,10 @@ void render(const Mesh &mesh, const SceneC &scene, const Knobs &knobs, const Fra
         tr.isFinal = isFinal;
         tr.sampleWeight = sampleWeight;
         tr.cnt = counters ? &cnts[tid] : nullptr;
+        tr.motion = !mesh.vel.empty() || scene.velocity[0] != 0.0f || scene.velocity[1] != 0.0f ||
+                    scene.velocity[2] != 0.0f;
+        const int ntx = std::max(1, std::min<int>(scene.txCount, kMaxTx));
+        const int nrx = std::min<int>(scene.rxCount, C);
         const int maxDepth = std::max(1, knobs.maxDepth);
         while (true) {
             int64_t begin, batch;
@@ -1079,7 +1136,12 @@ void render(const Mesh &mesh, const SceneC &scene, const Knobs &knobs, const Fra
                     std::memset(rec.nee, 0, (size_t)maxDepth * C);
                     recp = &rec;
                 }
-                tr.sample((uint64_t)(offset + i), recp);
+                // MIMO: every transmitter emits the same path index into its block of channels
+                for (int t = 0; t < ntx; ++t) {
+                    tr.txa = t == 0 ? &scene.tx : &scene.txMore[t - 1];
+                    tr.chBase = t * nrx;
+                    tr.sample((uint64_t)(offset + i), recp);
+                }
             }
         }
     };
@@ -1170,6 +1232,17 @@ void *orc_mesh_create(const float *V, int64_t nv, const int32_t *I, int64_t nt) 
 }
 void orc_mesh_destroy(void *m) { delete (Mesh *)m; }
 void orc_mesh_set_brute(void *m, int brute) { ((Mesh *)m)->brute = brute; }
+// per-triangle velocities (3 floats each, triangle order); vel NULL or n 0: static mesh
+int orc_mesh_set_velocities(void *mh, const float *vel, int64_t n) {
+    Mesh *m = (Mesh *)mh;
+    if (!vel || n == 0) {
+        m->vel.clear();
+        return 0;
+    }
+    if (n != m->nt) return -1;
+    m->vel.assign(vel, vel + 3 * n);
+    return 0;
+}
 int64_t orc_mesh_node_count(void *m) { return (int64_t)((Mesh *)m)->f.nodes.size(); }
 
 // rays: [n*6] o,d (float). mode: 0 BVH, 1 brute force. any_hit: 0 closest, 1 occlusion.
@@ -1206,15 +1279,17 @@ void orc_intersect(void *mh, int prec, int mode, int any_hit, const float *rays,
 }
 
 // One splat event into frame[2*N*C] (double). prec 0: FP32 arithmetic, else FP64.
-void orc_splat(int prec, const Rf *rf, int N, int C, int ch, double delta_t, double vre, double vim, double weight,
-               double *frame) {
+// one splat event into frame[2*M*N*C]; motion != 0 places it on the chirp axis by delta_v
+void orc_splat(int prec, const Rf *rf, int M, int N, int C, int ch, double delta_t, double vre, double vim,
+               double weight, int motion, double delta_v, double *frame) {
     if (prec == 0) {
-        std::vector<float> f(2 * (size_t)N * C, 0.0f);
-        splat_event<float>(f.data(), N, C, ch, (float)delta_t, Cx<float>{(float)vre, (float)vim}, (float)weight, *rf,
-                           nullptr);
+        std::vector<float> f(2 * (size_t)M * N * C, 0.0f);
+        splat_event<float>(f.data(), M, N, C, ch, (float)delta_t, Cx<float>{(float)vre, (float)vim}, (float)weight,
+                           *rf, nullptr, motion != 0, (float)delta_v);
         for (size_t k = 0; k < f.size(); ++k) frame[k] += f[k];
     } else {
-        splat_event<double>(frame, N, C, ch, delta_t, Cx<double>{vre, vim}, weight, *rf, nullptr);
+        splat_event<double>(frame, M, N, C, ch, delta_t, Cx<double>{vre, vim}, weight, *rf, nullptr, motion != 0,
+                            delta_v);
     }
 }
 

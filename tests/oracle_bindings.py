@@ -49,8 +49,10 @@ def lib():
         L.orc_mesh_node_count.argtypes = [vp]
         L.orc_intersect.argtypes = [vp, C.c_int, C.c_int, C.c_int, P(C.c_float), C.c_int64, C.c_float,
                                     C.c_float, P(C.c_int32), P(C.c_double), P(C.c_int64)]
-        L.orc_splat.argtypes = [C.c_int, P(c_rf_config), C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
-                                C.c_double, C.c_double, P(C.c_double)]
+        L.orc_splat.argtypes = [C.c_int, P(c_rf_config), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, P(C.c_double)]
+        L.orc_mesh_set_velocities.restype = C.c_int
+        L.orc_mesh_set_velocities.argtypes = [vp, P(C.c_float), C.c_int64]
         L.orc_guide_create.restype = vp
         L.orc_guide_create.argtypes = [C.c_float, C.c_float, C.c_int]
         for f in ("orc_guide_destroy", "orc_guide_reset", "orc_guide_step"):
@@ -123,6 +125,14 @@ class OracleMesh:
 
     def set_brute(self, brute: bool):
         lib().orc_mesh_set_brute(self.h, int(brute))
+
+    def set_velocities(self, vel):
+        if vel is None:
+            lib().orc_mesh_set_velocities(self.h, None, 0)
+            return
+        self.vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        if lib().orc_mesh_set_velocities(self.h, _p(self.vel, C.c_float), self.vel.shape[0]):
+            raise ValueError("one velocity per triangle expected")
 
     def intersect(self, rays: np.ndarray, prec=0, brute=False, any_hit=False, tnear=5e-5, tfar=np.inf):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)

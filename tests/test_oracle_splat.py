@@ -10,7 +10,7 @@ from hussar_amd.types import RFConfig, SPEED_OF_LIGHT
 def splat(oracle, rf, N, C, ch, dt, v, w, prec=2):
     import ctypes
     frame = np.zeros(2 * N * C, np.float64)
-    oracle.lib().orc_splat(prec, ctypes.byref(rf.to_c()), N, C, ch, dt, v.real, v.imag, w,
+    oracle.lib().orc_splat(prec, ctypes.byref(rf.to_c()), 1, N, C, ch, dt, v.real, v.imag, w, 0, 0.0,
                            frame.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return (frame[0::2] + 1j * frame[1::2]).reshape(N, C)
 
