@@ -7,7 +7,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .types import c_device_info, c_frame_config, c_knobs, c_run_stats, c_scene
+from .types import (c_device_info, c_frame_config, c_knobs, c_request, c_request_info, c_run_stats, c_scene,
+                    c_server_config, c_server_stats)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhussar_hip.so")
@@ -48,6 +49,16 @@ SIGNATURES = {
     "hsr_hip_device_info": (C.c_int, [C.c_int32, _P(c_device_info)]),
     "hsr_last_error": (C.c_char_p, []),
     "hsr_abi_version": (C.c_int32, []),
+    "hsr_server_create": (C.c_int, [_P(c_server_config), _P(_vp)]),
+    "hsr_server_destroy": (None, [_vp]),
+    "hsr_server_open_scene": (C.c_int, [_vp, _P(C.c_float), C.c_int64, _P(C.c_int32), C.c_int64, _P(C.c_int32)]),
+    "hsr_server_set_velocities": (C.c_int, [_vp, C.c_int32, _P(C.c_float), C.c_int64]),
+    "hsr_server_close_scene": (C.c_int, [_vp, C.c_int32]),
+    "hsr_server_submit": (C.c_int, [_vp, _P(c_request), _P(C.c_int64)]),
+    "hsr_server_poll": (C.c_int, [_vp, C.c_int64, _P(C.c_int32)]),
+    "hsr_server_wait": (C.c_int, [_vp, C.c_int64, _P(C.c_float), C.c_int64, _P(c_request_info)]),
+    "hsr_server_get_stats": (C.c_int, [_vp, _P(c_server_stats)]),
+    "hsr_server_reset_stats": (C.c_int, [_vp]),
 }
 
 _lib = None

@@ -17,11 +17,10 @@
 #include <vector>
 
 #include "../../include/hussar_hip.h"
+#include "backend.h"
 #include "bvh.h"
-#include "guide.h"
-#include "kernels.h"
 
-namespace {
+namespace hsr {
 
 thread_local std::string g_last_error;
 
@@ -30,90 +29,16 @@ hsr_status fail(hsr_status s, const std::string &msg) {
     return s;
 }
 
-#define HSR_HIP(call)                                                                             \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return fail(e_ == hipErrorOutOfMemory ? HSR_ERR_OOM : HSR_ERR_HIP,                    \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                       \
-    } while (0)
+}  // namespace hsr
 
-template <class T> hsr_status dalloc(T **p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess) return fail(HSR_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    return HSR_OK;
-}
-
-template <class T> void dfree(T *&p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-}  // namespace
-
-struct hsr_guide {
-    hsr_backend *backend = nullptr;  // used only while the backend is alive (runs, steps)
-    int device = 0;                  // own copy: destroy must not touch the backend
-    hsr::GuideHost host;
-    int32_t *d_sChild = nullptr;
-    float *d_sP0 = nullptr;
-    int32_t *d_tChild = nullptr;
-    double *d_tStat = nullptr;
-    int32_t capacity = 1 << 16;
-    bool dirty = true;  // host trees newer than the device copies
-};
-
-struct hsr_backend {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int cu = 0;
-    // mesh
-    float4 *d_nodes = nullptr;
-    float4 *d_tris = nullptr;
-    int32_t nodeCount = 0, slotCount = 0, bvhDepth = 0;
-    int64_t triangles = 0;
-    double bvh_ms = 0.0;
-    std::vector<int32_t> slotIds;  // triangle id of every slot (leaf order)
-    float4 *d_vel = nullptr;       // per slot velocity, null for a static mesh
-    // frame
-    bool configured = false;
-    hsr_frame_config fc{1, 0, 0};
-    int64_t frameDoubles = 0;  // 2 * chirps * N * C
-    double *d_frame_sum = nullptr, *d_frame_comp = nullptr;
-    double totalWeight = 0.0;
-    // scratch
-    double *d_partials = nullptr;
-    size_t partialCap = 0;  // doubles
-    double *d_stage_sum = nullptr, *d_stage_comp = nullptr;
-    size_t stageCap = 0;
-    unsigned long long *d_counters = nullptr;
-    int32_t *d_recHits = nullptr;
-    int8_t *d_recNee = nullptr;
-    int64_t recCap = 0, recPathsLast = 0;
-    int32_t recMaxDepth = 0, recC = 0;
-    // options
-    bool optCounters = false;
-    int64_t optRecord = 0;
-    int64_t optPathsPerLaunch = int64_t(1) << 24;
-    int32_t optPhaseMode = 0;
-    int32_t optPipeline = 1;
-    int32_t maxGrid = 0;
-    // wavefront buffers
-    void *d_wf = nullptr;
-    size_t wfBytes = 0;
-    int64_t wfCap = int64_t(1) << 22;  // paths per wavefront batch
-    // timing
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::vector<hipEvent_t> evTrav;  // pairs around traversal launches (wavefront)
-    hsr_run_stats stats{};
-};
+using hsr::dalloc;
+using hsr::dfree;
+using hsr::fail;
 
 // --------------------------------------------------------------------------------------
 extern "C" {
 
-const char *hsr_last_error(void) { return g_last_error.c_str(); }
+const char *hsr_last_error(void) { return hsr::g_last_error.c_str(); }
 int32_t hsr_abi_version(void) { return HSR_ABI_VERSION; }
 
 void hsr_knobs_default(hsr_knobs *k) {
@@ -217,6 +142,58 @@ void hsr_hip_destroy(hsr_backend *b) {
     delete b;
 }
 
+}  // extern "C"
+
+namespace hsr {
+hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64_t frameDoubles) {
+    HSR_HIP(hipSetDevice(b->device));
+    const size_t need = wavefront_bytes(P, C, maxDepth);
+    if (need > b->wfBytes) {
+        if (b->d_wf) (void)hipFree(b->d_wf);
+        b->d_wf = nullptr;
+        b->wfBytes = 0;
+        HSR_HIP(hipMalloc(&b->d_wf, need));
+        b->wfBytes = need;
+    }
+    b->wfCap = P;
+    // the largest one-chirp frame that still uses LDS tiles: N*C <= 4096 complex
+    const size_t tileDoubles = 2 * 4096;
+    const size_t needP = (size_t)b->cu * 2 * tileDoubles, needS = (size_t)64 * tileDoubles;
+    if (needP > b->partialCap) {
+        dfree(b->d_partials);
+        b->partialCap = 0;
+        hsr_status s;
+        if ((s = dalloc(&b->d_partials, needP)) != HSR_OK) return s;
+        b->partialCap = needP;
+    }
+    if (needS > b->stageCap) {
+        dfree(b->d_stage_sum);
+        dfree(b->d_stage_comp);
+        b->stageCap = 0;
+        hsr_status s;
+        if ((s = dalloc(&b->d_stage_sum, needS)) != HSR_OK) return s;
+        if ((s = dalloc(&b->d_stage_comp, needS)) != HSR_OK) return s;
+        b->stageCap = needS;
+    }
+    if (frameDoubles > b->frameCap) {
+        dfree(b->d_frame_sum);
+        dfree(b->d_frame_comp);
+        b->frameCap = 0;
+        hsr_status s;
+        if ((s = dalloc(&b->d_frame_sum, frameDoubles)) != HSR_OK) return s;
+        if ((s = dalloc(&b->d_frame_comp, frameDoubles)) != HSR_OK) return s;
+        b->frameCap = frameDoubles;
+    }
+    return HSR_OK;
+}
+
+int64_t backend_mesh_bytes(const hsr_backend *b) {
+    return (int64_t)b->nodeCount * 64 + (int64_t)b->slotCount * 48 + (b->d_vel ? (int64_t)b->slotCount * 16 : 0);
+}
+}  // namespace hsr
+
+extern "C" {
+
 hsr_status hsr_hip_set_velocities(hsr_backend *b, const float *vel, int64_t triangleCount) {
     if (!b) return fail(HSR_ERR_INVALID, "null backend");
     HSR_HIP(hipSetDevice(b->device));
@@ -249,19 +226,21 @@ hsr_status hsr_hip_configure_frame(hsr_backend *b, const hsr_frame_config *c) {
         return fail(HSR_ERR_INVALID, "frame dimensions out of range");
     if ((int64_t)c->chirpCount * c->samplesPerChirp * c->channelCount > (int64_t(1) << 28))
         return fail(HSR_ERR_INVALID, "frame too large");
-    // one chirp: the frame is accumulated in LDS tiles; more chirps: in device memory
-    if (c->chirpCount == 1 && hsr::trace_lds_bytes(c->samplesPerChirp, c->channelCount) > 65536)
-        return fail(HSR_ERR_UNSUPPORTED, "frame tile exceeds 64 KiB of LDS (samples x channels too large)");
+    // frames whose one-chirp tile fits in LDS are accumulated in per-workgroup LDS tiles,
+    // others directly in device memory (wavefront pipeline only)
     HSR_HIP(hipSetDevice(b->device));
     const int64_t nd = 2LL * c->chirpCount * c->samplesPerChirp * c->channelCount;
-    if (nd != b->frameDoubles) {
+    if (nd > b->frameCap) {
+        HSR_HIP(hipStreamSynchronize(b->stream));
         dfree(b->d_frame_sum);
         dfree(b->d_frame_comp);
+        b->frameCap = 0;
         hsr_status s;
         if ((s = dalloc(&b->d_frame_sum, nd)) != HSR_OK) return s;
         if ((s = dalloc(&b->d_frame_comp, nd)) != HSR_OK) return s;
-        b->frameDoubles = nd;
+        b->frameCap = nd;
     }
+    b->frameDoubles = nd;
     b->fc = *c;
     b->configured = true;
     b->maxGrid = 0;
@@ -349,11 +328,15 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     if (T > HSR_MAX_TX) return fail(HSR_ERR_INVALID, "txCount out of range");
     if (T * scene->rxCount > b->fc.channelCount)
         return fail(HSR_ERR_INVALID, "txCount * rxCount exceeds the frame's channels");
-    const bool motion = b->d_vel != nullptr || scene->velocity[0] != 0.0f || scene->velocity[1] != 0.0f ||
+    const hsr_backend *mesh = b->meshFrom ? b->meshFrom : b;
+    const bool motion = mesh->d_vel != nullptr || scene->velocity[0] != 0.0f || scene->velocity[1] != 0.0f ||
                         scene->velocity[2] != 0.0f;
     const int chirps = b->fc.chirpCount;
+    const bool cube = chirps > 1 || hsr::gather_lds_bytes(b->fc.samplesPerChirp, b->fc.channelCount) > 65536;
     if ((motion || chirps > 1) && b->optPipeline == 0)
         return fail(HSR_ERR_UNSUPPORTED, "motion and multi-chirp frames need the wavefront pipeline");
+    if (b->optPipeline == 0 && hsr::trace_lds_bytes(b->fc.samplesPerChirp, b->fc.channelCount) > 65536)
+        return fail(HSR_ERR_UNSUPPORTED, "the fused kernel's frame tile exceeds 64 KiB of LDS (samples x channels)");
     if (motion && !(scene->rfConfig.idleTime + scene->rfConfig.rampTime > 0.0f))
         return fail(HSR_ERR_INVALID, "motion needs a positive chirp period (idleTime + rampTime)");
     HSR_HIP(hipSetDevice(b->device));
@@ -384,8 +367,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.C = C;
     a.sampleWeight = sampleWeight;
     a.isFinal = isFinalIteration;
-    a.nodes = b->d_nodes;
-    a.tris = b->d_tris;
+    a.nodes = mesh->d_nodes;
+    a.tris = mesh->d_tris;
     if (guide) {
         a.gsChild = guide->d_sChild;
         a.gsP0 = guide->d_sP0;
@@ -398,8 +381,9 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.counters = b->d_counters;
     a.M = chirps;
     a.motion = motion;
-    a.vel = b->d_vel;
-    a.frameAcc = chirps > 1 ? b->d_frame_sum : nullptr;
+    a.vel = mesh->d_vel;
+    a.frameAcc = cube ? b->d_frame_sum : nullptr;
+    a.cube = cube;
 
     // grid: as many workgroups as fit at once, fewer for small budgets (>= 8 paths per lane)
     if (b->maxGrid == 0) {
@@ -467,14 +451,14 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         const int gridWork = b->cu * 4;
         const int gridGather = b->cu * 2;
         const size_t needP = (size_t)gridGather * MD;
-        if (needP > b->partialCap) {
+        if (!cube && needP > b->partialCap) {
             dfree(b->d_partials);
             hsr_status s;
             if ((s = dalloc(&b->d_partials, needP)) != HSR_OK) return s;
             b->partialCap = needP;
         }
         const int S = std::min(64, gridGather);
-        if ((size_t)S * MD > b->stageCap) {
+        if (!cube && (size_t)S * MD > b->stageCap) {
             dfree(b->d_stage_sum);
             dfree(b->d_stage_comp);
             hsr_status s;
@@ -482,7 +466,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             if ((s = dalloc(&b->d_stage_comp, (size_t)S * MD)) != HSR_OK) return s;
             b->stageCap = (size_t)S * MD;
         }
-        if (chirps == 1) HSR_HIP(hsr::launch_zero(b->d_partials, (int64_t)gridGather * MD, b->stream));
+        if (!cube) HSR_HIP(hsr::launch_zero(b->d_partials, (int64_t)gridGather * MD, b->stream));
         HSR_HIP(hipEventRecord(b->ev[0], b->stream));
         if (first) HSR_HIP(hipEventRecord(b->ev[2], b->stream));
         first = false;
@@ -497,9 +481,9 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
                                       b->stream, evC, &nClosest, evS, &nShadow));
         HSR_HIP(hipEventRecord(b->ev[1], b->stream));
-        // one chirp: the workgroup tiles are reduced into the frame; more chirps: the gather
-        // kernel has added into the frame directly
-        if (chirps == 1)
+        // LDS tiles: the workgroup tiles are reduced into the frame; cube: the gather kernel
+        // has added into the frame directly
+        if (!cube)
             HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, MD, b->d_stage_sum, b->d_stage_comp, S,
                                        b->d_frame_sum, b->d_frame_comp, b->stream));
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));

@@ -1,0 +1,112 @@
+// Internal state of the gfx950 backend shared by the C ABI (api.cpp) and the serving runtime
+// (serve.cpp): the backend and guide objects behind the opaque handles of hussar_hip.h, and
+// the error / device-memory helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hussar_hip.h"
+#include "guide.h"
+#include "kernels.h"
+
+namespace hsr {
+
+// Records msg as the calling thread's hsr_last_error() and returns s.
+hsr_status fail(hsr_status s, const std::string &msg);
+
+template <class T> hsr_status dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(HSR_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return HSR_OK;
+}
+
+template <class T> void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace hsr
+
+#define HSR_HIP(call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return hsr::fail(e_ == hipErrorOutOfMemory ? HSR_ERR_OOM : HSR_ERR_HIP,               \
+                             std::string(#call) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+struct hsr_guide {
+    hsr_backend *backend = nullptr;  // used only while the backend is alive (runs, steps)
+    int device = 0;                  // own copy: destroy must not touch the backend
+    hsr::GuideHost host;
+    int32_t *d_sChild = nullptr;
+    float *d_sP0 = nullptr;
+    int32_t *d_tChild = nullptr;
+    double *d_tStat = nullptr;
+    int32_t capacity = 1 << 16;
+    bool dirty = true;  // host trees newer than the device copies
+};
+
+struct hsr_backend {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu = 0;
+    // mesh
+    float4 *d_nodes = nullptr;
+    float4 *d_tris = nullptr;
+    int32_t nodeCount = 0, slotCount = 0, bvhDepth = 0;
+    int64_t triangles = 0;
+    double bvh_ms = 0.0;
+    std::vector<int32_t> slotIds;  // triangle id of every slot (leaf order)
+    float4 *d_vel = nullptr;       // per slot velocity, null for a static mesh
+    // when set, runs trace this backend's mesh instead of their own (a serving slot running
+    // a request on a resident scene; the server keeps the scene alive meanwhile)
+    const hsr_backend *meshFrom = nullptr;
+    // frame
+    bool configured = false;
+    hsr_frame_config fc{1, 0, 0};
+    int64_t frameDoubles = 0;  // 2 * chirps * N * C
+    int64_t frameCap = 0;      // doubles allocated (grow-only: no frees between frames)
+    double *d_frame_sum = nullptr, *d_frame_comp = nullptr;
+    double totalWeight = 0.0;
+    // scratch
+    double *d_partials = nullptr;
+    size_t partialCap = 0;  // doubles
+    double *d_stage_sum = nullptr, *d_stage_comp = nullptr;
+    size_t stageCap = 0;
+    unsigned long long *d_counters = nullptr;
+    int32_t *d_recHits = nullptr;
+    int8_t *d_recNee = nullptr;
+    int64_t recCap = 0, recPathsLast = 0;
+    int32_t recMaxDepth = 0, recC = 0;
+    // options
+    bool optCounters = false;
+    int64_t optRecord = 0;
+    int64_t optPathsPerLaunch = int64_t(1) << 24;
+    int32_t optPhaseMode = 0;
+    int32_t optPipeline = 1;
+    int32_t maxGrid = 0;
+    // wavefront buffers
+    void *d_wf = nullptr;
+    size_t wfBytes = 0;
+    int64_t wfCap = int64_t(1) << 22;  // paths per wavefront batch
+    // timing
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> evTrav;  // pairs around traversal launches (wavefront)
+    hsr_run_stats stats{};
+};
+
+namespace hsr {
+// Allocates the wavefront buffers for batches of P paths with up to C receivers and maxDepth
+// bounces, and a frame of frameDoubles doubles, ahead of time (serving slots allocate nothing
+// while requests run).
+hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64_t frameDoubles);
+// Device bytes of a backend's mesh (BVH nodes, triangle slots, velocities).
+int64_t backend_mesh_bytes(const hsr_backend *b);
+}  // namespace hsr
+

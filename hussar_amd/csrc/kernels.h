@@ -60,7 +60,8 @@ struct TraceArgs {
     float ego[3];       // platform velocity of all antennas [m/s]
     double chirpFreq;   // RFConfig::chirpFrequency, 1 / (idleTime + rampTime) [Hz]
     const float4 *vel;  // per triangle slot velocity (xyz) or null
-    double *frameAcc;   // M > 1: the frame sum [M][N][C] complex, accumulated with atomics
+    double *frameAcc;   // cube: the frame sum [M][N][C] complex, accumulated with atomics
+    int32_t cube;       // 1: splat into frameAcc (M > 1, or one chirp too large for an LDS tile)
 };
 
 hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t stream);

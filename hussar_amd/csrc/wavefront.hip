@@ -507,7 +507,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
     const int NC = a.N * a.C;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const bool cube = a.M > 1;  // uniform: splat into the frame cube in device memory
+    const bool cube = a.cube != 0;  // uniform: splat into the frame cube in device memory
     if (!cube) {
         for (int j = tid; j < 2 * NC; j += kWfBlock) tile[j] = 0.0;
         __syncthreads();
@@ -655,7 +655,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     const int maxDepth = a.maxDepth;
     hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 63) / 64), dim3(64), 0, s, w.cnt, maxDepth + 2);
     hipLaunchKernelGGL(k_wf_gen, dim3((unsigned)((w.n + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s, a, w);
-    const size_t glds = a.M > 1 ? 0 : gather_lds_bytes(a.N, a.C);
+    const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
     if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, 0);
     for (int k = 0; k <= maxDepth; ++k) {
         if (k >= 1 || !a.onlyIndirect) {

@@ -78,6 +78,29 @@ class c_device_info(C.Structure):
                 ("clock_mhz", C.c_int32)]
 
 
+class c_server_config(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("queueCapacity", C.c_int32),
+                ("maxDepth", C.c_int32), ("hbmBudget", C.c_int64), ("pathsPerBatch", C.c_int64),
+                ("frameReserveBytes", C.c_int64)]
+
+
+class c_request(C.Structure):
+    _fields_ = [("sceneId", C.c_int32), ("reserved0", C.c_int32), ("paths", C.c_int64), ("scene", c_scene),
+                ("knobs", c_knobs), ("frame", c_frame_config), ("reserved1", C.c_int32)]
+
+
+class c_request_info(C.Structure):
+    _fields_ = [("status", C.c_int32), ("slot", C.c_int32), ("queue_ms", C.c_double), ("run_ms", C.c_double),
+                ("latency_ms", C.c_double), ("totalWeight", C.c_double)]
+
+
+class c_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("queued", C.c_int64), ("running", C.c_int64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("frames_per_s", C.c_double),
+                ("hbm_used", C.c_int64), ("hbm_budget", C.c_int64), ("scenes", C.c_int32), ("slots", C.c_int32)]
+
+
 @dataclass
 class RFConfig:
     """radar::RFConfig (radar.h:113-136)."""

@@ -276,6 +276,85 @@ typedef struct hsr_device_info {
 } hsr_device_info;
 hsr_status hsr_hip_device_info(int32_t device, hsr_device_info *out);
 
+/* ---------------------------------------------------------------------------------------
+ * Serving runtime (no reference counterpart; the reference's only analogue is the
+ * visualizer's interruptible re-render loop, visualizer/src/Simulator.h:128-183).
+ *
+ * A server keeps scenes resident on one device (BVH built once per scene) and runs frame
+ * requests on `slots` workers, each with its own HIP stream, wavefront buffers, frame and
+ * guide, all allocated when the server is created: nothing is allocated while requests
+ * run. Requests wait in one FIFO queue; a free slot takes the next one, so requests of
+ * different scenes and frame shapes run concurrently and a short request does not wait for a
+ * long one to finish. Each request is a complete PathTracer::run (path.h:171-224) with the
+ * request's knobs (the guiding schedule included) on a cleared frame.
+ *
+ * Admission: the device bytes of the slots (reserved at create) plus those of the resident
+ * scenes must stay within hbmBudget (hsr_server_open_scene returns HSR_ERR_OOM otherwise);
+ * a request whose frame exceeds a slot's frame reservation (frameReserveBytes) or whose
+ * receivers, chirps or depth exceed the slot's reservation is refused at submit with
+ * HSR_ERR_OOM / HSR_ERR_UNSUPPORTED; a full queue returns HSR_ERR_BUSY.
+ * All functions are thread-safe. */
+typedef struct hsr_server hsr_server;
+
+typedef struct hsr_server_config {
+    int32_t device;
+    int32_t slots;             /* concurrent requests; 0 = 4 */
+    int32_t queueCapacity;     /* queued requests before submit returns HSR_ERR_BUSY; 0 = 4096 */
+    int32_t maxDepth;          /* largest knobs.maxDepth a request may use; 0 = 16 */
+    int64_t hbmBudget;         /* device bytes for slots and scenes; 0 = 90% of the free memory */
+    int64_t pathsPerBatch;     /* wavefront batch per slot; 0 = 2^22 */
+    int64_t frameReserveBytes; /* per slot frame (sum + compensation); 0 = 64 MiB */
+} hsr_server_config;
+
+typedef struct hsr_request {
+    int32_t sceneId;
+    int32_t reserved0;
+    int64_t paths;             /* the samples argument of PathTracer::run */
+    hsr_scene scene;
+    hsr_knobs knobs;
+    hsr_frame_config frame;
+    int32_t reserved1;
+} hsr_request;
+
+typedef struct hsr_request_info {
+    int32_t status;            /* hsr_status of the request's run */
+    int32_t slot;
+    double queue_ms;           /* submit -> start */
+    double run_ms;             /* start -> frame copied out */
+    double latency_ms;         /* submit -> done */
+    double totalWeight;
+} hsr_request_info;
+
+typedef struct hsr_server_stats {
+    int64_t submitted, completed, failed, queued, running;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency (submit -> done) of completed requests */
+    double frames_per_s;                    /* completed / (last completion - first submission) */
+    int64_t hbm_used, hbm_budget;
+    int32_t scenes, slots;
+} hsr_server_stats;
+
+hsr_status hsr_server_create(const hsr_server_config *config, hsr_server **out);
+/* Interrupts running requests, drops queued ones and frees everything. */
+void hsr_server_destroy(hsr_server *server);
+/* Builds the scene's BVH and makes it resident (the mesh is copied). */
+hsr_status hsr_server_open_scene(hsr_server *server, const float *vertices, int64_t vertexCount,
+                                 const int32_t *indices, int64_t triangleCount, int32_t *sceneId);
+/* Per-triangle velocities of a resident scene (see hsr_hip_set_velocities); only while no
+ * request of the scene is queued or running. */
+hsr_status hsr_server_set_velocities(hsr_server *server, int32_t sceneId, const float *velocities,
+                                     int64_t triangleCount);
+/* Releases a scene once its queued and running requests are done; later submits fail. */
+hsr_status hsr_server_close_scene(hsr_server *server, int32_t sceneId);
+hsr_status hsr_server_submit(hsr_server *server, const hsr_request *request, int64_t *ticket);
+/* done = 1 once the request has finished (successfully or not). */
+hsr_status hsr_server_poll(hsr_server *server, int64_t ticket, int32_t *done);
+/* Blocks until the request is done, copies its frame (complex64, frame.sampleCount()
+ * values; frame_c64 may be NULL) and releases the ticket. Returns the request's status. */
+hsr_status hsr_server_wait(hsr_server *server, int64_t ticket, float *frame_c64, int64_t capacity,
+                           hsr_request_info *info);
+hsr_status hsr_server_get_stats(hsr_server *server, hsr_server_stats *out);
+hsr_status hsr_server_reset_stats(hsr_server *server);
+
 const char *hsr_last_error(void);
 int32_t hsr_abi_version(void);
 

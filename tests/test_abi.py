@@ -45,6 +45,10 @@ STRUCTS = {
     "hsr_knobs": T.c_knobs,
     "hsr_run_stats": T.c_run_stats,
     "hsr_device_info": T.c_device_info,
+    "hsr_server_config": T.c_server_config,
+    "hsr_request": T.c_request,
+    "hsr_request_info": T.c_request_info,
+    "hsr_server_stats": T.c_server_stats,
 }
 
 
@@ -88,6 +92,10 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert L.hsr_hip_run(None, None, None, 0, 0, 1.0, 0, None, None) == 1
     assert L.hsr_hip_configure_frame(None, None) == 1
     assert b"null" in L.hsr_last_error()
+    assert L.hsr_server_create(None, None) == 1
+    assert L.hsr_server_submit(None, None, None) == 1
+    assert L.hsr_server_get_stats(None, None) == 1
+    L.hsr_server_destroy(None)
 
 
 def test_python_api_fails_loudly_without_the_library(monkeypatch, tmp_path):
