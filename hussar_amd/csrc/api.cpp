@@ -73,6 +73,7 @@ hsr_status hsr_hip_device_info(int32_t device, hsr_device_info *out) {
     out->hbm_bytes = (int64_t)p.totalGlobalMem;
     out->lds_bytes_per_cu = (int32_t)p.maxSharedMemoryPerMultiProcessor;
     out->clock_mhz = p.clockRate / 1000;
+    out->lds_bytes_per_block = (int32_t)p.sharedMemPerBlock;
     return HSR_OK;
 }
 
@@ -94,6 +95,8 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
         return fail(HSR_ERR_UNSUPPORTED, std::string("this library is built for gfx950, device is ") +
                                              prop.gcnArchName);
     b->cu = prop.multiProcessorCount;
+    b->ldsPerBlock = (int)prop.sharedMemPerBlock;
+    b->ldsPerCU = (int)prop.maxSharedMemoryPerMultiProcessor;
     HSR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto &e : b->ev) HSR_HIP(hipEventCreate(&e));
 
@@ -156,8 +159,8 @@ hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64
         b->wfBytes = need;
     }
     b->wfCap = P;
-    // the largest one-chirp frame that still uses LDS tiles: N*C <= 4096 complex
-    const size_t tileDoubles = 2 * 4096;
+    // the largest one-chirp frame that still uses LDS tiles
+    const size_t tileDoubles = (size_t)b->ldsPerBlock / sizeof(double);
     const size_t needP = (size_t)b->cu * 2 * tileDoubles, needS = (size_t)64 * tileDoubles;
     if (needP > b->partialCap) {
         dfree(b->d_partials);
@@ -332,7 +335,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     const bool motion = mesh->d_vel != nullptr || scene->velocity[0] != 0.0f || scene->velocity[1] != 0.0f ||
                         scene->velocity[2] != 0.0f;
     const int chirps = b->fc.chirpCount;
-    const bool cube = chirps > 1 || hsr::gather_lds_bytes(b->fc.samplesPerChirp, b->fc.channelCount) > 65536;
+    const size_t tileBytes = hsr::gather_lds_bytes(b->fc.samplesPerChirp, b->fc.channelCount);
+    const bool cube = chirps > 1 || tileBytes > (size_t)b->ldsPerBlock;
     if ((motion || chirps > 1) && b->optPipeline == 0)
         return fail(HSR_ERR_UNSUPPORTED, "motion and multi-chirp frames need the wavefront pipeline");
     if (b->optPipeline == 0 && hsr::trace_lds_bytes(b->fc.samplesPerChirp, b->fc.channelCount) > 65536)
@@ -449,7 +453,11 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         }
         const int gridTrav = b->cu * hsr::wavefront_trav_blocks_per_cu(b->optCounters);
         const int gridWork = b->cu * 4;
-        const int gridGather = b->cu * 2;
+        // gather: up to two resident workgroups per CU (their LDS tiles permitting), and no
+        // more workgroups than the batch has paths for (each one adds a tile to reduce)
+        const int tilesPerCU = cube ? 2 : (int)std::max<size_t>(1, std::min<size_t>(2, (size_t)b->ldsPerCU / tileBytes));
+        const int gridGather = (int)std::max<int64_t>(
+            1, std::min<int64_t>((int64_t)b->cu * tilesPerCU, (n + hsr::kTraceBlock - 1) / hsr::kTraceBlock));
         const size_t needP = (size_t)gridGather * MD;
         if (!cube && needP > b->partialCap) {
             dfree(b->d_partials);

@@ -56,6 +56,8 @@ struct hsr_backend {
     int device = 0;
     hipStream_t stream = nullptr;
     int cu = 0;
+    int ldsPerBlock = 65536;  // device limit of a workgroup's LDS (160 KiB on gfx950)
+    int ldsPerCU = 65536;
     // mesh
     float4 *d_nodes = nullptr;
     float4 *d_tris = nullptr;

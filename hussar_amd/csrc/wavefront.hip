@@ -653,18 +653,34 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     *nShadow = 0;
     (void)hipGetLastError();
     const int maxDepth = a.maxDepth;
+    // grids no larger than the batch's items can use (queue sizes are only known on the
+    // device; the batch's path count bounds them), so that small batches launch small grids
+    const int64_t n = w.n;
+    const int nrx = min(a.rxCount, a.C);
+    auto fit = [](int grid, int64_t items) {
+        return (int)std::max<int64_t>(1, std::min<int64_t>(grid, (items + kWfBlock - 1) / kWfBlock));
+    };
+    const int gClosest = fit(gridTrav, n);
+    const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket));
+    const int gHit = fit(gridWork, n);
+    const int gNee = fit(gridWork, n * nrx);
     hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 63) / 64), dim3(64), 0, s, w.cnt, maxDepth + 2);
     hipLaunchKernelGGL(k_wf_gen, dim3((unsigned)((w.n + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s, a, w);
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
-    if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, 0);
+    if (glds > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_wf_gather,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
+        if (e != hipSuccess) return e;
+    }
+    if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee, dim3(gNee), dim3(kWfBlock), 0, s, a, w, 0);
     for (int k = 0; k <= maxDepth; ++k) {
         if (k >= 1 || !a.onlyIndirect) {
             if (!a.poDiffraction) {
                 (void)hipEventRecord(evShadow[2 * *nShadow], s);
                 if (count)
-                    hipLaunchKernelGGL((k_wf_shadow<true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                    hipLaunchKernelGGL((k_wf_shadow<true>), dim3(gShadow), dim3(kWfBlock), 0, s, a, w, k);
                 else
-                    hipLaunchKernelGGL((k_wf_shadow<false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                    hipLaunchKernelGGL((k_wf_shadow<false>), dim3(gShadow), dim3(kWfBlock), 0, s, a, w, k);
                 (void)hipEventRecord(evShadow[2 * *nShadow + 1], s);
                 ++*nShadow;
             }
@@ -673,13 +689,13 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
         if (k < maxDepth) {
             (void)hipEventRecord(evClosest[2 * *nClosest], s);
             if (count)
-                hipLaunchKernelGGL((k_wf_trav<false, true>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                hipLaunchKernelGGL((k_wf_trav<false, true>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
             else
-                hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gridTrav), dim3(kWfBlock), 0, s, a, w, k);
+                hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
             (void)hipEventRecord(evClosest[2 * *nClosest + 1], s);
             ++*nClosest;
-            hipLaunchKernelGGL(k_wf_hit, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k);
-            hipLaunchKernelGGL(k_wf_nee, dim3(gridWork), dim3(kWfBlock), 0, s, a, w, k + 1);
+            hipLaunchKernelGGL(k_wf_hit, dim3(gHit), dim3(kWfBlock), 0, s, a, w, k);
+            hipLaunchKernelGGL(k_wf_nee, dim3(gNee), dim3(kWfBlock), 0, s, a, w, k + 1);
         }
     }
     return hipGetLastError();

@@ -36,7 +36,8 @@ def device_info(device: int = 0) -> dict:
     d = c_device_info()
     check(lib().hsr_hip_device_info(device, C.byref(d)))
     return {"name": d.name.decode(), "arch": d.arch.decode(), "compute_units": d.compute_units, "xcds": d.xcds,
-            "hbm_bytes": d.hbm_bytes, "lds_bytes_per_cu": d.lds_bytes_per_cu, "clock_mhz": d.clock_mhz}
+            "hbm_bytes": d.hbm_bytes, "lds_bytes_per_cu": d.lds_bytes_per_cu, "clock_mhz": d.clock_mhz,
+            "lds_bytes_per_block": d.lds_bytes_per_block}
 
 
 class Guide:

@@ -75,7 +75,7 @@ class c_run_stats(C.Structure):
 class c_device_info(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int32),
                 ("xcds", C.c_int32), ("hbm_bytes", C.c_int64), ("lds_bytes_per_cu", C.c_int32),
-                ("clock_mhz", C.c_int32)]
+                ("clock_mhz", C.c_int32), ("lds_bytes_per_block", C.c_int32), ("reserved0", C.c_int32)]
 
 
 class c_server_config(C.Structure):

@@ -273,6 +273,8 @@ typedef struct hsr_device_info {
     int64_t hbm_bytes;
     int32_t lds_bytes_per_cu;
     int32_t clock_mhz;
+    int32_t lds_bytes_per_block;
+    int32_t reserved0;
 } hsr_device_info;
 hsr_status hsr_hip_device_info(int32_t device, hsr_device_info *out);
 
