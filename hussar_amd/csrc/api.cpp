@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -137,6 +138,9 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_recHits);
     dfree(b->d_recNee);
     dfree(b->d_vel);
+    dfree(b->d_post);
+    dfree(b->d_post2);
+    if (b->d_postScratch) (void)hipFree(b->d_postScratch);
     if (b->d_wf) (void)hipFree(b->d_wf);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -615,6 +619,91 @@ hsr_status hsr_hip_fetch_frame(hsr_backend *b, float *out_c64, double *out_c128,
         if (out_c64) out_c64[i] = (float)v;
     }
     if (totalWeight) *totalWeight = tw;
+    return HSR_OK;
+}
+
+// The normalised frame (and, with fft, its forward 3-D DFT) in b->d_post.
+static hsr_status post_prepare(hsr_backend *b, bool fft) {
+    if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured");
+    const hsr_frame_config &f = b->fc;
+    if (fft && (f.chirpCount > hsr::kPostMaxAxis || f.samplesPerChirp > hsr::kPostMaxAxis ||
+                f.channelCount > hsr::kPostMaxAxis))
+        return fail(HSR_ERR_UNSUPPORTED, "frame axis longer than the device DFT supports");
+    HSR_HIP(hipSetDevice(b->device));
+    if (b->frameDoubles > b->postCap) {
+        HSR_HIP(hipStreamSynchronize(b->stream));
+        dfree(b->d_post);
+        dfree(b->d_post2);
+        b->postCap = 0;
+        hsr_status s;
+        if ((s = dalloc(&b->d_post, b->frameDoubles)) != HSR_OK) return s;
+        if ((s = dalloc(&b->d_post2, b->frameDoubles)) != HSR_OK) return s;
+        b->postCap = b->frameDoubles;
+    }
+    if (!b->d_postScratch) HSR_HIP(hipMalloc(&b->d_postScratch, hsr::post_scratch_bytes()));
+    const double inv = b->totalWeight != 0.0 ? 1.0 / b->totalWeight : 0.0;
+    HSR_HIP(hsr::launch_post_normalize(b->d_frame_sum, b->d_frame_comp, inv, b->frameDoubles, b->d_post, b->stream));
+    if (fft)
+        HSR_HIP(hsr::launch_post_fft3((double2 *)b->d_post, (double2 *)b->d_post2, f.chirpCount, f.samplesPerChirp,
+                                      f.channelCount, b->stream));
+    return HSR_OK;
+}
+
+hsr_status hsr_hip_fetch_frame_fft(hsr_backend *b, float *out_c64, double *out_c128) {
+    if (!b) return fail(HSR_ERR_INVALID, "null backend");
+    hsr_status s = post_prepare(b, true);
+    if (s != HSR_OK) return s;
+    std::vector<double> v(b->frameDoubles);
+    HSR_HIP(hipMemcpyAsync(v.data(), b->d_post, v.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HSR_HIP(hipStreamSynchronize(b->stream));
+    for (int64_t i = 0; i < b->frameDoubles; ++i) {
+        if (out_c128) out_c128[i] = v[i];
+        if (out_c64) out_c64[i] = (float)v[i];
+    }
+    return HSR_OK;
+}
+
+hsr_status hsr_hip_frame_peak(hsr_backend *b, int32_t fourier, int32_t index[3], double refined[3],
+                              double *magnitude) {
+    if (!b || !index || !refined) return fail(HSR_ERR_INVALID, "null argument");
+    hsr_status s = post_prepare(b, fourier != 0);
+    if (s != HSR_OK) return s;
+    const hsr_frame_config &f = b->fc;
+    const int64_t n = b->frameDoubles / 2;
+    int64_t *d_out = (int64_t *)((char *)b->d_postScratch + hsr::post_scratch_bytes() - sizeof(int64_t));
+    HSR_HIP(hsr::launch_post_argmax((const double2 *)b->d_post, n, b->d_postScratch, d_out, b->stream));
+    int64_t best = 0;
+    HSR_HIP(hipMemcpyAsync(&best, d_out, sizeof(best), hipMemcpyDeviceToHost, b->stream));
+    HSR_HIP(hipStreamSynchronize(b->stream));
+    // Frame::makeIndex (radar.h:497-504) and frequencyEstimation (radar.h:517-531, 631-643):
+    // per axis, from the last to the first, the neighbours at -off and +off (wrapping) give
+    // b - l/(l+m) when l > r, else b + r/(r+m)
+    const int dims[3] = {f.chirpCount, f.samplesPerChirp, f.channelCount};
+    int64_t rem = best;
+    for (int i = 2; i >= 0; --i) {
+        index[i] = (int32_t)(rem % dims[i]);
+        rem /= dims[i];
+    }
+    auto mag = [&](int64_t i, double *out) -> hsr_status {
+        double v[2];
+        HSR_HIP(hipMemcpy(v, b->d_post + 2 * i, sizeof(v), hipMemcpyDeviceToHost));
+        *out = std::hypot(v[0], v[1]);
+        return HSR_OK;
+    };
+    double m = 0.0;
+    if ((s = mag(best, &m)) != HSR_OK) return s;
+    if (magnitude) *magnitude = m;
+    int64_t off = 1;
+    for (int i = 2; i >= 0; --i) {
+        const int bi = index[i], mx = dims[i];
+        double l = 0.0, r = 0.0;
+        if ((s = mag(bi > 0 ? best - off : best + (int64_t)(mx - 1) * off, &l)) != HSR_OK) return s;
+        if ((s = mag(bi < mx - 1 ? best + off : best - (int64_t)(mx - 1) * off, &r)) != HSR_OK) return s;
+        if (l + m + r == 0.0) refined[i] = bi;
+        else if (l > r) refined[i] = bi - l / (l + m);
+        else refined[i] = bi + r / (r + m);
+        off *= mx;
+    }
     return HSR_OK;
 }
 

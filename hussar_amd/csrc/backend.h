@@ -97,6 +97,10 @@ struct hsr_backend {
     void *d_wf = nullptr;
     size_t wfBytes = 0;
     int64_t wfCap = int64_t(1) << 22;  // paths per wavefront batch
+    // post-processing (grow-only)
+    double *d_post = nullptr, *d_post2 = nullptr;
+    int64_t postCap = 0;  // doubles
+    void *d_postScratch = nullptr;
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> evTrav;  // pairs around traversal launches (wavefront)

@@ -115,6 +115,17 @@ hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum
                          double *frame_sum, double *frame_comp, hipStream_t stream);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t stream);
 size_t trace_lds_bytes(int N, int C);
+
+// ---------------------------------------------------------------- frame post-processing
+// post = (sum - comp) * inv, n doubles
+hipError_t launch_post_normalize(const double *sum, const double *comp, double inv, int64_t n, double *post,
+                                 hipStream_t stream);
+// in-place forward 3-D DFT of post [M][N][C] (complex double); tmp: a second buffer of that size
+hipError_t launch_post_fft3(double2 *post, double2 *tmp, int M, int N, int C, hipStream_t stream);
+// index of the largest |v| (first on ties) into *out; scratch: post_scratch_bytes()
+hipError_t launch_post_argmax(const double2 *v, int64_t n, void *scratch, int64_t *out, hipStream_t stream);
+size_t post_scratch_bytes();
+constexpr int kPostMaxAxis = 4096;  // longest DFT axis (line + twiddles in LDS)
 // resident workgroups of the path kernel per CU for this frame size
 int trace_blocks_per_cu(int N, int C, bool count);
 

@@ -220,6 +220,29 @@ class HipBackend:
             z = buf.view(np.complex64)
         return z.reshape(fc.chirpCount, fc.samplesPerChirp, fc.channelCount), tw.value
 
+    def fetch_fft(self, precision: str = "complex64") -> np.ndarray:
+        """Frame::fft of the frame (frame / totalWeight), computed on the device."""
+        fc = self.frame_config
+        n = fc.sampleCount()
+        if precision == "complex128":
+            buf = np.empty(2 * n, np.float64)
+            check(lib().hsr_hip_fetch_frame_fft(self._h, None, buf.ctypes.data_as(C.POINTER(C.c_double))))
+            z = buf[0::2] + 1j * buf[1::2]
+        else:
+            buf = np.empty(2 * n, np.float32)
+            check(lib().hsr_hip_fetch_frame_fft(self._h, buf.ctypes.data_as(C.POINTER(C.c_float)), None))
+            z = buf.view(np.complex64)
+        return z.reshape(fc.chirpCount, fc.samplesPerChirp, fc.channelCount)
+
+    def peak(self, fourier: bool = False):
+        """Frame::argmax and Frame::frequencyEstimation on the device: ((chirp, sample, channel),
+        refined position per axis, magnitude)."""
+        idx = (C.c_int32 * 3)()
+        ref = (C.c_double * 3)()
+        mag = C.c_double()
+        check(lib().hsr_hip_frame_peak(self._h, int(fourier), idx, ref, C.byref(mag)))
+        return tuple(idx), tuple(ref), mag.value
+
     def set_option(self, option: int, value: int) -> None:
         check(lib().hsr_hip_set_option(self._h, int(option), int(value)))
         if option == _lib.OPT_PIPELINE:

@@ -213,6 +213,20 @@ hsr_status hsr_guide_training_size(hsr_guide *guide, int32_t *nodeCount);
 hsr_status hsr_guide_get_training(hsr_guide *guide, int32_t *child, double *stat,
                                   int32_t capacity);
 
+/* Frame::fft (libradar/include/radar/radar.h:420-435: FFTW's forward 3-D transform over
+ * [chirp][sample][channel], unnormalised) of the frame (frame / totalWeight), computed on the
+ * device in FP64 by direct DFTs along each axis (any axis length up to 4096). The
+ * accumulation frame is unchanged. out_c64 / out_c128 as in hsr_hip_fetch_frame. */
+hsr_status hsr_hip_fetch_frame_fft(hsr_backend *backend, float *out_c64, double *out_c128);
+
+/* Frame::argmax (radar.h:534-546) and Frame::frequencyEstimation (radar.h:517-531,631-643)
+ * of the frame (fourier = 0) or of its 3-D DFT (fourier = 1), the search on the device:
+ * index = (chirp, sample, channel) of the largest |value| (the first one on ties), refined =
+ * the interpolated position per axis (b - l/(l+m) when the left neighbour is larger, else
+ * b + r/(r+m), neighbours wrapping), magnitude = |value| at index (may be NULL). */
+hsr_status hsr_hip_frame_peak(hsr_backend *backend, int32_t fourier, int32_t index[3], double refined[3],
+                              double *magnitude);
+
 /* Statistics of the last top-level call (hsr_hip_run, or all the runs of one
  * hsr_path_tracer_run / hsr_path_tracer_run_schedule), measured with HIP events on the
  * stream the kernels run on. */

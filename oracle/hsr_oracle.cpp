@@ -1293,6 +1293,41 @@ void orc_splat(int prec, const Rf *rf, int M, int N, int C, int ch, double delta
     }
 }
 
+// Frame::argmax (radar.h:534-546: the first largest |value|) and Frame::frequencyEstimation
+// (radar.h:517-531, 631-643: per axis from the last, neighbours at -off/+off wrapping, b -
+// l/(l+m) when l > r, else b + r/(r+m), b when all three are zero) of a complex128 frame
+// [M][N][C]. index[3] and refined[3] in (chirp, sample, channel) order.
+void orc_frame_peak(const double *frame, int M, int N, int C, int32_t *index, double *refined) {
+    const int dims[3] = {M, N, C};
+    const int64_t n = (int64_t)M * N * C;
+    auto mag = [&](int64_t i) { return std::hypot(frame[2 * i], frame[2 * i + 1]); };
+    int64_t best = 0;
+    double bm = mag(0);
+    for (int64_t i = 1; i < n; ++i) {
+        const double c = mag(i);
+        if (c > bm) {
+            bm = c;
+            best = i;
+        }
+    }
+    int64_t rem = best;
+    for (int i = 2; i >= 0; --i) {
+        index[i] = (int32_t)(rem % dims[i]);
+        rem /= dims[i];
+    }
+    int64_t off = 1;
+    for (int i = 2; i >= 0; --i) {
+        const int b = index[i], mx = dims[i];
+        const double l = mag(b > 0 ? best - off : best + (int64_t)(mx - 1) * off);
+        const double m = bm;
+        const double r = mag(b < mx - 1 ? best + off : best - (int64_t)(mx - 1) * off);
+        if (l + m + r == 0.0) refined[i] = b;
+        else if (l > r) refined[i] = b - l / (l + m);
+        else refined[i] = b + r / (r + m);
+        off *= mx;
+    }
+}
+
 // ---- guide
 void *orc_guide_create(float uniformProb, float splitThreshold, int secondMoment) {
     Guide *g = new Guide();

@@ -51,6 +51,7 @@ def lib():
                                     C.c_float, P(C.c_int32), P(C.c_double), P(C.c_int64)]
         L.orc_splat.argtypes = [C.c_int, P(c_rf_config), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                 C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, P(C.c_double)]
+        L.orc_frame_peak.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_int, P(C.c_int32), P(C.c_double)]
         L.orc_mesh_set_velocities.restype = C.c_int
         L.orc_mesh_set_velocities.argtypes = [vp, P(C.c_float), C.c_int64]
         L.orc_guide_create.restype = vp
@@ -202,6 +203,17 @@ class OracleGuide:
             lib().orc_guide_destroy(self.h)
         except Exception:
             pass
+
+
+def frame_peak(z: np.ndarray):
+    """Frame::argmax + frequencyEstimation of a complex frame [M, N, C]."""
+    z = np.ascontiguousarray(z, np.complex128)
+    M, N, C_ = z.shape
+    buf = z.view(np.float64).reshape(-1)
+    idx = np.zeros(3, np.int32)
+    ref = np.zeros(3, np.float64)
+    lib().orc_frame_peak(_p(buf, C.c_double), M, N, C_, _p(idx, C.c_int32), _p(ref, C.c_double))
+    return tuple(int(i) for i in idx), tuple(float(r) for r in ref)
 
 
 def frame_size(fc) -> int:

@@ -96,3 +96,19 @@ def test_splat_fp32_instance_close(oracle):
     b = splat(oracle, rf, 256, 1, 0, dt, 1.0 + 0j, 1.0, prec=2)
     # FP32 phase of a ~4.8e3 rad argument: error of order 1e-3 relative
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 5e-3
+
+
+@pytest.mark.parametrize("dist", [1.5, 3.7, 12.0])
+def test_frequency_estimation_recovers_the_fractional_bin(oracle, dist):
+    """Frame::frequencyEstimation (radar.h:631-643) of a single splat: the neighbours of the
+    peak hold |sinc(d)| and |sinc(1-d)| times the same amplitude, so b + r/(r+m) (or b -
+    l/(l+m)) is exactly the fractional bin x of the closed form; axes of length one give 0.5."""
+    rf = RFConfig()
+    N = 256
+    dt = 2 * dist / SPEED_OF_LIGHT
+    z = splat(oracle, rf, N, 1, 0, dt, 1.0 + 0.5j, 1.0)
+    _, x = closed_form(rf, N, dt, 1.0 + 0.5j, 1.0)
+    idx, ref = oracle.frame_peak(z.reshape(1, N, 1))
+    assert idx == (0, int(np.floor(x + 0.5)) % N, 0)
+    assert abs(ref[1] - x) < 1e-9
+    assert ref[0] == 0.5 and ref[2] == 0.5
