@@ -218,6 +218,48 @@ def serving_bench(args):
     }
 
 
+def precision_bench(args):
+    """Targets at 5, 50 and 150 m over a 1M-triangle rough ground, 1 Tx x 4 Rx, 512 bins,
+    2^24 paths, depth 4: throughput of each IF-phase variant and its relative L2 against the
+    FP64 oracle on a bounded sample (the first 2^16 paths)."""
+    from hussar_amd._lib import OPT_PHASE_MODE
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bindings as ob
+    mesh = scenes.precision_mesh(seed=2)
+    rf = scenes.dihedral_rf()
+    sc = scenes.radar_array(rf, channels=4, position=(0.0, 0.0, 0.0))
+    kn = H.Knobs(maxDepth=4, doGuiding=False)
+    fc = H.FrameConfig(1, 512, 4)
+    n, sample = 1 << 24, 1 << 16
+    b = H.HipBackend(mesh)
+    b._configure(fc)
+    om = ob.OracleMesh.from_mesh(mesh)
+    zf, twf, _, _ = ob.render(om, sc, kn, fc, prec=ob.PREC_FP64, count=sample, threads=min(16, os.cpu_count() or 1))
+    zf = zf / twf
+    out = {}
+    for mode, name in ((0, "fp64"), (2, "double-float"), (1, "fp32")):
+        b.set_option(OPT_PHASE_MODE, mode)
+        b.clear()
+        b.run_raw(sc, kn, 0, sample)
+        z, _ = b.fetch("complex128")
+        rl2 = float(np.linalg.norm(z - zf) / np.linalg.norm(zf))
+        b.clear()
+        b.run_raw(sc, kn, 0, n)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.clear()
+            b.run_raw(sc, kn, 0, n)
+        b.fetch()
+        dt = (time.perf_counter() - t0) / args.steps
+        out[name] = {"paths_per_s": n / dt, "ms_per_frame": 1e3 * dt, "if_rel_l2_vs_fp64": rl2}
+    return {"metric": "paths/s & IF rel-L2 vs FP64 per phase variant", "value": out["fp64"]["paths_per_s"],
+            "unit": "paths/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "data": "synthetic",
+            "config": {"workload": "plates + trihedrals at 5/50/150 m over a 1M-triangle rough ground (seed 2), "
+                                   "1 Tx x 4 Rx, 512 bins, 2^24 paths, depth 4, no guiding",
+                       "triangles": int(mesh.triangle_count()), "accuracy_sample_paths": sample},
+            "variants": out}
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: MIB MiB read + MIB MiB written per dispatch. Under
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it gives the factor between the counters' KiB and
@@ -259,6 +301,9 @@ def main():
     threads = min(16, os.cpu_count() or 1)
     if args.workload == "serving":
         print(json.dumps(serving_bench(args)), flush=True)
+        return
+    if args.workload == "precision":
+        print(json.dumps(precision_bench(args)), flush=True)
         return
 
     mesh, scene, knobs, fc, sched, desc = workload(args.workload)

@@ -281,7 +281,8 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
             b->optPipeline = (int32_t)value;
             return HSR_OK;
         case HSR_OPT_PHASE_MODE:
-            if (value < 0 || value > 1) return fail(HSR_ERR_UNSUPPORTED, "phase mode must be 0 (FP64) or 1 (FP32)");
+            if (value < 0 || value > 2)
+                return fail(HSR_ERR_UNSUPPORTED, "phase mode must be 0 (FP64), 1 (FP32) or 2 (double-float)");
             b->optPhaseMode = (int32_t)value;
             return HSR_OK;
         default: return fail(HSR_ERR_INVALID, "unknown option");

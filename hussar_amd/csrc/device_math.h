@@ -184,6 +184,37 @@ __host__ __device__ inline void sincos_d(double x, double *s_out, double *c_out)
     *c_out = c;
 }
 
+// ----------------------------------------------------------------- double-float
+// An unevaluated pair hi + lo of floats carrying about 48 significant bits: Knuth's two-sum,
+// the fused-multiply-add two-product (Ogita, Rump and Oishi 2005) and the usual
+// renormalisation. Used by the double-float variant of the IF phase (the arithmetic of GPUs
+// without fast FP64).
+struct df {
+    float hi, lo;
+};
+__device__ inline df df_two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ inline df df_fast_two_sum(float a, float b) {
+    const float s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ inline df df_add(df a, df b) {
+    df s = df_two_sum(a.hi, b.hi);
+    const df t = df_two_sum(a.lo, b.lo);
+    s.lo = s.lo + t.hi;
+    s = df_fast_two_sum(s.hi, s.lo);
+    s.lo = s.lo + t.lo;
+    return df_fast_two_sum(s.hi, s.lo);
+}
+__device__ inline df df_mul(df a, df b) {
+    const float p = a.hi * b.hi;
+    const float e = __fmaf_rn(a.hi, b.hi, -p);
+    return df_fast_two_sum(p, e + (a.hi * b.lo + a.lo * b.hi));
+}
+
 // UniformSampleSphere (libhussar/include/hussar/core/sampling.h:38-44) in double
 __host__ __device__ inline d3 uniform_sample_sphere(double u0, double u1) {
     const double phi = (2.0 * 3.14159265358979323846) * u0;

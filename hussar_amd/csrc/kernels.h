@@ -27,7 +27,7 @@ struct TraceArgs {
     int32_t onlyIndirect, maxDepth, poDiffraction, doGuiding, doFiltering, filteringSphere, filterGuiding,
         primaryOrder, produceDebugImage;
     float filteringMin, filteringMax, filteringRadius;
-    int32_t phaseMode;  // 0 FP64 chain (default), 1 FP32 as the reference
+    int32_t phaseMode;  // IF phase: 0 FP64 (default), 1 FP32 as the reference, 2 double-float
     // frame
     int32_t N, C;
     // work: Halton indices offset + [0, count); wave w takes [w*chunk, (w+1)*chunk)

@@ -620,6 +620,20 @@ __device__ inline Event make_event(const TraceArgs &a, double tn, cf v, float we
         const float dtf = (float)dt;
         const float arg = ((2.0f * kPiF) * (a.startFreq - dtf * a.freqSlope / 2.0f)) * dtf;
         ph = {cosf(arg), sinf(arg)};
+    } else if (a.phaseMode == 2) {
+        // double-float: turns = (f0 - Δt S/2) Δt in float pairs, the fraction of a turn, then
+        // an FP32 sin/cos of the reduced angle
+        const float th = (float)dt;
+        const df t{th, (float)(dt - (double)th)};
+        const df half = df_mul(t, df{a.freqSlope * 0.5f, 0.0f});
+        const df turns = df_mul(df_add(df{a.startFreq, 0.0f}, df{-half.hi, -half.lo}), t);
+        const float whole = floorf(turns.hi);
+        const df fr = df_add(df{turns.hi - whole, 0.0f}, df{turns.lo, 0.0f});
+        float f = fr.hi + fr.lo;
+        f = f - floorf(f);
+        float sf, cf_;
+        sincosf((2.0f * kPiF) * f, &sf, &cf_);
+        ph = {cf_, sf};
     } else {
         const double turns = ((double)a.startFreq - dt * (double)a.freqSlope / 2.0) * dt;
         double s, c;

@@ -264,7 +264,9 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *  HSR_OPT_PIPELINE      0 = fused path kernel, 1 = wavefront stages (default: path
  *                        state in device memory, persistent traversal kernels). Both
  *                        give the same hits, NEE outcomes and frame up to summation order.
- *  HSR_OPT_PHASE_MODE    0 = FP64 phase (default), 1 = FP32 phase as in the reference. */
+ *  HSR_OPT_PHASE_MODE    IF phase of every contribution (measureRay, integrator.h:107-110):
+ *                        0 = FP64 (default), 1 = FP32 as in the reference, 2 = double-float
+ *                        (pairs of floats, the arithmetic of GPUs without fast FP64). */
 #define HSR_OPT_COUNTERS 1
 #define HSR_OPT_RECORD_HITS 2
 #define HSR_OPT_PATHS_PER_LAUNCH 3
