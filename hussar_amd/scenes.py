@@ -186,6 +186,67 @@ def vehicle_mesh(seed: int = 1, triangles: int = 200_000, extent=(2.0, 1.5, 5.0)
     return mesh, np.array([0.0, ext[1] / 2, zc])
 
 
+def _grid(origin, du, dv, nu, nv, rough, normal, seed):
+    """nu x nv quads spanning origin + [0,1]du + [0,1]dv with vertices displaced along
+    `normal` by U[-rough, rough] (shared vertices, 2 triangles per quad)."""
+    uu, vv = np.meshgrid(np.linspace(0.0, 1.0, nu + 1), np.linspace(0.0, 1.0, nv + 1), indexing="ij")
+    off = (uniforms(seed, (nu + 1) * (nv + 1)) * 2.0 - 1.0) * rough
+    V = (np.asarray(origin)[None] + uu.reshape(-1, 1) * np.asarray(du)[None] + vv.reshape(-1, 1) * np.asarray(dv)[None]
+         + off[:, None] * np.asarray(normal)[None])
+    ii, jj = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    v00 = (ii * (nv + 1) + jj).reshape(-1)
+    v10, v01 = v00 + (nv + 1), v00 + 1
+    v11 = v10 + 1
+    T = np.concatenate([np.stack([v00, v10, v11], 1), np.stack([v00, v11, v01], 1)], axis=1).reshape(-1, 3)
+    return V.astype(np.float32), T.astype(np.int32)
+
+
+def urban_mesh(seed: int = 4, facade_nz: int = 1000, facade_ny: int = 300, ground_nx: int = 400,
+               ground_nz: int = 1000, vehicles: int = 4, poles: int = 20):
+    """A street canyon along +z: two rough facades (x = ±8 m, 15 m high, z 5..105 m), a rough
+    street surface (y = 0), `vehicles` vehicle-like clutter blocks in lanes x = ±2 m at
+    10..40 m moving along ±z at U[0, 20] m/s, and `poles` static poles. With the defaults:
+    2,000,000 facade and ground triangles plus 8,000 vehicle and 240 pole triangles. Returns
+    (mesh, per-triangle velocities float32 [T, 3])."""
+    m = TriangleMesh()
+    vel = []
+    parts = [
+        _grid([-8.0, 0.0, 5.0], [0.0, 0.0, 100.0], [0.0, 15.0, 0.0], facade_nz, facade_ny, 0.05, [1.0, 0, 0], seed),
+        _grid([8.0, 0.0, 5.0], [0.0, 15.0, 0.0], [0.0, 0.0, 100.0], facade_ny, facade_nz, 0.05, [1.0, 0, 0], seed + 1),
+        _grid([-8.0, 0.0, 5.0], [16.0, 0.0, 0.0], [0.0, 0.0, 100.0], ground_nx, ground_nz, 0.02, [0, 1.0, 0], seed + 2),
+    ]
+    for V, T in parts:
+        m.append(V, T)
+        vel.append(np.zeros((T.shape[0], 3), np.float32))
+    u = uniforms(seed + 3, 2 * vehicles).reshape(vehicles, 2)
+    for k in range(vehicles):
+        car, _ = vehicle_mesh(seed=40 + seed + k, triangles=2000, extent=(1.8, 1.5, 4.5), range_lo=10.0, range_hi=40.0)
+        V = car.vertexBuffer.copy()
+        V[:, 0] += np.float32(-2.0 if k % 2 == 0 else 2.0)
+        m.append(V, car.indexBuffer)
+        speed = 20.0 * u[k, 0] * (1.0 if u[k, 1] < 0.5 else -1.0)
+        vel.append(np.tile(np.float32([0.0, 0.0, speed]), (car.triangle_count(), 1)))
+    if poles:
+        zs = 7.5 + 5.0 * np.arange(poles // 2)
+        lo = np.concatenate([np.stack([np.full_like(zs, -6.5), np.zeros_like(zs), zs], 1),
+                             np.stack([np.full_like(zs, 6.35), np.zeros_like(zs), zs], 1)])
+        pm = _assemble(_box_quads(lo, lo + np.array([0.15, 6.0, 0.15])).reshape(-1, 4, 3))
+        m.append(pm.vertexBuffer, pm.indexBuffer)
+        vel.append(np.zeros((pm.triangle_count(), 3), np.float32))
+    return m, np.concatenate(vel)
+
+
+def mimo_array(rf: RFConfig, tx: int = 3, rx: int = 4, position=(0.0, 1.0, 0.0)) -> Scene:
+    """TDM-MIMO array looking down +z: rx receivers at λ/2 and tx transmitters at rx·λ/2
+    along x, giving tx·rx virtual channels on a λ/2 grid."""
+    base = radar_array(rf, channels=rx, position=position)
+    lam = SPEED_OF_LIGHT / rf.startFreq
+    p = np.asarray(position, np.float64)
+    txs = [NFAntenna((p + k * rx * 0.5 * lam * np.array([1.0, 0, 0])).astype(np.float32), np.eye(3, dtype=np.float32))
+           for k in range(tx)]
+    return Scene(rfConfig=rf, tx=txs[0], rx=base.rx, rxs=base.rxs, txs=txs)
+
+
 def radar_array(rf: RFConfig, channels: int = 4, position=(0.0, 0.75, 0.0),
                 rotation: np.ndarray | None = None) -> Scene:
     """1 Tx at ``position`` and ``channels`` receivers at position + k·λ/2 along the array's
