@@ -29,7 +29,7 @@ from hussar_amd._lib import OPT_COUNTERS, OPT_PIPELINE, OPT_RECORD_HITS  # noqa:
 
 METRIC = "paths/s & frames/s at ≤1e-3 IF rel-L2 vs FP64; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES = 64, 48
+NODE_BYTES, TRI_BYTES = 128, 48  # four-wide BVH node, triangle slot
 
 
 def workload(name):

@@ -1,6 +1,9 @@
-// Binned-SAH BVH2 builder (host). Produces the two-children-per-node layout the traversal
-// kernels fetch as four float4 per node (Aila & Laine 2009 style), with triangle slots in leaf
-// order. Subtrees above a size threshold are built in parallel with std::thread.
+// Binned-SAH BVH builder (host). A BVH2 is built first (binned SAH over all three axes, boxes
+// widened conservatively), then collapsed into four-wide nodes: starting from a node's two
+// children, the inner child with the largest surface area is replaced by its two children
+// until four children are reached or only leaves remain. The kernels fetch a four-wide node
+// as eight float4 (child boxes as six structure-of-arrays rows, then the child references),
+// so one dependent fetch tests four boxes; triangle slots stay in leaf order.
 #include "bvh.h"
 
 #include <algorithm>
@@ -192,7 +195,79 @@ struct Builder {
     }
 };
 
+// BVH2 child c of node n: its widened box and reference
+void child2(const std::vector<float> &n2, int32_t n, int c, float lo[3], float hi[3], int32_t &ref) {
+    const float *p = &n2[16 * (size_t)n];
+    const int o = c ? 4 : 0;
+    lo[0] = p[o + 0];
+    hi[0] = p[o + 1];
+    lo[1] = p[o + 2];
+    hi[1] = p[o + 3];
+    lo[2] = p[8 + 2 * c];
+    hi[2] = p[9 + 2 * c];
+    std::memcpy(&ref, &p[12 + c], 4);
+}
+
+// Collapses the BVH2 subtree of node n2 into a four-wide node appended to n4 (depth-first);
+// returns its index.
+int32_t collapse_node(const std::vector<float> &n2, int32_t node, std::vector<float> &n4, int depth, int &maxd) {
+    struct E {
+        float lo[3], hi[3];
+        int32_t ref;
+        float area() const {
+            const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+            return dx * dy + dy * dz + dz * dx;
+        }
+    };
+    E e[4];
+    int n = 2;
+    child2(n2, node, 0, e[0].lo, e[0].hi, e[0].ref);
+    child2(n2, node, 1, e[1].lo, e[1].hi, e[1].ref);
+    while (n < 4) {
+        int bi = -1;
+        float ba = -1.0f;
+        for (int i = 0; i < n; ++i)
+            if (e[i].ref >= 0 && e[i].area() > ba) {
+                ba = e[i].area();
+                bi = i;
+            }
+        if (bi < 0) break;
+        const int32_t r = e[bi].ref;
+        child2(n2, r, 0, e[bi].lo, e[bi].hi, e[bi].ref);
+        child2(n2, r, 1, e[n].lo, e[n].hi, e[n].ref);
+        ++n;
+    }
+    maxd = std::max(maxd, depth);
+    const int32_t me = (int32_t)(n4.size() / 32);
+    n4.resize(n4.size() + 32, 0.0f);
+    int32_t refs[4];
+    for (int i = 0; i < 4; ++i) {
+        if (i < n) refs[i] = e[i].ref >= 0 ? collapse_node(n2, e[i].ref, n4, depth + 1, maxd) : e[i].ref;
+        else refs[i] = kNoChild;
+    }
+    float *q = &n4[32 * (size_t)me];
+    for (int i = 0; i < 4; ++i) {
+        const bool used = i < n;
+        q[0 + i] = used ? e[i].lo[0] : 0.0f;
+        q[4 + i] = used ? e[i].hi[0] : 0.0f;
+        q[8 + i] = used ? e[i].lo[1] : 0.0f;
+        q[12 + i] = used ? e[i].hi[1] : 0.0f;
+        q[16 + i] = used ? e[i].lo[2] : 0.0f;
+        q[20 + i] = used ? e[i].hi[2] : 0.0f;
+        std::memcpy(&q[24 + i], &refs[i], 4);
+    }
+    return me;
+}
+
 }  // namespace
+
+void collapse4(const std::vector<float> &n2, std::vector<float> &n4, int32_t &depth4) {
+    n4.clear();
+    n4.reserve(n2.size());
+    int maxd = 0;
+    collapse_node(n2, 0, n4, 0, maxd);
+    depth4 = maxd;
+}
 
 bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH &out, int threads) {
     (void)threads;
@@ -250,8 +325,8 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
             B.fill(nodes, 0, 0, mid, (int32_t)nt, r0, r1, 0);
         }
     }
-    out.nodes.swap(nodes);
-    out.nodeCount = (int32_t)(out.nodes.size() / 16);
+    collapse4(nodes, out.nodes, out.depth4);
+    out.nodeCount = (int32_t)(out.nodes.size() / 32);
     out.slotCount = slotCount;
     out.maxDepth = B.maxDepth.load();
     out.maxLeaf = B.maxLeaf.load();
@@ -267,7 +342,8 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
         r[4] = b[0] - a[0]; r[5] = b[1] - a[1]; r[6] = b[2] - a[2]; r[7] = 0.0f;
         r[8] = c[0] - a[0]; r[9] = c[1] - a[1]; r[10] = c[2] - a[2]; r[11] = 0.0f;
     }
-    return out.maxDepth < kMaxDepth;
+    // the closest-hit traversal keeps at most three pending children per level of the path
+    return out.maxDepth < kMaxDepth && 3 * (out.depth4 + 1) + 1 <= kStackEntries;
 }
 
 }  // namespace hsr

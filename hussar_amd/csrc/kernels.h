@@ -9,7 +9,8 @@ namespace hsr {
 constexpr int kMaxChannels = 16;
 constexpr int kTraceBlock = 256;      // threads per workgroup of the path kernel (4 waves)
 constexpr int kStackShort = 16;       // LDS-staged traversal stack entries per thread
-constexpr int kStackSpill = 64;       // further entries per thread in private memory (BVH depth <= 80)
+constexpr int kStackSpill = 128;      // further entries per thread in private memory (16 + 128 =
+                                      // kStackEntries of bvh.h: three per level of the 4-wide tree)
 
 struct DevAntenna {
     float pos[3];
@@ -34,7 +35,7 @@ struct TraceArgs {
     int64_t offset, count, chunk;
     float sampleWeight;
     int32_t isFinal;
-    // mesh: BVH2 nodes (4 float4 each) and triangles (3 float4 each)
+    // mesh: four-wide BVH nodes (8 float4 each) and triangles (3 float4 each)
     const float4 *nodes;
     const float4 *tris;
     // per-workgroup partial frames [gridDim.x][N*C] (complex double)

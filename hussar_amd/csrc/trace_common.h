@@ -49,9 +49,11 @@ struct Stack {
     }
 };
 
-// BVH2 node (64 B): c0.x lo/hi, c0.y lo/hi | c1.x lo/hi, c1.y lo/hi | c0.z lo/hi, c1.z lo/hi |
-// child0, child1. A child >= 0 is an inner node; < 0 encodes a leaf: bit 31 set, bits 27-30
-// count-1, bits 0-26 first triangle slot.
+// Four-wide BVH node (128 B = 8 float4): lo.x[4] | hi.x[4] | lo.y[4] | hi.y[4] | lo.z[4] |
+// hi.z[4] | child[4] | unused. A child >= 0 is an inner node; kNoChild an unused slot; any
+// other negative value a leaf: bit 31 set, bits 27-30 count-1, bits 0-26 first triangle slot.
+constexpr int32_t kNoChild = 0x7fffffff;
+
 __device__ inline void leaf_decode(int32_t c, int &first, int &count) {
     first = c & 0x07FFFFFF;
     count = ((c >> 27) & 0xF) + 1;
@@ -74,6 +76,95 @@ __device__ inline bool slab(float lx, float hx, float ly, float hy, float lz, fl
     return t0 <= t1 && t0 <= tbest;
 }
 
+struct Node4 {
+    float4 lx, hx, ly, hy, lz, hz;
+    int4 ch;
+};
+
+__device__ inline Node4 load_node4(const float4 *__restrict__ nodes, int32_t n) {
+    const float4 *p = nodes + 8 * (int64_t)n;
+    Node4 q;
+    q.lx = p[0];
+    q.hx = p[1];
+    q.ly = p[2];
+    q.hy = p[3];
+    q.lz = p[4];
+    q.hz = p[5];
+    q.ch = __builtin_bit_cast(int4, p[6]);
+    return q;
+}
+
+template <int i> __device__ inline float lane4(const float4 &v) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+template <int i> __device__ inline int32_t lane4(const int4 &v) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+template <int i> __device__ inline bool slab_child(const Node4 &q, const SlabRay &r, float tb, float &te) {
+    return slab(lane4<i>(q.lx), lane4<i>(q.hx), lane4<i>(q.ly), lane4<i>(q.hy), lane4<i>(q.lz), lane4<i>(q.hz), r,
+                tb, te) &&
+           lane4<i>(q.ch) != kNoChild;
+}
+
+// Boxes of a node's four children against one ray: bit i of the mask = child i is hit before tb.
+__device__ inline uint32_t slab4(const Node4 &q, const SlabRay &r, float tb, float te[4]) {
+    uint32_t m = 0;
+    if (slab_child<0>(q, r, tb, te[0])) m |= 1u;
+    if (slab_child<1>(q, r, tb, te[1])) m |= 2u;
+    if (slab_child<2>(q, r, tb, te[2])) m |= 4u;
+    if (slab_child<3>(q, r, tb, te[3])) m |= 8u;
+    return m;
+}
+
+__device__ inline int32_t child_ref(const int4 &c, int i) {
+    return i == 0 ? c.x : (i == 1 ? c.y : (i == 2 ? c.z : c.w));
+}
+
+__device__ inline void cswap(float &ka, int32_t &va, float &kb, int32_t &vb) {
+    if (kb < ka) {
+        const float tk = ka;
+        ka = kb;
+        kb = tk;
+        const int32_t tv = va;
+        va = vb;
+        vb = tv;
+    }
+}
+
+// Closest-hit order from a visited inner node with hit mask m != 0: the hit children sorted by
+// entry distance (a five-comparator network over four keys, misses keyed +inf), the nearest
+// returned as the next node, the others pushed farthest first so that they pop nearest first.
+__device__ inline int32_t order_children(const Node4 &q, uint32_t m, const float te[4], Stack &st) {
+    if ((m & (m - 1)) == 0) return child_ref(q.ch, __builtin_ctz(m));
+    const float inf = __builtin_inff();
+    float k0 = (m & 1u) ? te[0] : inf, k1 = (m & 2u) ? te[1] : inf;
+    float k2 = (m & 4u) ? te[2] : inf, k3 = (m & 8u) ? te[3] : inf;
+    int32_t v0 = q.ch.x, v1 = q.ch.y, v2 = q.ch.z, v3 = q.ch.w;
+    cswap(k0, v0, k1, v1);
+    cswap(k2, v2, k3, v3);
+    cswap(k0, v0, k2, v2);
+    cswap(k1, v1, k3, v3);
+    cswap(k1, v1, k2, v2);
+    const int n = __popc(m);
+    if (n == 4) st.push(v3);
+    if (n >= 3) st.push(v2);
+    st.push(v1);
+    return v0;
+}
+
+// Any-hit order from a visited inner node with hit mask m != 0: the lowest hit child next,
+// the others pushed.
+__device__ inline int32_t any_children(const Node4 &q, uint32_t m, Stack &st) {
+    const int first = __builtin_ctz(m);
+    m &= m - 1;
+    while (m) {
+        st.push(child_ref(q.ch, __builtin_ctz(m)));
+        m &= m - 1;
+    }
+    return child_ref(q.ch, first);
+}
+
 template <bool kCount>
 __device__ inline int32_t closest_hit(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
                                       f3 d, Stack &st, float &tout, int32_t &slot, uint32_t &cn, uint32_t &ct) {
@@ -83,29 +174,16 @@ __device__ inline int32_t closest_hit(const float4 *__restrict__ nodes, const fl
     st.sp = 0;
     int32_t node = 0;
     while (true) {
-        while (node >= 0) {
+        if (node >= 0) {
             if (kCount) ++cn;
-            const float4 a = nodes[4 * node + 0];
-            const float4 b = nodes[4 * node + 1];
-            const float4 c = nodes[4 * node + 2];
-            const int4 ch = __builtin_bit_cast(int4, nodes[4 * node + 3]);
-            float te0, te1;
-            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, r, best, te0);
-            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, r, best, te1);
-            if (h0 && h1) {
-                const bool first0 = te0 <= te1;
-                st.push(first0 ? ch.y : ch.x);
-                node = first0 ? ch.x : ch.y;
-            } else if (h0) {
-                node = ch.x;
-            } else if (h1) {
-                node = ch.y;
-            } else {
-                if (st.sp == 0) goto done;
-                node = st.pop();
+            const Node4 q = load_node4(nodes, node);
+            float te[4];
+            const uint32_t m = slab4(q, r, best, te);
+            if (m) {
+                node = order_children(q, m, te, st);
+                continue;
             }
-        }
-        {
+        } else {
             int first, count;
             leaf_decode(node, first, count);
             for (int i = first; i < first + count; ++i) {
@@ -126,7 +204,6 @@ __device__ inline int32_t closest_hit(const float4 *__restrict__ nodes, const fl
         if (st.sp == 0) break;
         node = st.pop();
     }
-done:
     tout = best;
     slot = best_slot;
     return best_id;
@@ -139,38 +216,27 @@ __device__ inline bool occluded(const float4 *__restrict__ nodes, const float4 *
     st.sp = 0;
     int32_t node = 0;
     while (true) {
-        while (node >= 0) {
+        if (node >= 0) {
             if (kCount) ++cn;
-            const float4 a = nodes[4 * node + 0];
-            const float4 b = nodes[4 * node + 1];
-            const float4 c = nodes[4 * node + 2];
-            const int4 ch = __builtin_bit_cast(int4, nodes[4 * node + 3]);
-            float te0, te1;
-            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, r, tfar, te0);
-            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, r, tfar, te1);
-            if (h0 && h1) {
-                const bool first0 = te0 <= te1;
-                st.push(first0 ? ch.y : ch.x);
-                node = first0 ? ch.x : ch.y;
-            } else if (h0) {
-                node = ch.x;
-            } else if (h1) {
-                node = ch.y;
-            } else {
-                if (st.sp == 0) return false;
-                node = st.pop();
+            const Node4 q = load_node4(nodes, node);
+            float te[4];
+            const uint32_t m = slab4(q, r, tfar, te);
+            if (m) {
+                node = any_children(q, m, st);
+                continue;
             }
-        }
-        int first, count;
-        leaf_decode(node, first, count);
-        for (int i = first; i < first + count; ++i) {
-            if (kCount) ++ct;
-            const float4 p = tris[3 * i + 0];
-            const float4 e1 = tris[3 * i + 1];
-            const float4 e2 = tris[3 * i + 2];
-            if (tri_intersect({p.x, p.y, p.z}, {e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, o, d, kEps, tfar) <
-                __builtin_inff())
-                return true;
+        } else {
+            int first, count;
+            leaf_decode(node, first, count);
+            for (int i = first; i < first + count; ++i) {
+                if (kCount) ++ct;
+                const float4 p = tris[3 * i + 0];
+                const float4 e1 = tris[3 * i + 1];
+                const float4 e2 = tris[3 * i + 2];
+                if (tri_intersect({p.x, p.y, p.z}, {e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, o, d, kEps, tfar) <
+                    __builtin_inff())
+                    return true;
+            }
         }
         if (st.sp == 0) return false;
         node = st.pop();
@@ -499,25 +565,11 @@ __device__ inline void trav_run(const float4 *__restrict__ nodes, const float4 *
     for (int s = 0; s < steps && t.node != kDone; ++s) {
         if (t.node >= 0) {
             if (kCount) ++cn;
-            const float4 a = nodes[4 * t.node + 0];
-            const float4 b = nodes[4 * t.node + 1];
-            const float4 c = nodes[4 * t.node + 2];
-            const int4 ch = __builtin_bit_cast(int4, nodes[4 * t.node + 3]);
-            const float tb = t.tfar;
-            float te0, te1;
-            const bool h0 = slab(a.x, a.y, a.z, a.w, c.x, c.y, r, tb, te0);
-            const bool h1 = slab(b.x, b.y, b.z, b.w, c.z, c.w, r, tb, te1);
-            if (h0 && h1) {
-                const bool first0 = te0 <= te1;
-                st.push(first0 ? ch.y : ch.x);
-                t.node = first0 ? ch.x : ch.y;
-            } else if (h0) {
-                t.node = ch.x;
-            } else if (h1) {
-                t.node = ch.y;
-            } else {
-                t.node = st.sp ? st.pop() : kDone;
-            }
+            const Node4 q = load_node4(nodes, t.node);
+            float te[4];
+            const uint32_t m = slab4(q, r, t.tfar, te);
+            if (m) t.node = kAny ? any_children(q, m, st) : order_children(q, m, te, st);
+            else t.node = st.sp ? st.pop() : kDone;
         } else {
             int first, count;
             leaf_decode(t.node, first, count);

@@ -300,34 +300,44 @@ __device__ inline void packet_run(const float4 *__restrict__ nodes, const float4
     for (int s = 0; s < steps && pk.node != kDone; ++s) {
         if (pk.node >= 0) {
             if (kCount) ++cn;
-            const float4 a = nodes[4 * pk.node + 0];
-            const float4 b = nodes[4 * pk.node + 1];
-            const float4 c = nodes[4 * pk.node + 2];
-            const int4 ch = __builtin_bit_cast(int4, nodes[4 * pk.node + 3]);
-            const float a0 = a.x - pk.o.x, a1 = a.y - pk.o.x, a2 = a.z - pk.o.y, a3 = a.w - pk.o.y;
-            const float b0 = b.x - pk.o.x, b1 = b.y - pk.o.x, b2 = b.z - pk.o.y, b3 = b.w - pk.o.y;
-            const float c0 = c.x - pk.o.z, c1 = c.y - pk.o.z, c2 = c.z - pk.o.z, c3 = c.w - pk.o.z;
-            bool h0 = false, h1 = false;
-            float te0 = __builtin_inff(), te1 = __builtin_inff();
+            const Node4 q = load_node4(nodes, pk.node);
+            // the children's faces relative to the packet's shared origin, then per ray a slab
+            // test with its own direction: a child is visited when any live ray hits it
+            float dl[4][6];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float lx = i == 0 ? q.lx.x : (i == 1 ? q.lx.y : (i == 2 ? q.lx.z : q.lx.w));
+                const float hx = i == 0 ? q.hx.x : (i == 1 ? q.hx.y : (i == 2 ? q.hx.z : q.hx.w));
+                const float ly = i == 0 ? q.ly.x : (i == 1 ? q.ly.y : (i == 2 ? q.ly.z : q.ly.w));
+                const float hy = i == 0 ? q.hy.x : (i == 1 ? q.hy.y : (i == 2 ? q.hy.z : q.hy.w));
+                const float lz = i == 0 ? q.lz.x : (i == 1 ? q.lz.y : (i == 2 ? q.lz.z : q.lz.w));
+                const float hz = i == 0 ? q.hz.x : (i == 1 ? q.hz.y : (i == 2 ? q.hz.z : q.hz.w));
+                dl[i][0] = lx - pk.o.x;
+                dl[i][1] = hx - pk.o.x;
+                dl[i][2] = ly - pk.o.y;
+                dl[i][3] = hy - pk.o.y;
+                dl[i][4] = lz - pk.o.z;
+                dl[i][5] = hz - pk.o.z;
+            }
+            uint32_t m = 0;
 #pragma unroll
             for (int r = 0; r < kPacket; ++r) {
                 if (pk.alive & (1u << r)) {
-                    float e0, e1;
-                    if (slab_diff(a0, a1, a2, a3, c0, c1, pk.invd[r], pk.tfar[r], e0)) { h0 = true; te0 = fminf(te0, e0); }
-                    if (slab_diff(b0, b1, b2, b3, c2, c3, pk.invd[r], pk.tfar[r], e1)) { h1 = true; te1 = fminf(te1, e1); }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        float e;
+                        if (slab_diff(dl[i][0], dl[i][1], dl[i][2], dl[i][3], dl[i][4], dl[i][5], pk.invd[r],
+                                      pk.tfar[r], e))
+                            m |= 1u << i;
+                    }
                 }
             }
-            if (h0 && h1) {
-                const bool first0 = te0 <= te1;
-                st.push(first0 ? ch.y : ch.x);
-                pk.node = first0 ? ch.x : ch.y;
-            } else if (h0) {
-                pk.node = ch.x;
-            } else if (h1) {
-                pk.node = ch.y;
-            } else {
-                pk.node = st.sp ? st.pop() : kDone;
-            }
+            if (q.ch.x == kNoChild) m &= ~1u;
+            if (q.ch.y == kNoChild) m &= ~2u;
+            if (q.ch.z == kNoChild) m &= ~4u;
+            if (q.ch.w == kNoChild) m &= ~8u;
+            if (m) pk.node = any_children(q, m, st);
+            else pk.node = st.sp ? st.pop() : kDone;
         } else {
             int first, count;
             leaf_decode(pk.node, first, count);
