@@ -111,9 +111,11 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     b->bvhDepth = bvh.maxDepth;
     b->triangles = triangleCount;
     hsr_status s;
-    if ((s = dalloc(&b->d_nodes, (size_t)bvh.nodeCount * 8)) != HSR_OK) return s;  // 8 float4 per node
+    if ((s = dalloc(&b->d_nodes, (size_t)bvh.nodeCount * 4)) != HSR_OK) return s;   // 4 float4 per node
+    if ((s = dalloc(&b->d_nodesF, (size_t)bvh.nodeCount * 8)) != HSR_OK) return s;  // 8 float4 per node
     if ((s = dalloc(&b->d_tris, (size_t)bvh.slotCount * 3)) != HSR_OK) return s;
     HSR_HIP(hipMemcpy(b->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(float), hipMemcpyHostToDevice));
+    HSR_HIP(hipMemcpy(b->d_nodesF, bvh.nodesF.data(), bvh.nodesF.size() * sizeof(float), hipMemcpyHostToDevice));
     HSR_HIP(hipMemcpy(b->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(float), hipMemcpyHostToDevice));
     b->slotIds.resize(bvh.slotCount);
     for (int32_t s2 = 0; s2 < bvh.slotCount; ++s2) std::memcpy(&b->slotIds[s2], &bvh.tris[12 * (size_t)s2 + 3], 4);
@@ -128,6 +130,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     dfree(b->d_nodes);
+    dfree(b->d_nodesF);
     dfree(b->d_tris);
     dfree(b->d_frame_sum);
     dfree(b->d_frame_comp);
@@ -195,7 +198,7 @@ hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64
 }
 
 int64_t backend_mesh_bytes(const hsr_backend *b) {
-    return (int64_t)b->nodeCount * 128 + (int64_t)b->slotCount * 48 + (b->d_vel ? (int64_t)b->slotCount * 16 : 0);
+    return (int64_t)b->nodeCount * (64 + 128) + (int64_t)b->slotCount * 48 + (b->d_vel ? (int64_t)b->slotCount * 16 : 0);
 }
 }  // namespace hsr
 
@@ -279,6 +282,10 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
         case HSR_OPT_PIPELINE:
             if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "pipeline must be 0 (fused) or 1 (wavefront)");
             b->optPipeline = (int32_t)value;
+            return HSR_OK;
+        case HSR_OPT_TRAV_WAVES:
+            if (value < 0 || value > 64) return fail(HSR_ERR_INVALID, "traversal waves per CU out of range");
+            b->optTravWaves = (int32_t)value;
             return HSR_OK;
         case HSR_OPT_PHASE_MODE:
             if (value < 0 || value > 2)
@@ -377,6 +384,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.sampleWeight = sampleWeight;
     a.isFinal = isFinalIteration;
     a.nodes = mesh->d_nodes;
+    a.nodesF = mesh->d_nodesF;
     a.tris = mesh->d_tris;
     if (guide) {
         a.gsChild = guide->d_sChild;
@@ -456,7 +464,9 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             a.recNee = b->d_recNee;
             a.recPaths = recPaths;
         }
-        const int gridTrav = b->cu * hsr::wavefront_trav_blocks_per_cu(b->optCounters);
+        const int travBlocks = hsr::wavefront_trav_blocks_per_cu(b->optCounters);
+        const int gridTrav =
+            b->cu * (b->optTravWaves > 0 ? std::max(1, std::min(travBlocks, (b->optTravWaves + 3) / 4)) : travBlocks);
         const int gridWork = b->cu * 4;
         // gather: up to two resident workgroups per CU (their LDS tiles permitting), and no
         // more workgroups than the batch has paths for (each one adds a tile to reduce)

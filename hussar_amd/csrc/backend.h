@@ -59,7 +59,8 @@ struct hsr_backend {
     int ldsPerBlock = 65536;  // device limit of a workgroup's LDS (160 KiB on gfx950)
     int ldsPerCU = 65536;
     // mesh
-    float4 *d_nodes = nullptr;
+    float4 *d_nodes = nullptr;   // quantised four-wide nodes
+    float4 *d_nodesF = nullptr;  // float-box four-wide nodes
     float4 *d_tris = nullptr;
     int32_t nodeCount = 0, slotCount = 0, bvhDepth = 0;
     int64_t triangles = 0;
@@ -92,6 +93,7 @@ struct hsr_backend {
     int64_t optPathsPerLaunch = int64_t(1) << 24;
     int32_t optPhaseMode = 0;
     int32_t optPipeline = 1;
+    int32_t optTravWaves = 0;  // resident waves per CU of the traversal kernels, 0 = all that fit
     int32_t maxGrid = 0;
     // wavefront buffers
     void *d_wf = nullptr;

@@ -269,6 +269,70 @@ void collapse4(const std::vector<float> &n2, std::vector<float> &n4, int32_t &de
     depth4 = maxd;
 }
 
+float dequant(float origin, uint32_t q, float scale) {
+    volatile float prod = (float)q * scale;  // exact: q < 256 and scale a power of two
+    return origin + prod;
+}
+
+bool quantize4(const std::vector<float> &n4, std::vector<float> &out) {
+    const size_t count = n4.size() / 32;
+    out.assign(16 * count, 0.0f);
+    for (size_t nd = 0; nd < count; ++nd) {
+        const float *q = &n4[32 * nd];
+        int32_t refs[4];
+        std::memcpy(refs, &q[24], 16);
+        uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+        float origin[3];
+        uint32_t biased[3];
+        for (int k = 0; k < 3; ++k) {
+            // rows: lo.x 0, hi.x 4, lo.y 8, hi.y 12, lo.z 16, hi.z 20
+            const float *lo = &q[8 * k], *hi = &q[8 * k + 4];
+            float o = std::numeric_limits<float>::infinity(), top = -std::numeric_limits<float>::infinity();
+            for (int i = 0; i < 4; ++i)
+                if (refs[i] != kNoChild) {
+                    o = std::min(o, lo[i]);
+                    top = std::max(top, hi[i]);
+                }
+            origin[k] = o;
+            // smallest power-of-two step with which 255 steps reach the top
+            int e = -120;
+            const double ext = (double)top - (double)o;
+            if (ext > 0.0) e = std::max(-120, (int)std::ceil(std::log2(ext / 255.0)));
+            for (;; ++e) {
+                if (e > 120) return false;
+                const float s = std::ldexp(1.0f, e);
+                if (dequant(o, 255, s) >= top) break;
+            }
+            const float s = std::ldexp(1.0f, e);
+            biased[k] = (uint32_t)(e + 127);
+            for (int i = 0; i < 4; ++i) {
+                uint32_t a = 0, b = 0;
+                if (refs[i] != kNoChild) {
+                    double fa = std::floor(((double)lo[i] - (double)o) / (double)s);
+                    double fb = std::ceil(((double)hi[i] - (double)o) / (double)s);
+                    a = (uint32_t)std::min(255.0, std::max(0.0, fa));
+                    b = (uint32_t)std::min(255.0, std::max(0.0, fb));
+                    while (a > 0 && dequant(o, a, s) > lo[i]) --a;
+                    while (b < 255 && dequant(o, b, s) < hi[i]) ++b;
+                    if (dequant(o, a, s) > lo[i] || dequant(o, b, s) < hi[i]) return false;
+                }
+                qlo[k] |= a << (8 * i);
+                qhi[k] |= b << (8 * i);
+            }
+        }
+        float *d = &out[16 * nd];
+        d[0] = origin[0];
+        d[1] = origin[1];
+        d[2] = origin[2];
+        const uint32_t ex = biased[0] | (biased[1] << 8) | (biased[2] << 16);
+        std::memcpy(&d[3], &ex, 4);
+        const uint32_t w[8] = {qlo[0], qhi[0], qlo[1], qhi[1], qlo[2], qhi[2], 0u, 0u};
+        std::memcpy(&d[4], w, 32);
+        std::memcpy(&d[12], refs, 16);
+    }
+    return true;
+}
+
 bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH &out, int threads) {
     (void)threads;
     if (nt < 0 || nt >= (1 << 27)) return false;
@@ -325,8 +389,9 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
             B.fill(nodes, 0, 0, mid, (int32_t)nt, r0, r1, 0);
         }
     }
-    collapse4(nodes, out.nodes, out.depth4);
-    out.nodeCount = (int32_t)(out.nodes.size() / 32);
+    collapse4(nodes, out.nodesF, out.depth4);
+    if (!quantize4(out.nodesF, out.nodes)) return false;
+    out.nodeCount = (int32_t)(out.nodes.size() / 16);
     out.slotCount = slotCount;
     out.maxDepth = B.maxDepth.load();
     out.maxLeaf = B.maxLeaf.load();

@@ -35,8 +35,10 @@ struct TraceArgs {
     int64_t offset, count, chunk;
     float sampleWeight;
     int32_t isFinal;
-    // mesh: four-wide BVH nodes (8 float4 each) and triangles (3 float4 each)
+    // mesh: four-wide BVH nodes, quantised (4 float4 each) and with float boxes (8 float4
+    // each, same node indices), and triangles (3 float4 each)
     const float4 *nodes;
+    const float4 *nodesF;
     const float4 *tris;
     // per-workgroup partial frames [gridDim.x][N*C] (complex double)
     double2 *partials;

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                 bool vis = true;
                 if (!a.poDiffraction) {
                     if (kCount) ++crs;
-                    vis = !occluded<kCount>(a.nodes, a.tris, ns.o, ns.d, ns.tfar, st, cn, ct);
+                    vis = !occluded<kCount>(a.nodesF, a.tris, ns.o, ns.d, ns.tfar, st, cn, ct);
                 }
                 int8_t flag = 1;
                 if (vis) {

@@ -49,9 +49,12 @@ struct Stack {
     }
 };
 
-// Four-wide BVH node (128 B = 8 float4): lo.x[4] | hi.x[4] | lo.y[4] | hi.y[4] | lo.z[4] |
-// hi.z[4] | child[4] | unused. A child >= 0 is an inner node; kNoChild an unused slot; any
-// other negative value a leaf: bit 31 set, bits 27-30 count-1, bits 0-26 first triangle slot.
+// Four-wide BVH node (64 B = 4 float4) with 8-bit child boxes: origin.xyz + biased exponents
+// | qlo.x, qhi.x, qlo.y, qhi.y | qlo.z, qhi.z, -, - | child[4]; byte i of a q word belongs to
+// child i and decodes to origin + q * 2^(e-127) (product exact, sum rounded; the builder
+// chose q so that the decoded box contains the child's box). A child >= 0 is an inner node;
+// kNoChild an unused slot; any other negative value a leaf: bit 31 set, bits 27-30 count-1,
+// bits 0-26 first triangle slot.
 constexpr int32_t kNoChild = 0x7fffffff;
 
 __device__ inline void leaf_decode(int32_t c, int &first, int &count) {
@@ -81,7 +84,15 @@ struct Node4 {
     int4 ch;
 };
 
-__device__ inline Node4 load_node4(const float4 *__restrict__ nodes, int32_t n) {
+__device__ inline float4 dequant4(float o, uint32_t w, float s) {
+    return make_float4(o + (float)(w & 0xFFu) * s, o + (float)((w >> 8) & 0xFFu) * s,
+                       o + (float)((w >> 16) & 0xFFu) * s, o + (float)(w >> 24) * s);
+}
+
+// The float layout of the same node (128 B = 8 float4: lo.x[4] | hi.x[4] | lo.y[4] | hi.y[4] |
+// lo.z[4] | hi.z[4] | child[4] | -), used by the any-hit traversals: their packets test every
+// child against several rays, where exact boxes save more visits than the smaller fetch.
+__device__ inline Node4 load_node4f(const float4 *__restrict__ nodes, int32_t n) {
     const float4 *p = nodes + 8 * (int64_t)n;
     Node4 q;
     q.lx = p[0];
@@ -91,6 +102,26 @@ __device__ inline Node4 load_node4(const float4 *__restrict__ nodes, int32_t n) 
     q.lz = p[4];
     q.hz = p[5];
     q.ch = __builtin_bit_cast(int4, p[6]);
+    return q;
+}
+
+__device__ inline Node4 load_node4(const float4 *__restrict__ nodes, int32_t n) {
+    const float4 *p = nodes + 4 * (int64_t)n;
+    const float4 h = p[0];
+    const uint4 a = __builtin_bit_cast(uint4, p[1]);
+    const uint4 b = __builtin_bit_cast(uint4, p[2]);
+    const uint32_t ex = __float_as_uint(h.w);
+    const float sx = __uint_as_float((ex & 0xFFu) << 23);
+    const float sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
+    Node4 q;
+    q.lx = dequant4(h.x, a.x, sx);
+    q.hx = dequant4(h.x, a.y, sx);
+    q.ly = dequant4(h.y, a.z, sy);
+    q.hy = dequant4(h.y, a.w, sy);
+    q.lz = dequant4(h.z, b.x, sz);
+    q.hz = dequant4(h.z, b.y, sz);
+    q.ch = __builtin_bit_cast(int4, p[3]);
     return q;
 }
 
@@ -210,7 +241,7 @@ __device__ inline int32_t closest_hit(const float4 *__restrict__ nodes, const fl
 }
 
 template <bool kCount>
-__device__ inline bool occluded(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o, f3 d,
+__device__ inline bool occluded(const float4 *__restrict__ nodesF, const float4 *__restrict__ tris, f3 o, f3 d,
                                 float tfar, Stack &st, uint32_t &cn, uint32_t &ct) {
     SlabRay r{o, {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}, kEps};
     st.sp = 0;
@@ -218,7 +249,7 @@ __device__ inline bool occluded(const float4 *__restrict__ nodes, const float4 *
     while (true) {
         if (node >= 0) {
             if (kCount) ++cn;
-            const Node4 q = load_node4(nodes, node);
+            const Node4 q = load_node4f(nodesF, node);
             float te[4];
             const uint32_t m = slab4(q, r, tfar, te);
             if (m) {
@@ -565,7 +596,7 @@ __device__ inline void trav_run(const float4 *__restrict__ nodes, const float4 *
     for (int s = 0; s < steps && t.node != kDone; ++s) {
         if (t.node >= 0) {
             if (kCount) ++cn;
-            const Node4 q = load_node4(nodes, t.node);
+            const Node4 q = kAny ? load_node4f(nodes, t.node) : load_node4(nodes, t.node);
             float te[4];
             const uint32_t m = slab4(q, r, t.tfar, te);
             if (m) t.node = kAny ? any_children(q, m, st) : order_children(q, m, te, st);

@@ -28,8 +28,21 @@ enum : int { I_DEPTH = 0, I_FLAGS, I_N };
 enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
 
 constexpr int kWfBlock = 256;
-constexpr int kTravSteps = 16;   // node visits between refills of idle lanes
-constexpr int kRefillMin = 16;   // idle lanes that trigger a refill
+#ifndef HSR_TRAV_STEPS
+#define HSR_TRAV_STEPS 16
+#endif
+#ifndef HSR_REFILL_MIN
+#define HSR_REFILL_MIN 16
+#endif
+// minimum resident waves per SIMD asked of the register allocator for the shading stages
+#ifndef HSR_HIT_WAVES
+#define HSR_HIT_WAVES 4
+#endif
+#ifndef HSR_NEE_WAVES
+#define HSR_NEE_WAVES 4
+#endif
+constexpr int kTravSteps = HSR_TRAV_STEPS;  // node visits between refills of idle lanes
+constexpr int kRefillMin = HSR_REFILL_MIN;  // idle lanes that trigger a refill
 constexpr int kPacket = 4;       // shadow rays (receivers of one path) traced together per lane
 
 __device__ inline void load_state(const WfArgs &w, int64_t i, PathState &p) {
@@ -255,7 +268,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int
             continue;
         }
         if (has) {
-            trav_run<kAny, kCount>(a.nodes, a.tris, t, st, kTravSteps, cn, ct);
+            trav_run<kAny, kCount>(kAny ? a.nodesF : a.nodes, a.tris, t, st, kTravSteps, cn, ct);
             if (t.node == kDone) {
                 if (kAny) w.s_vis[item] = t.best_id >= 0 ? 1 : 2;
                 else w.hit[item] = make_int2(t.best_slot, __float_as_int(t.tfar));
@@ -300,7 +313,7 @@ __device__ inline void packet_run(const float4 *__restrict__ nodes, const float4
     for (int s = 0; s < steps && pk.node != kDone; ++s) {
         if (pk.node >= 0) {
             if (kCount) ++cn;
-            const Node4 q = load_node4(nodes, pk.node);
+            const Node4 q = load_node4f(nodes, pk.node);
             // the children's faces relative to the packet's shared origin, then per ray a slab
             // test with its own direction: a child is visited when any live ray hits it
             float dl[4][6];
@@ -419,7 +432,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
             continue;
         }
         if (has) {
-            packet_run<kCount>(a.nodes, a.tris, pk, st, kTravSteps, cn, ct);
+            packet_run<kCount>(a.nodesF, a.tris, pk, st, kTravSteps, cn, ct);
             if (pk.node == kDone) {
                 for (int r = 0; r < rays; ++r) w.s_vis[slot0 + r] = (pk.alive & (1u << r)) ? 2 : 1;
                 has = false;
@@ -434,7 +447,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
 }
 
 // ------------------------------------------------------------------------ hit
-__global__ __launch_bounds__(kWfBlock) void k_wf_hit(TraceArgs a, WfArgs w, int depth) {
+__global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_hit(TraceArgs a, WfArgs w, int depth) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
@@ -469,7 +482,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_hit(TraceArgs a, WfArgs w, int 
 
 // ------------------------------------------------------------------------ NEE set-up
 // Items j < n_gat * nrx: path q_gat[j / nrx], receiver j % nrx.
-__global__ __launch_bounds__(kWfBlock) void k_wf_nee(TraceArgs a, WfArgs w, int depth) {
+__global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee(TraceArgs a, WfArgs w, int depth) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;

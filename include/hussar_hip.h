@@ -272,6 +272,10 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
 #define HSR_OPT_PATHS_PER_LAUNCH 3
 #define HSR_OPT_PIPELINE 4
 #define HSR_OPT_PHASE_MODE 5
+/*  HSR_OPT_TRAV_WAVES    wavefront pipeline: resident waves per CU given to each traversal
+ *                        kernel (0 = as many as fit, the default); fewer waves trace more
+ *                        rays each, which shortens the tail of a launch. */
+#define HSR_OPT_TRAV_WAVES 6
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]
