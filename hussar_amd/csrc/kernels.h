@@ -76,8 +76,8 @@ struct WfCounts {
 };
 
 // Structure-of-arrays path state and queues for a batch of up to P paths.
-//  sd: 15 double fields x P  (o.xyz, d.xyz, n_in.xyz, d_in.xyz, time, r, cosTheta)
-//  sf: 18 float fields x P   (H 6, H_in 6, guidingWeight 2, x0, x1, pdf, freq)
+//  sd: 9 double fields x P   (o.xyz, d.xyz, time, r, path-length rate)
+//  sf: 15 float fields x P   (H 6, guidingWeight 2, x0, x1, pdf, freq, vertex velocity 3)
 //  si: 2 int fields x P      (depth, flags: bit 0 = the path ends after its pending NEE)
 //  shadow slots [P * C]: ray (o.xyz, tfar | d.xyz), NEE payload (tn, v, plane-filter guiding
 //  term), status bits (1 set-up, 2 needs a visibility query, 4 add v to the guiding weight,

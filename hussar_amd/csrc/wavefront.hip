@@ -1,18 +1,18 @@
 // Wavefront pipeline of the radar integrator (PathTracer::sample, path.h:236-451, split into
 // stages over a batch of paths whose state lives in HBM):
 //
-//   k_wf_gen      Halton sample, guiding warp, Tx sample; NEE set-up at depth 0 when
-//                 onlyIndirect is off; the first extension queue.
+//   k_wf_gen      Halton sample, guiding warp, Tx sample; the first extension queue.
+//   k_wf_nee0     when onlyIndirect is off: one thread per (path, receiver), the direct
+//                 Tx -> Rx NEE set-up of depth 0
 //   per depth k:
-//   k_wf_shadow   any-hit BVH traversal of the depth-k shadow rays (persistent, lane refill)
+//   k_wf_shadow   any-hit BVH traversal of the depth-k shadow rays, a packet of up to four
+//                 receivers of one path per lane (persistent, lane refill)
 //   k_wf_gather   per path: guiding weight of the visible NEE terms, splat of the visible
 //                 events into the workgroup's LDS frame tile, guiding record of paths that end
-//   k_wf_extend   closest-hit traversal of the extension rays (persistent, lane refill)
-//   k_wf_hit      hit update and reflection, the next extension queue and the list of paths
-//                 with an NEE at depth k+1; guiding record of paths that miss or graze
-//   k_wf_nee      one thread per (path, receiver): NEE set-up of depth k+1 into the shadow
-//                 slot; the shadow queue stays path-major, so neighbouring lanes trace the
-//                 nearly identical rays towards receivers a few millimetres apart
+//   k_wf_trav     closest-hit traversal of the extension rays (persistent, lane refill)
+//   k_wf_shade    hit update and reflection, the next extension queue and gather list, the
+//                 guiding record of paths that miss or graze, and the NEE set-up of depth
+//                 k+1 towards every receiver with its shadow packets
 //
 // Queues are built with one atomic per wave and their sizes stay on the device, so a batch
 // is enqueued without host synchronisation. The traversal kernels hold only the ray and
@@ -22,8 +22,10 @@
 
 namespace hsr {
 
-enum : int { D_OX = 0, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_NX, D_NY, D_NZ, D_IX, D_IY, D_IZ, D_TIME, D_R, D_COS, D_RATE, D_N };
-enum : int { F_H = 0, F_HIN = 6, F_GWR = 12, F_GWI, F_X0, F_X1, F_PDF, F_FREQ, F_VX, F_VY, F_VZ, F_N };
+// Path state rows. The vertex of a bounce (normal, incoming direction and field, cosθ) is
+// produced and consumed inside k_wf_shade and never stored.
+enum : int { D_OX = 0, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_TIME, D_R, D_RATE, D_N };
+enum : int { F_H = 0, F_GWR = 6, F_GWI, F_X0, F_X1, F_PDF, F_FREQ, F_VX, F_VY, F_VZ, F_N };
 enum : int { I_DEPTH = 0, I_FLAGS, I_N };
 enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
 
@@ -36,7 +38,7 @@ constexpr int kWfBlock = 256;
 #endif
 // minimum resident waves per SIMD asked of the register allocator for the shading stages
 #ifndef HSR_HIT_WAVES
-#define HSR_HIT_WAVES 4
+#define HSR_HIT_WAVES 1
 #endif
 #ifndef HSR_NEE_WAVES
 #define HSR_NEE_WAVES 4
@@ -45,49 +47,17 @@ constexpr int kTravSteps = HSR_TRAV_STEPS;  // node visits between refills of id
 constexpr int kRefillMin = HSR_REFILL_MIN;  // idle lanes that trigger a refill
 constexpr int kPacket = 4;       // shadow rays (receivers of one path) traced together per lane
 
-__device__ inline void load_state(const WfArgs &w, int64_t i, PathState &p) {
-    const double *d = w.sd;
-    const float *f = w.sf;
-    const int64_t P = w.P;
-    p.o = {d[D_OX * P + i], d[D_OY * P + i], d[D_OZ * P + i]};
-    p.d = {d[D_DX * P + i], d[D_DY * P + i], d[D_DZ * P + i]};
-    p.n_in = {d[D_NX * P + i], d[D_NY * P + i], d[D_NZ * P + i]};
-    p.d_in = {d[D_IX * P + i], d[D_IY * P + i], d[D_IZ * P + i]};
-    p.time = d[D_TIME * P + i];
-    p.r = d[D_R * P + i];
-    p.cosTheta = d[D_COS * P + i];
-    p.H = {{f[(F_H + 0) * P + i], f[(F_H + 1) * P + i]},
-           {f[(F_H + 2) * P + i], f[(F_H + 3) * P + i]},
-           {f[(F_H + 4) * P + i], f[(F_H + 5) * P + i]}};
-    p.H_in = {{f[(F_HIN + 0) * P + i], f[(F_HIN + 1) * P + i]},
-              {f[(F_HIN + 2) * P + i], f[(F_HIN + 3) * P + i]},
-              {f[(F_HIN + 4) * P + i], f[(F_HIN + 5) * P + i]}};
-    p.gw = {f[F_GWR * P + i], f[F_GWI * P + i]};
-    p.x0 = f[F_X0 * P + i];
-    p.x1 = f[F_X1 * P + i];
-    p.pdf = f[F_PDF * P + i];
-    p.freq = f[F_FREQ * P + i];
-    p.depth = w.si[I_DEPTH * P + i];
-    p.local = w.base + i;
-}
-
 __device__ inline void store_state(const WfArgs &w, int64_t i, const PathState &p) {
     double *d = w.sd;
     float *f = w.sf;
     const int64_t P = w.P;
     d[D_OX * P + i] = p.o.x; d[D_OY * P + i] = p.o.y; d[D_OZ * P + i] = p.o.z;
     d[D_DX * P + i] = p.d.x; d[D_DY * P + i] = p.d.y; d[D_DZ * P + i] = p.d.z;
-    d[D_NX * P + i] = p.n_in.x; d[D_NY * P + i] = p.n_in.y; d[D_NZ * P + i] = p.n_in.z;
-    d[D_IX * P + i] = p.d_in.x; d[D_IY * P + i] = p.d_in.y; d[D_IZ * P + i] = p.d_in.z;
     d[D_TIME * P + i] = p.time;
     d[D_R * P + i] = p.r;
-    d[D_COS * P + i] = p.cosTheta;
     f[(F_H + 0) * P + i] = p.H.x.re; f[(F_H + 1) * P + i] = p.H.x.im;
     f[(F_H + 2) * P + i] = p.H.y.re; f[(F_H + 3) * P + i] = p.H.y.im;
     f[(F_H + 4) * P + i] = p.H.z.re; f[(F_H + 5) * P + i] = p.H.z.im;
-    f[(F_HIN + 0) * P + i] = p.H_in.x.re; f[(F_HIN + 1) * P + i] = p.H_in.x.im;
-    f[(F_HIN + 2) * P + i] = p.H_in.y.re; f[(F_HIN + 3) * P + i] = p.H_in.y.im;
-    f[(F_HIN + 4) * P + i] = p.H_in.z.re; f[(F_HIN + 5) * P + i] = p.H_in.z.im;
     f[F_GWR * P + i] = p.gw.re;
     f[F_GWI * P + i] = p.gw.im;
     f[F_X0 * P + i] = p.x0;
@@ -137,36 +107,18 @@ __device__ inline void load_record(const WfArgs &w, int64_t i, PathState &p) {
     p.x1 = f[F_X1 * P + i];
     p.pdf = f[F_PDF * P + i];
 }
-__device__ inline void load_vertex(const WfArgs &w, int64_t i, PathState &p) {
-    const double *d = w.sd;
-    const float *f = w.sf;
-    const int64_t P = w.P;
-    p.n_in = {d[D_NX * P + i], d[D_NY * P + i], d[D_NZ * P + i]};
-    p.d_in = {d[D_IX * P + i], d[D_IY * P + i], d[D_IZ * P + i]};
-    p.cosTheta = d[D_COS * P + i];
-    p.H_in = {{f[(F_HIN + 0) * P + i], f[(F_HIN + 1) * P + i]},
-              {f[(F_HIN + 2) * P + i], f[(F_HIN + 3) * P + i]},
-              {f[(F_HIN + 4) * P + i], f[(F_HIN + 5) * P + i]}};
-    p.freq = f[F_FREQ * P + i];
-}
-// after a hit: everything apply_hit changed
+// after a hit: the chain, the field and the depth for the next bounce
 __device__ inline void store_hit(const WfArgs &w, int64_t i, const PathState &p) {
     double *d = w.sd;
     float *f = w.sf;
     const int64_t P = w.P;
     d[D_OX * P + i] = p.o.x; d[D_OY * P + i] = p.o.y; d[D_OZ * P + i] = p.o.z;
     d[D_DX * P + i] = p.d.x; d[D_DY * P + i] = p.d.y; d[D_DZ * P + i] = p.d.z;
-    d[D_NX * P + i] = p.n_in.x; d[D_NY * P + i] = p.n_in.y; d[D_NZ * P + i] = p.n_in.z;
-    d[D_IX * P + i] = p.d_in.x; d[D_IY * P + i] = p.d_in.y; d[D_IZ * P + i] = p.d_in.z;
     d[D_TIME * P + i] = p.time;
     d[D_R * P + i] = p.r;
-    d[D_COS * P + i] = p.cosTheta;
     f[(F_H + 0) * P + i] = p.H.x.re; f[(F_H + 1) * P + i] = p.H.x.im;
     f[(F_H + 2) * P + i] = p.H.y.re; f[(F_H + 3) * P + i] = p.H.y.im;
     f[(F_H + 4) * P + i] = p.H.z.re; f[(F_H + 5) * P + i] = p.H.z.im;
-    f[(F_HIN + 0) * P + i] = p.H_in.x.re; f[(F_HIN + 1) * P + i] = p.H_in.x.im;
-    f[(F_HIN + 2) * P + i] = p.H_in.y.re; f[(F_HIN + 3) * P + i] = p.H_in.y.im;
-    f[(F_HIN + 4) * P + i] = p.H_in.z.re; f[(F_HIN + 5) * P + i] = p.H_in.z.im;
     w.si[I_DEPTH * P + i] = p.depth;
 }
 
@@ -447,7 +399,29 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
 }
 
 // ------------------------------------------------------------------------ hit
-__global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_hit(TraceArgs a, WfArgs w, int depth) {
+// One shadow slot of the NEE set-up ns in slot j; returns whether it needs a visibility query.
+__device__ inline bool write_slot(const TraceArgs &a, const WfArgs &w, int64_t j, const NeeSetup &ns) {
+    if (a.motion) w.s_dv[j] = ns.dv;
+    w.s_o[j] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.tfar);
+    w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, 0.0f);
+    w.s_tn[j] = ns.tn;
+    w.s_v[j] = make_float2(ns.v.re, ns.v.im);
+    w.s_g1[j] = make_float2(ns.gw1.re, ns.gw1.im);
+    const bool test = !a.poDiffraction;
+    const int st = S_SETUP | (test ? S_TEST : 0) | (ns.gwAdd ? S_GW : 0) | (ns.splat ? S_SPLAT : 0) |
+                   ((ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) ? S_GW1 : 0);
+    w.s_st[j] = (int8_t)st;
+    w.s_vis[j] = 0;
+    return test;
+}
+
+// ------------------------------------------------------------------------ shade
+// Per path of the depth-k extension queue: the hit update and reflection, the next extension
+// queue and the gather list of depth k+1, the guiding record of paths that miss or graze, and
+// the NEE set-up of the new vertex towards every receiver (its shadow slots and one shadow
+// packet per four receivers). The vertex (normal, incoming direction and field, cosθ) stays
+// in registers from the hit update to the NEE set-up.
+__global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs a, WfArgs w, int depth) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
@@ -455,6 +429,8 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_hit(TraceArgs a,
     const int32_t *q = (depth & 1) ? w.q_ext1 : w.q_ext0;
     int32_t *qnext = (depth & 1) ? w.q_ext0 : w.q_ext1;
     WfCounts *next = &w.cnt[depth + 1];
+    const int nrx = min(a.rxCount, a.C);
+    const int groups = (nrx + kPacket - 1) / kPacket;
     for (int64_t b = wave * 64; b < n; b += waves * 64) {
         const int64_t k = b + lane;
         const bool valid = k < n;
@@ -477,17 +453,29 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_hit(TraceArgs a,
         if (alive) store_hit(w, i, p);
         if (alive && a.motion) store_motion(w, i, p);
         finish_record(a, valid && !alive, p, lane);
+        // NEE set-up at the new vertex (path.h:290-397 of the next bounce)
+        if (alive) p.freq = w.sf[F_FREQ * w.P + i];
+        for (int g = 0; g < groups; ++g) {
+            bool lead = false;
+            if (alive) {
+                const int c1 = min(nrx, (g + 1) * kPacket);
+                for (int c = g * kPacket; c < c1; ++c) lead = write_slot(a, w, i * a.C + c, nee_setup(a, p, c));
+            }
+            const int64_t qs = wave_append(&next->n_shd, lead, lane);
+            if (lead) w.q_shd[qs] = (int32_t)(i * 4 + g);
+        }
     }
 }
 
-// ------------------------------------------------------------------------ NEE set-up
-// Items j < n_gat * nrx: path q_gat[j / nrx], receiver j % nrx.
-__global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee(TraceArgs a, WfArgs w, int depth) {
+// ------------------------------------------------------------------------ NEE set-up, depth 0
+// The direct paths Tx -> Rx (onlyIndirect off): items j < n_gat * nrx of the depth-0 gather
+// list, path q_gat[j / nrx], receiver j % nrx.
+__global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a, WfArgs w) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
     const int nrx = min(a.rxCount, a.C);
-    WfCounts *cnt = &w.cnt[depth];
+    WfCounts *cnt = &w.cnt[0];
     const int64_t n = (int64_t)cnt->n_gat * nrx;
     for (int64_t b = wave * 64; b < n; b += waves * 64) {
         const int64_t k = b + lane;
@@ -499,22 +487,10 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee(TraceArgs a,
             const int ch = (int)(k % nrx);
             PathState p;
             load_chain(w, i, p);
-            if (p.depth > 0) load_vertex(w, i, p);
-            else p.freq = w.sf[F_FREQ * w.P + i];
+            p.freq = w.sf[F_FREQ * w.P + i];
             if (a.motion) load_motion(w, i, p);
-            const NeeSetup ns = nee_setup(a, p, ch);
             j = i * a.C + ch;
-            if (a.motion) w.s_dv[j] = ns.dv;
-            w.s_o[j] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.tfar);
-            w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, 0.0f);
-            w.s_tn[j] = ns.tn;
-            w.s_v[j] = make_float2(ns.v.re, ns.v.im);
-            w.s_g1[j] = make_float2(ns.gw1.re, ns.gw1.im);
-            test = !a.poDiffraction;
-            const int st = S_SETUP | (test ? S_TEST : 0) | (ns.gwAdd ? S_GW : 0) | (ns.splat ? S_SPLAT : 0) |
-                           ((ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) ? S_GW1 : 0);
-            w.s_st[j] = (int8_t)st;
-            w.s_vis[j] = 0;
+            test = write_slot(a, w, j, nee_setup(a, p, ch));
         }
         // one shadow packet per four receivers of a path: entry = path * 4 + group
         const bool lead = test && (k % nrx) % kPacket == 0;
@@ -695,7 +671,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
         if (e != hipSuccess) return e;
     }
-    if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee, dim3(gNee), dim3(kWfBlock), 0, s, a, w, 0);
+    if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee0, dim3(gNee), dim3(kWfBlock), 0, s, a, w);
     for (int k = 0; k <= maxDepth; ++k) {
         if (k >= 1 || !a.onlyIndirect) {
             if (!a.poDiffraction) {
@@ -717,8 +693,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
                 hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
             (void)hipEventRecord(evClosest[2 * *nClosest + 1], s);
             ++*nClosest;
-            hipLaunchKernelGGL(k_wf_hit, dim3(gHit), dim3(kWfBlock), 0, s, a, w, k);
-            hipLaunchKernelGGL(k_wf_nee, dim3(gNee), dim3(kWfBlock), 0, s, a, w, k + 1);
+            hipLaunchKernelGGL(k_wf_shade, dim3(gHit), dim3(kWfBlock), 0, s, a, w, k);
         }
     }
     return hipGetLastError();
