@@ -78,7 +78,8 @@ struct WfCounts {
 // Structure-of-arrays path state and queues for a batch of up to P paths.
 //  sd: 9 double fields x P   (o.xyz, d.xyz, time, r, path-length rate)
 //  sf: 15 float fields x P   (H 6, guidingWeight 2, x0, x1, pdf, freq, vertex velocity 3)
-//  si: 2 int fields x P      (depth, flags: bit 0 = the path ends after its pending NEE)
+//  si: 3 int fields x P      (depth, flags: bit 0 = the path ends after its pending NEE,
+//                            the path's index within the batch)
 //  shadow slots [P * C]: ray (o.xyz, tfar | d.xyz), NEE payload (tn, v, plane-filter guiding
 //  term), status bits (1 set-up, 2 needs a visibility query, 4 add v to the guiding weight,
 //  8 splat, 16 add the plane-filter term) and the visibility (1 occluded, 2 visible).
@@ -98,7 +99,13 @@ struct WfArgs {
     int8_t *s_st, *s_vis;
     int32_t *q_ext0, *q_ext1, *q_shd, *q_gat;
     WfCounts *cnt;  // [maxDepth + 2]
+    // coherence order of the batch: slot s holds the path of batch index inv[s]
+    uint32_t *key;  // [P] cell of each batch index
+    uint32_t *bins; // [kSortBins] cell counts, then the next free slot of each cell
+    int32_t *inv;   // [P]
 };
+constexpr int kSortBins = 4096;   // 64 x 64 cells of the primary sample square (Morton order)
+constexpr int kSortBlocks = 256;  // workgroups of the counting sort
 
 // Bytes of device memory the wavefront buffers need for P paths and C channels.
 size_t wavefront_bytes(int64_t P, int C, int maxDepth);

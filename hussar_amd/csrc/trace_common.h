@@ -393,12 +393,19 @@ struct PathState {
     f3 vprev;       // motion: velocity of the current vertex (the Tx at depth 0)
 };
 
-__device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p) {
-    const float u_f = radical_inverse_2(index);
+// The primary sample of a Halton index before the guiding warp: dimensions 1 and 2 in the order
+// get2D evaluates them (halton.h:24-27).
+__device__ inline void primary_sample(const TraceArgs &a, uint64_t index, float &x0, float &x1) {
     const float u_a = radical_inverse_b<3>(index);
     const float u_b = radical_inverse_b<5>(index);
-    float x0 = a.primaryOrder == 0 ? u_b : u_a;
-    float x1 = a.primaryOrder == 0 ? u_a : u_b;
+    x0 = a.primaryOrder == 0 ? u_b : u_a;
+    x1 = a.primaryOrder == 0 ? u_a : u_b;
+}
+
+__device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p) {
+    const float u_f = radical_inverse_2(index);
+    float x0, x1;
+    primary_sample(a, index, x0, x1);
     float pdf = 1.0f;
     if (a.doGuiding && a.guideOn) pdf = guide_sample(a, x0, x1);
     const d3 dl = uniform_sample_sphere((double)x0, (double)x1);

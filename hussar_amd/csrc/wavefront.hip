@@ -26,7 +26,7 @@ namespace hsr {
 // produced and consumed inside k_wf_shade and never stored.
 enum : int { D_OX = 0, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_TIME, D_R, D_RATE, D_N };
 enum : int { F_H = 0, F_GWR = 6, F_GWI, F_X0, F_X1, F_PDF, F_FREQ, F_VX, F_VY, F_VZ, F_N };
-enum : int { I_DEPTH = 0, I_FLAGS, I_N };
+enum : int { I_DEPTH = 0, I_FLAGS, I_LOCAL, I_N };
 enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
 
 constexpr int kWfBlock = 256;
@@ -65,6 +65,7 @@ __device__ inline void store_state(const WfArgs &w, int64_t i, const PathState &
     f[F_PDF * P + i] = p.pdf;
     f[F_FREQ * P + i] = p.freq;
     w.si[I_DEPTH * P + i] = p.depth;
+    w.si[I_LOCAL * P + i] = (int32_t)(p.local - w.base);
 }
 
 // motion: the path-length rate and the current vertex's velocity
@@ -90,7 +91,7 @@ __device__ inline void load_chain(const WfArgs &w, int64_t i, PathState &p) {
     p.time = d[D_TIME * P + i];
     p.r = d[D_R * P + i];
     p.depth = w.si[I_DEPTH * P + i];
-    p.local = w.base + i;
+    p.local = w.base + w.si[I_LOCAL * P + i];
 }
 __device__ inline void load_field(const WfArgs &w, int64_t i, PathState &p) {
     const float *f = w.sf;
@@ -160,15 +161,100 @@ __device__ inline void emit_extend(const WfArgs &w, bool alive, bool ends, int64
     if (alive) w.si[I_FLAGS * w.P + i] = ends ? 1 : 0;
 }
 
+// ------------------------------------------------------------------------ coherence order
+// The batch's paths are laid out in slot order sorted by the cell of their primary sample
+// (before the guiding warp, which maps sample-space neighbourhoods to direction-space
+// neighbourhoods at the scale of the guiding tree): neighbouring lanes then trace nearly
+// parallel rays from the same antenna, and the queues of later bounces, appended in wave
+// order, keep much of that coherence. A counting sort over kSortBins Morton-ordered cells:
+// per-workgroup counts in LDS added to global counts, an exclusive scan of the counts, then a
+// scatter in which each workgroup reserves one range per cell with one atomic and ranks its
+// elements with LDS counters. The order within a cell depends on the atomics; each path's
+// result does not.
+__device__ inline uint32_t morton_cell(float x0, float x1) {
+    const uint32_t cx = min(63u, (uint32_t)(x0 * 64.0f)), cy = min(63u, (uint32_t)(x1 * 64.0f));
+    uint32_t k = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) k |= (((cx >> b) & 1u) << (2 * b)) | (((cy >> b) & 1u) << (2 * b + 1));
+    return k;
+}
+
+__device__ inline void sort_chunk(int64_t n, int64_t &b0, int64_t &b1) {
+    const int64_t per = (n + kSortBlocks - 1) / kSortBlocks;
+    b0 = min(n, (int64_t)blockIdx.x * per);
+    b1 = min(n, b0 + per);
+}
+
+__global__ __launch_bounds__(kWfBlock) void k_sort_count(TraceArgs a, WfArgs w) {
+    __shared__ uint32_t h[kSortBins];
+    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock) h[c] = 0;
+    __syncthreads();
+    int64_t b0, b1;
+    sort_chunk(w.n, b0, b1);
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kWfBlock) {
+        float x0, x1;
+        primary_sample(a, (uint64_t)(a.offset + w.base + i), x0, x1);
+        const uint32_t k = morton_cell(x0, x1);
+        w.key[i] = k;
+        atomicAdd(&h[k], 1u);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock)
+        if (h[c]) atomicAdd(&w.bins[c], h[c]);
+}
+
+// Exclusive scan of the kSortBins counts in place, by one workgroup of 1024 (4 cells each).
+__global__ __launch_bounds__(1024) void k_sort_scan(WfArgs w) {
+    __shared__ uint32_t part[1024];
+    constexpr int kPer = kSortBins / 1024;
+    uint32_t *v = w.bins + (size_t)threadIdx.x * kPer;
+    uint32_t s = 0;
+    for (int j = 0; j < kPer; ++j) s += v[j];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;  // exclusive prefix of this thread's cells
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t c = v[j];
+        v[j] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(kWfBlock) void k_sort_scatter(WfArgs w) {
+    __shared__ uint32_t h[kSortBins];
+    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock) h[c] = 0;
+    __syncthreads();
+    int64_t b0, b1;
+    sort_chunk(w.n, b0, b1);
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kWfBlock) atomicAdd(&h[w.key[i]], 1u);
+    __syncthreads();
+    // one range per cell for this workgroup's elements
+    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock)
+        if (h[c]) h[c] = atomicAdd(&w.bins[c], h[c]);
+    __syncthreads();
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kWfBlock) {
+        const uint32_t pos = atomicAdd(&h[w.key[i]], 1u);
+        w.inv[pos] = (int32_t)i;
+    }
+}
+
 // ------------------------------------------------------------------------ generation
+// Thread s generates the path of batch index inv[s] into slot s.
 __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * kWfBlock + threadIdx.x;
     const bool valid = i < w.n;
     PathState p;
     if (valid) {
-        p.local = w.base + i;
-        generate(a, (uint64_t)(a.offset + w.base + i), p);
+        const int64_t m = w.inv[i];
+        p.local = w.base + m;
+        generate(a, (uint64_t)(a.offset + w.base + m), p);
     }
     const bool nee = valid && !a.onlyIndirect;
     emit_gather(w, nee, i, &w.cnt[0], lane);
@@ -528,7 +614,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
             p.x1 = w.sf[F_X1 * P + i];
             p.pdf = w.sf[F_PDF * P + i];
             p.depth = w.si[I_DEPTH * P + i];
-            p.local = w.base + i;
+            p.local = w.base + w.si[I_LOCAL * P + i];
         }
         const bool rec = valid && a.recNee && p.local < a.recPaths && p.depth >= 1;
         for (int ch = 0; ch < nrx; ++ch) {
@@ -581,9 +667,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
     }
 }
 
-__global__ void k_wf_zero_counts(WfCounts *c, int n) {
+__global__ void k_wf_zero_counts(WfCounts *c, int n, uint32_t *bins) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) c[i] = WfCounts{0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < kSortBins) bins[i] = 0;
 }
 
 // ------------------------------------------------------------------------ host side
@@ -603,6 +690,7 @@ size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
     b += 2 * align256(sizeof(int8_t) * S);
     b += 3 * align256(sizeof(int32_t) * P) + align256(sizeof(int32_t) * S);
     b += align256(sizeof(WfCounts) * (maxDepth + 2));
+    b += 2 * align256(sizeof(uint32_t) * P) + align256(sizeof(uint32_t) * kSortBins);
     return b;
 }
 
@@ -634,6 +722,9 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.q_gat = (int32_t *)take(sizeof(int32_t) * P);
     w.q_shd = (int32_t *)take(sizeof(int32_t) * S);
     w.cnt = (WfCounts *)take(sizeof(WfCounts) * (maxDepth + 2));
+    w.key = (uint32_t *)take(sizeof(uint32_t) * P);
+    w.inv = (int32_t *)take(sizeof(int32_t) * P);
+    w.bins = (uint32_t *)take(sizeof(uint32_t) * kSortBins);
 }
 
 size_t gather_lds_bytes(int N, int C) { return (size_t)N * C * 2 * sizeof(double); }
@@ -663,7 +754,11 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket));
     const int gHit = fit(gridWork, n);
     const int gNee = fit(gridWork, n * nrx);
-    hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 63) / 64), dim3(64), 0, s, w.cnt, maxDepth + 2);
+    hipLaunchKernelGGL(k_wf_zero_counts, dim3((max(maxDepth + 2, kSortBins) + 255) / 256), dim3(256), 0, s, w.cnt,
+                       maxDepth + 2, w.bins);
+    hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(kWfBlock), 0, s, a, w);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(kWfBlock), 0, s, w);
     hipLaunchKernelGGL(k_wf_gen, dim3((unsigned)((w.n + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s, a, w);
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
     if (glds > 65536) {
