@@ -19,6 +19,7 @@ STATUS_NAMES = {0: "HSR_OK", 1: "HSR_ERR_INVALID", 2: "HSR_ERR_HIP", 3: "HSR_ERR
 OPT_COUNTERS, OPT_RECORD_HITS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE, OPT_PHASE_MODE = 1, 2, 3, 4, 5
 PHASE_FP64, PHASE_FP32, PHASE_DOUBLE_FLOAT = 0, 1, 2
 OPT_TRAV_WAVES = 6
+OPT_BATCH_PATHS = 7
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER

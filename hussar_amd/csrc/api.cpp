@@ -283,6 +283,11 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
             if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "pipeline must be 0 (fused) or 1 (wavefront)");
             b->optPipeline = (int32_t)value;
             return HSR_OK;
+        case HSR_OPT_BATCH_PATHS:
+            if (value < 1024 || value > (int64_t(1) << 24))
+                return fail(HSR_ERR_INVALID, "paths per batch must lie in [1024, 2^24]");
+            b->wfCap = value;
+            return HSR_OK;
         case HSR_OPT_TRAV_WAVES:
             if (value < 0 || value > 64) return fail(HSR_ERR_INVALID, "traversal waves per CU out of range");
             b->optTravWaves = (int32_t)value;

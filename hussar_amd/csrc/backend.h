@@ -98,7 +98,7 @@ struct hsr_backend {
     // wavefront buffers
     void *d_wf = nullptr;
     size_t wfBytes = 0;
-    int64_t wfCap = int64_t(1) << 22;  // paths per wavefront batch
+    int64_t wfCap = int64_t(1) << 24;  // paths per wavefront batch (HSR_OPT_BATCH_PATHS)
     // post-processing (grow-only)
     double *d_post = nullptr, *d_post2 = nullptr;
     int64_t postCap = 0;  // doubles

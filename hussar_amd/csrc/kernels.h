@@ -104,7 +104,11 @@ struct WfArgs {
     uint32_t *bins; // [kSortBins] cell counts, then the next free slot of each cell
     int32_t *inv;   // [P]
 };
-constexpr int kSortBins = 4096;   // 64 x 64 cells of the primary sample square (Morton order)
+#ifndef HSR_SORT_BITS
+#define HSR_SORT_BITS 7
+#endif
+constexpr int kSortBits = HSR_SORT_BITS;             // cells per axis: 2^kSortBits
+constexpr int kSortBins = 1 << (2 * kSortBits);      // cells of the primary sample square (Morton order)
 constexpr int kSortBlocks = 256;  // workgroups of the counting sort
 
 // Bytes of device memory the wavefront buffers need for P paths and C channels.

@@ -172,10 +172,11 @@ __device__ inline void emit_extend(const WfArgs &w, bool alive, bool ends, int64
 // elements with LDS counters. The order within a cell depends on the atomics; each path's
 // result does not.
 __device__ inline uint32_t morton_cell(float x0, float x1) {
-    const uint32_t cx = min(63u, (uint32_t)(x0 * 64.0f)), cy = min(63u, (uint32_t)(x1 * 64.0f));
+    constexpr uint32_t side = 1u << kSortBits;
+    const uint32_t cx = min(side - 1, (uint32_t)(x0 * (float)side)), cy = min(side - 1, (uint32_t)(x1 * (float)side));
     uint32_t k = 0;
 #pragma unroll
-    for (int b = 0; b < 6; ++b) k |= (((cx >> b) & 1u) << (2 * b)) | (((cy >> b) & 1u) << (2 * b + 1));
+    for (int b = 0; b < kSortBits; ++b) k |= (((cx >> b) & 1u) << (2 * b)) | (((cy >> b) & 1u) << (2 * b + 1));
     return k;
 }
 
