@@ -276,9 +276,10 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        kernel (0 = as many as fit, the default); fewer waves trace more
  *                        rays each, which shortens the tail of a launch. */
 #define HSR_OPT_TRAV_WAVES 6
-/*  HSR_OPT_BATCH_PATHS   wavefront pipeline: paths per batch (default 2^22, at most 2^24);
- *                        larger batches order their rays more coherently but hold more path
- *                        state in device memory (about 0.5 KB per path with 4 receivers). */
+/*  HSR_OPT_BATCH_PATHS   wavefront pipeline: paths per batch (default and maximum 2^24, at
+ *                        least 1024); larger batches order their rays more coherently but hold
+ *                        more path state in device memory (about 0.5 KB per path with 4
+ *                        receivers). */
 #define HSR_OPT_BATCH_PATHS 7
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
