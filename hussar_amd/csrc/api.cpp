@@ -10,7 +10,6 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -20,6 +19,12 @@
 #include "../../include/hussar_hip.h"
 #include "backend.h"
 #include "bvh.h"
+
+#ifndef HSR_GATHER_PER_CU
+#define HSR_GATHER_PER_CU 4
+#endif
+// gather workgroups per CU (each holds a frame tile in LDS and adds a partial frame)
+static constexpr int kGatherPerCU = HSR_GATHER_PER_CU;
 
 namespace hsr {
 
@@ -473,9 +478,11 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         const int gridTrav =
             b->cu * (b->optTravWaves > 0 ? std::max(1, std::min(travBlocks, (b->optTravWaves + 3) / 4)) : travBlocks);
         const int gridWork = b->cu * 4;
-        // gather: up to two resident workgroups per CU (their LDS tiles permitting), and no
-        // more workgroups than the batch has paths for (each one adds a tile to reduce)
-        const int tilesPerCU = cube ? 2 : (int)std::max<size_t>(1, std::min<size_t>(2, (size_t)b->ldsPerCU / tileBytes));
+        // gather: up to kGatherPerCU resident workgroups per CU (their LDS tiles permitting),
+        // and no more workgroups than the batch has paths for (each one adds a tile to reduce)
+        const int tilesPerCU =
+            cube ? kGatherPerCU
+                 : (int)std::max<size_t>(1, std::min<size_t>(kGatherPerCU, (size_t)b->ldsPerCU / tileBytes));
         const int gridGather = (int)std::max<int64_t>(
             1, std::min<int64_t>((int64_t)b->cu * tilesPerCU, (n + hsr::kTraceBlock - 1) / hsr::kTraceBlock));
         const size_t needP = (size_t)gridGather * MD;

@@ -127,7 +127,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     // ---- store the workgroup's frame tile
     __syncthreads();
     double2 *dst = a.partials + (size_t)blockIdx.x * NC;
-    for (int i = tid; i < NC; i += kTraceBlock) dst[i] = make_double2(tile[2 * i], tile[2 * i + 1]);
+    for (int i = tid; i < NC; i += kTraceBlock) {
+        const int t = tile_re(i / a.C, i % a.C, a.N);
+        dst[i] = make_double2(tile[t], tile[NC + t]);
+    }
     if (kCount) {
         atomicAdd(&a.counters[0], (unsigned long long)cn);
         atomicAdd(&a.counters[1], (unsigned long long)ct);

@@ -761,30 +761,61 @@ __device__ inline int wrap_bin(int v, int N) {
     return v < 0 ? v + N : v;
 }
 
-// Wave-cooperative splat of this lane's event (if any) into the workgroup's LDS tile.
+// The workgroup's LDS frame tile of one chirp holds channel-major rows of bins, real parts
+// then imaginary parts: re[c*N + b] at tile[c*N + b], im at tile[N*C + c*N + b]. The 33 taps of
+// one event then fall on consecutive doubles of one row, so the lanes of one LDS atomic touch
+// distinct, consecutive banks.
+__device__ inline int tile_re(int b, int c, int N) { return c * N + b; }
+
+// Splat of the wave's events (one per lane at most, all of frame channel ch) into the LDS tile.
+// Taps k = -16..15 go two events per pass, one per half-wave (lane & 31 = k + 16); the
+// k = +16 taps and the single taps of delta events are added lane-parallel, one per lane.
 __device__ inline void flush_events(bool has, const Event &e, int ch, double *tile, int N, int C, int lane) {
-    uint64_t m = __ballot(has);
+    double *re = tile + ch * N, *im = tile + N * C + ch * N;
+    const bool spread = has && !e.delta;
+    if (has) {
+        const int k = e.delta ? 0 : 16;
+        const int b = wrap_bin(e.center + k, N);
+        const float coef = e.delta ? 1.0f : 1.0f / (e.s - (float)k);
+        atomicAdd(&re[b], (double)(e.A.re * coef));
+        atomicAdd(&im[b], (double)(e.A.im * coef));
+    }
+    uint64_t m = __ballot(spread);
+    const bool hi = lane >= 32;
+    const int k = (lane & 31) - 16;
     while (m) {
-        const int src = __builtin_ctzll(m);
+        const int sa = __builtin_ctzll(m);
         m &= m - 1;
-        const int center = __builtin_amdgcn_readlane(e.center, src);
-        const float s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.s), src));
-        const float Are = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.A.re), src));
-        const float Aim = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.A.im), src));
-        const int delta = __builtin_amdgcn_readlane(e.delta, src);
-        if (delta) {
-            if (lane == 0) {
-                const int b = wrap_bin(center, N);
-                atomicAdd(&tile[2 * (b * C + ch) + 0], (double)Are);
-                atomicAdd(&tile[2 * (b * C + ch) + 1], (double)Aim);
-            }
-        } else if (lane < 33) {
-            const int k = lane - 16;
-            const int b = wrap_bin(center + k, N);
+        const int sb = m ? __builtin_ctzll(m) : sa;
+        const bool two = m != 0;
+        m &= m - 1;
+        const int ca = __builtin_amdgcn_readlane(e.center, sa), cb = __builtin_amdgcn_readlane(e.center, sb);
+        const int fa = __builtin_amdgcn_readlane(__float_as_int(e.s), sa);
+        const int fb = __builtin_amdgcn_readlane(__float_as_int(e.s), sb);
+        const int ra = __builtin_amdgcn_readlane(__float_as_int(e.A.re), sa);
+        const int rb = __builtin_amdgcn_readlane(__float_as_int(e.A.re), sb);
+        const int ia = __builtin_amdgcn_readlane(__float_as_int(e.A.im), sa);
+        const int ib = __builtin_amdgcn_readlane(__float_as_int(e.A.im), sb);
+        if (!hi || two) {
+            const int center = hi ? cb : ca;
+            const float s = __int_as_float(hi ? fb : fa);
             const float coef = 1.0f / (s - (float)k);
-            atomicAdd(&tile[2 * (b * C + ch) + 0], (double)(Are * coef));
-            atomicAdd(&tile[2 * (b * C + ch) + 1], (double)(Aim * coef));
+            const int b = wrap_bin(center + k, N);
+            atomicAdd(&re[b], (double)(__int_as_float(hi ? rb : ra) * coef));
+            atomicAdd(&im[b], (double)(__int_as_float(hi ? ib : ia) * coef));
         }
+    }
+}
+
+// Adds the LDS tile into the workgroup's partial frame [N][C] (complex, frame order).
+__device__ inline void tile_flush(const double *tile, double2 *dst, int N, int C, int tid, int threads) {
+    const int NC = N * C;
+    for (int j = tid; j < NC; j += threads) {
+        const int t = tile_re(j / C, j % C, N);
+        double2 v = dst[j];
+        v.x += tile[t];
+        v.y += tile[NC + t];
+        dst[j] = v;
     }
 }
 

@@ -589,7 +589,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a
 // ------------------------------------------------------------------------ gather + splat
 __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, int depth, double2 *partials,
                                                         bool count) {
-    extern __shared__ double tile[];  // [N*C] complex double (one chirp); unused with M > 1
+    extern __shared__ double tile[];  // one chirp's N*C complex doubles (tile_re); unused with M > 1
     const int NC = a.N * a.C;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -659,13 +659,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
     if (count) atomicAdd(&a.counters[4], (unsigned long long)cev);
     if (cube) return;
     __syncthreads();
-    double2 *dst = partials + (size_t)blockIdx.x * NC;
-    for (int j = tid; j < NC; j += kWfBlock) {
-        double2 v = dst[j];
-        v.x += tile[2 * j];
-        v.y += tile[2 * j + 1];
-        dst[j] = v;
-    }
+    tile_flush(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
 }
 
 __global__ void k_wf_zero_counts(WfCounts *c, int n, uint32_t *bins) {
