@@ -71,8 +71,17 @@ hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t
 
 // ---------------------------------------------------------------- wavefront pipeline
 // Per-depth queue counters (device memory, zeroed per batch).
+// Queue sizes and consumer heads of one depth, each counter on a 256-byte line of its own:
+// device-scope atomics on one address are serialised (about 12 ns apiece on MI355X), while
+// atomics on distinct lines proceed in parallel.
+struct alignas(256) WfCounter {
+    unsigned int v;
+    unsigned int pad[63];
+};
+constexpr int kQueueSegments = 8;  // consumer heads per queue: one per XCD of MI355X
 struct WfCounts {
-    unsigned int n_ext, head_ext, n_shd, head_shd, n_gat, pad0, pad1, pad2;
+    WfCounter n_ext, n_shd, n_gat;
+    WfCounter head_ext[kQueueSegments], head_shd[kQueueSegments];
 };
 
 // Structure-of-arrays path state and queues for a batch of up to P paths.

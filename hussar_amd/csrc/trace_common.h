@@ -640,18 +640,6 @@ __device__ inline void trav_run(const float4 *__restrict__ nodes, const float4 *
     }
 }
 
-// Wave-aggregated append to a device queue: one atomic per wave; returns this lane's position
-// (or -1). All lanes of the wave must call it.
-__device__ inline int64_t wave_append(unsigned int *counter, bool pred, int lane) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return -1;
-    const int leader = __builtin_ctzll(m);
-    unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
-    base = __shfl(base, leader, 64);
-    return pred ? (int64_t)base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
-}
-
 // One NEE event ready for the frame: taps A·1/(s-k) at sample bins center+k (or one tap at
 // center when delta), and with motion and M > 1 chirps the same along the chirp axis
 // (A·1/(sc-l)·1/(s-k) at chirp cc+l), the separable kernel of Frame::splat (radar.h:564-627).

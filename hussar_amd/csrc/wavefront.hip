@@ -14,10 +14,13 @@
 //                 guiding record of paths that miss or graze, and the NEE set-up of depth
 //                 k+1 towards every receiver with its shadow packets
 //
-// Queues are built with one atomic per wave and their sizes stay on the device, so a batch
-// is enqueued without host synchronisation. The traversal kernels hold only the ray and
-// stack, which keeps their register use low and their occupancy high; a lane whose ray is
-// done takes the next one from the queue while the others continue.
+// Queue sizes stay on the device, so a batch is enqueued without host synchronisation.
+// Device-scope atomics on one counter are serialised (about 12 ns apiece on MI355X), so they
+// are kept rare: producers stage their queue entries in LDS and append a workgroup's range of
+// items with one atomic per queue, and the persistent traversals claim their queue in chunks
+// of kClaim items per atomic. The traversal kernels hold only the ray and stack, which keeps
+// their register use low and their occupancy high; a lane whose ray is done takes the next
+// one from its wave's claim while the others continue.
 #include "trace_common.h"
 
 namespace hsr {
@@ -46,6 +49,103 @@ constexpr int kWfBlock = 256;
 constexpr int kTravSteps = HSR_TRAV_STEPS;  // node visits between refills of idle lanes
 constexpr int kRefillMin = HSR_REFILL_MIN;  // idle lanes that trigger a refill
 constexpr int kPacket = 4;       // shadow rays (receivers of one path) traced together per lane
+constexpr int kMaxGroups = (kMaxChannels + kPacket - 1) / kPacket;  // shadow packets per path
+#ifndef HSR_CLAIM
+#define HSR_CLAIM 64
+#endif
+#ifndef HSR_STAGE_ITERS
+#define HSR_STAGE_ITERS 8
+#endif
+constexpr unsigned kClaim = HSR_CLAIM;          // queue items a traversal wave claims per atomic
+constexpr int kStageIters = HSR_STAGE_ITERS;    // workgroup rounds per producer range
+constexpr int kRange = kStageIters * kWfBlock;  // items of one producer range
+
+// Sum of a per-lane count over the wave, added to a device counter by one lane.
+__device__ inline void wave_count(unsigned long long *counter, uint32_t v, int lane) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) atomicAdd(counter, (unsigned long long)v);
+}
+
+// ------------------------------------------------------------------------ queues
+// Producer side: a workgroup stages the entries it appends to a queue over one range of items
+// in an LDS list (one LDS atomic per wave and append), then reserves the list's place in the
+// device queue with one atomic and copies it out. Entries keep the order of their range.
+__device__ inline void stage_append(int32_t *list, unsigned *sn, bool pred, int32_t value, int lane) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int leader = __builtin_ctzll(m);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(sn, (unsigned)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+// After a barrier that ends the range's appends: thread `who` reserves the list's place
+// (sb = position, sc = count) and empties the list's count for the next range.
+__device__ inline void stage_reserve(unsigned *sn, unsigned *sb, unsigned *sc, unsigned *counter, int who) {
+    if ((int)threadIdx.x == who) {
+        const unsigned c = *sn;
+        *sc = c;
+        *sb = c ? atomicAdd(counter, c) : 0u;
+        *sn = 0;
+    }
+}
+// After the barrier that follows stage_reserve: the workgroup copies the list out.
+__device__ inline void stage_copy(const int32_t *list, const unsigned *sb, const unsigned *sc, int32_t *queue) {
+    const unsigned c = *sc, b = *sb;
+    for (unsigned j = threadIdx.x; j < c; j += kWfBlock) queue[b + j] = list[j];
+}
+
+// Consumer side: a queue of n items is cut into kQueueSegments contiguous segments, each with
+// its own head counter. A persistent traversal wave claims positions [next, end) of the
+// segment of its workgroup's XCD (workgroups are dispatched to the XCDs in turn), kClaim at a
+// time with one atomic on that segment's head, and hands them to its idle lanes in order;
+// when the segment is used up it takes from the following segments. The XCDs then work on
+// distinct parts of the coherence-ordered queue, and the heads' atomics spread over
+// kQueueSegments lines.
+struct Claim {
+    unsigned next, end;
+    int seg, tried;
+    bool dry;  // every segment is used up: nothing is left to claim
+};
+
+__device__ inline Claim claim_begin() {
+    return Claim{0u, 0u, (int)(blockIdx.x % kQueueSegments), 0, false};
+}
+
+__device__ inline unsigned seg_lo(unsigned n, int s) {
+    return (unsigned)(((uint64_t)n * (uint64_t)s) / kQueueSegments);
+}
+
+// Positions for the lanes of `idle` (wave-uniform) from the wave's claim, claiming further
+// chunks as needed; returns this lane's position, or ~0u. Every lane of the wave calls it.
+__device__ inline unsigned claim_items(WfCounter *heads, unsigned n, uint64_t idle, int lane, Claim &c) {
+    unsigned my = ~0u;
+    while (idle) {
+        while (c.next >= c.end && !c.dry) {
+            const unsigned lo = seg_lo(n, c.seg), hi = seg_lo(n, c.seg + 1);
+            unsigned base = 0;
+            if (lane == 0) base = atomicAdd(&heads[c.seg].v, kClaim);
+            base = __shfl(base, 0, 64);
+            if (base < hi - lo) {
+                c.next = lo + base;
+                c.end = min(lo + base + kClaim, hi);
+            } else if (++c.tried == kQueueSegments) {
+                c.dry = true;
+            } else {
+                c.seg = (c.seg + 1) % kQueueSegments;
+            }
+        }
+        if (c.next >= c.end) break;
+        const unsigned take = min(c.end - c.next, (unsigned)__popcll(idle));
+        const unsigned rank = (unsigned)__popcll(idle & ((1ull << lane) - 1ull));
+        const bool mine = ((idle >> lane) & 1ull) && rank < take;
+        if (mine) my = c.next + rank;
+        idle &= ~__ballot(mine);
+        c.next += take;
+    }
+    return my;
+}
 
 __device__ inline void store_state(const WfArgs &w, int64_t i, const PathState &p) {
     double *d = w.sd;
@@ -140,20 +240,9 @@ __device__ inline void finish_record(const TraceArgs &a, bool finishing, const P
     record_stats(pend, leaf, val, a.gtStat, lane);
 }
 
-// Paths with an NEE at their current vertex go to the gather list; k_wf_nee then sets up one
-// shadow slot per (path, receiver) (all lanes of the wave call it).
-__device__ inline void emit_gather(const WfArgs &w, bool on, int64_t i, WfCounts *cnt, int lane) {
-    const int64_t g = wave_append(&cnt->n_gat, on, lane);
-    if (on) w.q_gat[g] = (int32_t)i;
-}
-
-// Queue the extension ray of a path that goes on, or flag it to end after its NEE.
-__device__ inline void emit_extend(const WfArgs &w, bool alive, bool ends, int64_t i, const PathState &p,
-                                   int32_t *qnext, WfCounts *cnt, int lane) {
-    const bool ext = alive && !ends;
-    const int64_t q = wave_append(&cnt->n_ext, ext, lane);
-    if (ext) {
-        qnext[q] = (int32_t)i;
+// The extension ray of a path that goes on, and the flag of a path that ends after its NEE.
+__device__ inline void write_extend(const WfArgs &w, bool alive, bool ends, int64_t i, const PathState &p) {
+    if (alive && !ends) {
         const f3 o = cast3<float>(p.o), d = cast3<float>(p.d);
         w.ray_o[i] = make_float4(o.x, o.y, o.z, 0.0f);
         w.ray_d[i] = make_float4(d.x, d.y, d.z, 0.0f);
@@ -246,24 +335,42 @@ __global__ __launch_bounds__(kWfBlock) void k_sort_scatter(WfArgs w) {
 }
 
 // ------------------------------------------------------------------------ generation
-// Thread s generates the path of batch index inv[s] into slot s.
+// Thread s generates the path of batch index inv[s] into slot s; a workgroup generates
+// ranges of kRange slots and stages their depth-0 gather and extension entries.
 __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
+    __shared__ int32_t l_gat[kRange], l_ext[kRange];
+    __shared__ unsigned sn[2], sb[2], sc[2];
     const int lane = threadIdx.x & 63;
-    const int64_t i = (int64_t)blockIdx.x * kWfBlock + threadIdx.x;
-    const bool valid = i < w.n;
-    PathState p;
-    if (valid) {
-        const int64_t m = w.inv[i];
-        p.local = w.base + m;
-        generate(a, (uint64_t)(a.offset + w.base + m), p);
+    if (threadIdx.x < 2) sn[threadIdx.x] = 0;
+    __syncthreads();
+    WfCounts *cnt = &w.cnt[0];
+    for (int64_t r0 = (int64_t)blockIdx.x * kRange; r0 < w.n; r0 += (int64_t)gridDim.x * kRange) {
+        for (int r = 0; r < kStageIters; ++r) {
+            const int64_t i = r0 + r * kWfBlock + threadIdx.x;
+            const bool valid = i < w.n;
+            PathState p;
+            if (valid) {
+                const int64_t m = w.inv[i];
+                p.local = w.base + m;
+                generate(a, (uint64_t)(a.offset + w.base + m), p);
+            }
+            const bool nee = valid && !a.onlyIndirect;
+            stage_append(l_gat, &sn[0], nee, (int32_t)i, lane);
+            const bool ends = valid && path_ends(a, p);
+            stage_append(l_ext, &sn[1], valid && !ends, (int32_t)i, lane);
+            write_extend(w, valid, ends, i, p);
+            if (valid) store_state(w, i, p);
+            if (valid && a.motion) store_motion(w, i, p);
+            finish_record(a, valid && ends && !nee, p, lane);
+        }
+        __syncthreads();
+        stage_reserve(&sn[0], &sb[0], &sc[0], &cnt->n_gat.v, 0);
+        stage_reserve(&sn[1], &sb[1], &sc[1], &cnt->n_ext.v, 64);
+        __syncthreads();
+        stage_copy(l_gat, &sb[0], &sc[0], w.q_gat);
+        stage_copy(l_ext, &sb[1], &sc[1], w.q_ext0);
+        __syncthreads();
     }
-    const bool nee = valid && !a.onlyIndirect;
-    emit_gather(w, nee, i, &w.cnt[0], lane);
-    const bool ends = valid && path_ends(a, p);
-    emit_extend(w, valid, ends, i, p, w.q_ext0, &w.cnt[0], lane);
-    if (valid) store_state(w, i, p);
-    if (valid && a.motion) store_motion(w, i, p);
-    finish_record(a, valid && ends && !nee, p, lane);
 }
 
 // ------------------------------------------------------------------------ traversal
@@ -277,23 +384,19 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int
     st.lds = lds_stack + tid;
     st.sp = 0;
     WfCounts *cnt = &w.cnt[depth];
-    const unsigned int n = kAny ? cnt->n_shd : cnt->n_ext;
-    unsigned int *head = kAny ? &cnt->head_shd : &cnt->head_ext;
+    const unsigned int n = kAny ? cnt->n_shd.v : cnt->n_ext.v;
+    WfCounter *heads = kAny ? cnt->head_shd : cnt->head_ext;
     const int32_t *queue = kAny ? w.q_shd : ((depth & 1) ? w.q_ext1 : w.q_ext0);
     uint32_t cn = 0, ct = 0, cr = 0;
     Trav t;
     int32_t item = -1;
-    bool has = false, exhausted = false;
+    bool has = false;
+    Claim cl = claim_begin();
     while (true) {
         const uint64_t idle = __ballot(!has);
-        if (!exhausted && __popcll(idle) >= kRefillMin) {
-            const int leader = __builtin_ctzll(idle);
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
-            base = __shfl(base, leader, 64);
-            if (base + (unsigned int)__popcll(idle) >= n) exhausted = true;
-            const unsigned int my = base + (unsigned int)__popcll(idle & ((1ull << lane) - 1ull));
-            if (!has && my < n) {
+        if (!cl.dry && __popcll(idle) >= kRefillMin) {
+            const unsigned int my = claim_items(heads, n, idle, lane, cl);
+            if (my != ~0u) {
                 item = queue[my];
                 const float4 o = kAny ? w.s_o[item] : w.ray_o[item];
                 const float4 d = kAny ? w.s_d[item] : w.ray_d[item];
@@ -303,7 +406,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int
             }
         }
         if (!__any(has)) {
-            if (exhausted) break;
+            if (cl.dry) break;
             continue;
         }
         if (has) {
@@ -316,9 +419,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int
         }
     }
     if (kCount) {
-        atomicAdd(&a.counters[0], (unsigned long long)cn);
-        atomicAdd(&a.counters[1], (unsigned long long)ct);
-        atomicAdd(&a.counters[kAny ? 3 : 2], (unsigned long long)cr);
+        wave_count(&a.counters[0], cn, lane);
+        wave_count(&a.counters[1], ct, lane);
+        wave_count(&a.counters[kAny ? 3 : 2], cr, lane);
     }
 }
 
@@ -420,24 +523,20 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
     st.lds = lds_stack + tid;
     st.sp = 0;
     WfCounts *cnt = &w.cnt[depth];
-    const unsigned int n = cnt->n_shd;
-    unsigned int *head = &cnt->head_shd;
+    const unsigned int n = cnt->n_shd.v;
+    WfCounter *heads = cnt->head_shd;
     const int nrx = min(a.rxCount, a.C);
     uint32_t cn = 0, ct = 0, cr = 0;
     Packet pk;
     int64_t slot0 = 0;
     int rays = 0;
-    bool has = false, exhausted = false;
+    bool has = false;
+    Claim cl = claim_begin();
     while (true) {
         const uint64_t idle = __ballot(!has);
-        if (!exhausted && __popcll(idle) >= kRefillMin) {
-            const int leader = __builtin_ctzll(idle);
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(head, (unsigned int)__popcll(idle));
-            base = __shfl(base, leader, 64);
-            if (base + (unsigned int)__popcll(idle) >= n) exhausted = true;
-            const unsigned int my = base + (unsigned int)__popcll(idle & ((1ull << lane) - 1ull));
-            if (!has && my < n) {
+        if (!cl.dry && __popcll(idle) >= kRefillMin) {
+            const unsigned int my = claim_items(heads, n, idle, lane, cl);
+            if (my != ~0u) {
                 const int32_t e = w.q_shd[my];
                 const int64_t i = e >> 2;
                 const int c0 = (e & 3) * kPacket;
@@ -467,7 +566,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
             }
         }
         if (!__any(has)) {
-            if (exhausted) break;
+            if (cl.dry) break;
             continue;
         }
         if (has) {
@@ -479,9 +578,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
         }
     }
     if (kCount) {
-        atomicAdd(&a.counters[5], (unsigned long long)cn);
-        atomicAdd(&a.counters[6], (unsigned long long)ct);
-        atomicAdd(&a.counters[3], (unsigned long long)cr);
+        wave_count(&a.counters[5], cn, lane);
+        wave_count(&a.counters[6], ct, lane);
+        wave_count(&a.counters[3], cr, lane);
     }
 }
 
@@ -509,48 +608,61 @@ __device__ inline bool write_slot(const TraceArgs &a, const WfArgs &w, int64_t j
 // packet per four receivers). The vertex (normal, incoming direction and field, cosθ) stays
 // in registers from the hit update to the NEE set-up.
 __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs a, WfArgs w, int depth) {
+    __shared__ int32_t l_gat[kRange], l_ext[kRange], l_shd[kRange * kMaxGroups];
+    __shared__ unsigned sn[3], sb[3], sc[3];
     const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
-    const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
-    const unsigned int n = w.cnt[depth].n_ext;
+    if (threadIdx.x < 3) sn[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned int n = w.cnt[depth].n_ext.v;
     const int32_t *q = (depth & 1) ? w.q_ext1 : w.q_ext0;
     int32_t *qnext = (depth & 1) ? w.q_ext0 : w.q_ext1;
     WfCounts *next = &w.cnt[depth + 1];
     const int nrx = min(a.rxCount, a.C);
     const int groups = (nrx + kPacket - 1) / kPacket;
-    for (int64_t b = wave * 64; b < n; b += waves * 64) {
-        const int64_t k = b + lane;
-        const bool valid = k < n;
-        const int64_t i = valid ? q[k] : 0;
-        PathState p;
-        bool alive = false;
-        if (valid) {
-            load_chain(w, i, p);
-            load_field(w, i, p);
-            if (a.motion) load_motion(w, i, p);
-            const int2 h = w.hit[i];
-            const int32_t id = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
-            if (a.recHits && p.local < a.recPaths) a.recHits[p.local * a.maxDepth + p.depth] = id;
-            alive = h.x >= 0 && apply_hit(a, p, h.x);
-            if (!alive) load_record(w, i, p);
-        }
-        emit_gather(w, alive, i, next, lane);
-        const bool ends = alive && path_ends(a, p);
-        emit_extend(w, alive, ends, i, p, qnext, next, lane);
-        if (alive) store_hit(w, i, p);
-        if (alive && a.motion) store_motion(w, i, p);
-        finish_record(a, valid && !alive, p, lane);
-        // NEE set-up at the new vertex (path.h:290-397 of the next bounce)
-        if (alive) p.freq = w.sf[F_FREQ * w.P + i];
-        for (int g = 0; g < groups; ++g) {
-            bool lead = false;
-            if (alive) {
-                const int c1 = min(nrx, (g + 1) * kPacket);
-                for (int c = g * kPacket; c < c1; ++c) lead = write_slot(a, w, i * a.C + c, nee_setup(a, p, c));
+    for (int64_t r0 = (int64_t)blockIdx.x * kRange; r0 < n; r0 += (int64_t)gridDim.x * kRange) {
+        for (int r = 0; r < kStageIters; ++r) {
+            const int64_t k = r0 + r * kWfBlock + threadIdx.x;
+            const bool valid = k < n;
+            const int64_t i = valid ? q[k] : 0;
+            PathState p;
+            bool alive = false;
+            if (valid) {
+                load_chain(w, i, p);
+                load_field(w, i, p);
+                if (a.motion) load_motion(w, i, p);
+                const int2 h = w.hit[i];
+                const int32_t id = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
+                if (a.recHits && p.local < a.recPaths) a.recHits[p.local * a.maxDepth + p.depth] = id;
+                alive = h.x >= 0 && apply_hit(a, p, h.x);
+                if (!alive) load_record(w, i, p);
             }
-            const int64_t qs = wave_append(&next->n_shd, lead, lane);
-            if (lead) w.q_shd[qs] = (int32_t)(i * 4 + g);
+            stage_append(l_gat, &sn[0], alive, (int32_t)i, lane);
+            const bool ends = alive && path_ends(a, p);
+            stage_append(l_ext, &sn[1], alive && !ends, (int32_t)i, lane);
+            write_extend(w, alive, ends, i, p);
+            if (alive) store_hit(w, i, p);
+            if (alive && a.motion) store_motion(w, i, p);
+            finish_record(a, valid && !alive, p, lane);
+            // NEE set-up at the new vertex (path.h:290-397 of the next bounce)
+            if (alive) p.freq = w.sf[F_FREQ * w.P + i];
+            for (int g = 0; g < groups; ++g) {
+                bool lead = false;
+                if (alive) {
+                    const int c1 = min(nrx, (g + 1) * kPacket);
+                    for (int c = g * kPacket; c < c1; ++c) lead = write_slot(a, w, i * a.C + c, nee_setup(a, p, c));
+                }
+                stage_append(l_shd, &sn[2], lead, (int32_t)(i * 4 + g), lane);
+            }
         }
+        __syncthreads();
+        stage_reserve(&sn[0], &sb[0], &sc[0], &next->n_gat.v, 0);
+        stage_reserve(&sn[1], &sb[1], &sc[1], &next->n_ext.v, 64);
+        stage_reserve(&sn[2], &sb[2], &sc[2], &next->n_shd.v, 128);
+        __syncthreads();
+        stage_copy(l_gat, &sb[0], &sc[0], w.q_gat);
+        stage_copy(l_ext, &sb[1], &sc[1], qnext);
+        stage_copy(l_shd, &sb[2], &sc[2], w.q_shd);
+        __syncthreads();
     }
 }
 
@@ -558,31 +670,39 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
 // The direct paths Tx -> Rx (onlyIndirect off): items j < n_gat * nrx of the depth-0 gather
 // list, path q_gat[j / nrx], receiver j % nrx.
 __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a, WfArgs w) {
+    __shared__ int32_t l_shd[kRange];
+    __shared__ unsigned sn, sb, sc;
     const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + threadIdx.x) >> 6;
-    const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
+    if (threadIdx.x == 0) sn = 0;
+    __syncthreads();
     const int nrx = min(a.rxCount, a.C);
     WfCounts *cnt = &w.cnt[0];
-    const int64_t n = (int64_t)cnt->n_gat * nrx;
-    for (int64_t b = wave * 64; b < n; b += waves * 64) {
-        const int64_t k = b + lane;
-        const bool valid = k < n;
-        bool test = false;
-        int64_t j = 0;
-        if (valid) {
-            const int64_t i = w.q_gat[k / nrx];
-            const int ch = (int)(k % nrx);
-            PathState p;
-            load_chain(w, i, p);
-            p.freq = w.sf[F_FREQ * w.P + i];
-            if (a.motion) load_motion(w, i, p);
-            j = i * a.C + ch;
-            test = write_slot(a, w, j, nee_setup(a, p, ch));
+    const int64_t n = (int64_t)cnt->n_gat.v * nrx;
+    for (int64_t r0 = (int64_t)blockIdx.x * kRange; r0 < n; r0 += (int64_t)gridDim.x * kRange) {
+        for (int r = 0; r < kStageIters; ++r) {
+            const int64_t k = r0 + r * kWfBlock + threadIdx.x;
+            const bool valid = k < n;
+            bool test = false;
+            int64_t j = 0;
+            if (valid) {
+                const int64_t i = w.q_gat[k / nrx];
+                const int ch = (int)(k % nrx);
+                PathState p;
+                load_chain(w, i, p);
+                p.freq = w.sf[F_FREQ * w.P + i];
+                if (a.motion) load_motion(w, i, p);
+                j = i * a.C + ch;
+                test = write_slot(a, w, j, nee_setup(a, p, ch));
+            }
+            // one shadow packet per four receivers of a path: entry = path * 4 + group
+            const bool lead = test && (k % nrx) % kPacket == 0;
+            stage_append(l_shd, &sn, lead, (int32_t)((j / a.C) * 4 + (int32_t)((k % nrx) / kPacket)), lane);
         }
-        // one shadow packet per four receivers of a path: entry = path * 4 + group
-        const bool lead = test && (k % nrx) % kPacket == 0;
-        const int64_t q = wave_append(&cnt->n_shd, lead, lane);
-        if (lead) w.q_shd[q] = (int32_t)((j / a.C) * 4 + (int32_t)((k % nrx) / kPacket));
+        __syncthreads();
+        stage_reserve(&sn, &sb, &sc, &cnt->n_shd.v, 0);
+        __syncthreads();
+        stage_copy(l_shd, &sb, &sc, w.q_shd);
+        __syncthreads();
     }
 }
 
@@ -600,7 +720,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
     }
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + tid) >> 6;
     const int64_t waves = ((int64_t)gridDim.x * kWfBlock) >> 6;
-    const unsigned int n = w.cnt[depth].n_gat;
+    const unsigned int n = w.cnt[depth].n_gat.v;
     const int nrx = min(a.rxCount, a.C);
     const int64_t P = w.P;
     uint32_t cev = 0;
@@ -656,7 +776,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
         const bool finishing = valid && (w.si[I_FLAGS * P + i] & 1);
         finish_record(a, finishing, p, lane);
     }
-    if (count) atomicAdd(&a.counters[4], (unsigned long long)cev);
+    if (count) wave_count(&a.counters[4], cev, lane);
     if (cube) return;
     __syncthreads();
     tile_flush(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
@@ -664,7 +784,15 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
 
 __global__ void k_wf_zero_counts(WfCounts *c, int n, uint32_t *bins) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) c[i] = WfCounts{0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        c[i].n_ext.v = 0;
+        c[i].n_shd.v = 0;
+        c[i].n_gat.v = 0;
+        for (int k = 0; k < kQueueSegments; ++k) {
+            c[i].head_ext[k].v = 0;
+            c[i].head_shd[k].v = 0;
+        }
+    }
     if (i < kSortBins) bins[i] = 0;
 }
 
@@ -742,19 +870,20 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     // device; the batch's path count bounds them), so that small batches launch small grids
     const int64_t n = w.n;
     const int nrx = min(a.rxCount, a.C);
-    auto fit = [](int grid, int64_t items) {
-        return (int)std::max<int64_t>(1, std::min<int64_t>(grid, (items + kWfBlock - 1) / kWfBlock));
+    auto fit = [](int grid, int64_t items, int64_t per) {
+        return (int)std::max<int64_t>(1, std::min<int64_t>(grid, (items + per - 1) / per));
     };
-    const int gClosest = fit(gridTrav, n);
-    const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket));
-    const int gHit = fit(gridWork, n);
-    const int gNee = fit(gridWork, n * nrx);
+    const int gClosest = fit(gridTrav, n, kWfBlock);
+    const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket), kWfBlock);
+    const int gShade = fit(gridWork, n, kRange);
+    const int gNee = fit(gridWork, n * nrx, kRange);
+    const int gGen = fit(1 << 30, n, kRange);
     hipLaunchKernelGGL(k_wf_zero_counts, dim3((max(maxDepth + 2, kSortBins) + 255) / 256), dim3(256), 0, s, w.cnt,
                        maxDepth + 2, w.bins);
     hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(kWfBlock), 0, s, a, w);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w);
     hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(kWfBlock), 0, s, w);
-    hipLaunchKernelGGL(k_wf_gen, dim3((unsigned)((w.n + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s, a, w);
+    hipLaunchKernelGGL(k_wf_gen, dim3(gGen), dim3(kWfBlock), 0, s, a, w);
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
     if (glds > 65536) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_wf_gather,
@@ -783,7 +912,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
                 hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
             (void)hipEventRecord(evClosest[2 * *nClosest + 1], s);
             ++*nClosest;
-            hipLaunchKernelGGL(k_wf_shade, dim3(gHit), dim3(kWfBlock), 0, s, a, w, k);
+            hipLaunchKernelGGL(k_wf_shade, dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
         }
     }
     return hipGetLastError();
