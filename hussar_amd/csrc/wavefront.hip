@@ -124,8 +124,14 @@ __device__ inline unsigned claim_items(WfCounter *heads, unsigned n, uint64_t id
     while (idle) {
         while (c.next >= c.end && !c.dry) {
             const unsigned lo = seg_lo(n, c.seg), hi = seg_lo(n, c.seg + 1);
-            unsigned base = 0;
-            if (lane == 0) base = atomicAdd(&heads[c.seg].v, kClaim);
+            // in the segments of other XCDs, a plain load first: a used-up segment then costs
+            // no atomic (the waves that find nothing left would otherwise queue up on the
+            // heads at the end of every launch)
+            unsigned base = hi - lo;
+            if (lane == 0 &&
+                (c.tried == 0 ||
+                 __hip_atomic_load(&heads[c.seg].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < hi - lo))
+                base = atomicAdd(&heads[c.seg].v, kClaim);
             base = __shfl(base, 0, 64);
             if (base < hi - lo) {
                 c.next = lo + base;
