@@ -29,7 +29,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     for (int i = tid; i < 2 * NC; i += kTraceBlock) tile[i] = 0.0;
     __syncthreads();
 
+    int32_t spill[kStackSpill];
     Stack st;
+    st.spill = spill;
     st.lds = lds_stack + tid;
     st.sp = 0;
     uint32_t cn = 0, ct = 0, crc = 0, crs = 0, cev = 0;

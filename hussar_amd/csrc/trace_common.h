@@ -34,10 +34,13 @@ __device__ inline mat3<double> rotd(const DevAntenna &a) {
 __device__ inline d3 posd(const DevAntenna &a) { return {(double)a.pos[0], (double)a.pos[1], (double)a.pos[2]}; }
 
 // ------------------------------------------------------------------------ traversal stack
+// The spill entries live in a private array of the kernel, reached through a pointer, so that
+// the stack pointer itself stays in a register (kept in one aggregate with the array, it would
+// be placed in private memory with it and every push and pop would load it from there).
 struct Stack {
-    int32_t *lds;  // column of this thread: lds[i * kTraceBlock]
+    int32_t *lds;    // column of this thread: lds[i * kTraceBlock]
+    int32_t *spill;  // [kStackSpill] private
     int sp;
-    int32_t spill[kStackSpill];
     __device__ inline void push(int32_t v) {
         if (sp < kStackShort) lds[sp * kTraceBlock] = v;
         else spill[sp - kStackShort] = v;
@@ -84,9 +87,11 @@ struct Node4 {
     int4 ch;
 };
 
+// q * 2^k is exact for q < 256, so one fused multiply-add rounds exactly as the product and
+// sum do separately (the form the builder checked its boxes with).
 __device__ inline float4 dequant4(float o, uint32_t w, float s) {
-    return make_float4(o + (float)(w & 0xFFu) * s, o + (float)((w >> 8) & 0xFFu) * s,
-                       o + (float)((w >> 16) & 0xFFu) * s, o + (float)(w >> 24) * s);
+    return make_float4(__builtin_fmaf((float)(w & 0xFFu), s, o), __builtin_fmaf((float)((w >> 8) & 0xFFu), s, o),
+                       __builtin_fmaf((float)((w >> 16) & 0xFFu), s, o), __builtin_fmaf((float)(w >> 24), s, o));
 }
 
 // The float layout of the same node (128 B = 8 float4: lo.x[4] | hi.x[4] | lo.y[4] | hi.y[4] |

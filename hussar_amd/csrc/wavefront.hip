@@ -386,7 +386,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int
     __shared__ int32_t lds_stack[kStackShort * kWfBlock];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    int32_t spill[kStackSpill];
     Stack st;
+    st.spill = spill;
     st.lds = lds_stack + tid;
     st.sp = 0;
     WfCounts *cnt = &w.cnt[depth];
@@ -525,7 +527,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
     __shared__ int32_t lds_stack[kStackShort * kWfBlock];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    int32_t spill[kStackSpill];
     Stack st;
+    st.spill = spill;
     st.lds = lds_stack + tid;
     st.sp = 0;
     WfCounts *cnt = &w.cnt[depth];
