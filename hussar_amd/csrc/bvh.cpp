@@ -17,8 +17,18 @@
 namespace hsr {
 namespace {
 
-constexpr int kBins = 16;
-constexpr int kMaxLeaf = 8;
+#ifndef HSR_SAH_BINS
+#define HSR_SAH_BINS 16
+#endif
+#ifndef HSR_SAH_TRAV
+#define HSR_SAH_TRAV 0.5f
+#endif
+constexpr int kBins = HSR_SAH_BINS;
+constexpr float kSahTrav = HSR_SAH_TRAV;  // cost of a node test relative to a triangle test
+#ifndef HSR_MAX_LEAF
+#define HSR_MAX_LEAF 8
+#endif
+constexpr int kMaxLeaf = HSR_MAX_LEAF;
 constexpr int kMedianDepth = 48;
 constexpr int kMaxDepth = 80;
 
@@ -142,8 +152,8 @@ struct Builder {
             if (n <= kMaxLeaf) return -1;
             return b + n / 2;  // coincident centroids: split by count
         }
-        // SAH with traversal cost 1 per node test relative to 1 per triangle test
-        if (n <= kMaxLeaf && leaf_cost <= best + bb.area()) return -1;
+        // SAH: a leaf when its triangle tests cost no more than a node test and the split
+        if (n <= kMaxLeaf && leaf_cost <= best + kSahTrav * bb.area()) return -1;
         const int k = best_axis;
         const float scale = (float)kBins / (cb.hi[k] - cb.lo[k]);
         auto it = std::partition(order.begin() + b, order.begin() + e, [&](int32_t t) {
