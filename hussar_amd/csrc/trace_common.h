@@ -760,43 +760,26 @@ __device__ inline int wrap_bin(int v, int N) {
 // distinct, consecutive banks.
 __device__ inline int tile_re(int b, int c, int N) { return c * N + b; }
 
-// Splat of the wave's events (one per lane at most, all of frame channel ch) into the LDS tile.
-// Taps k = -16..15 go two events per pass, one per half-wave (lane & 31 = k + 16); the
-// k = +16 taps and the single taps of delta events are added lane-parallel, one per lane.
+// Splat of the wave's events (one per lane at most, all of frame channel ch) into the LDS tile:
+// each lane adds its own event's 33 taps, starting at tap (lane mod 33) so that lanes whose
+// events share a centre bin (neighbouring paths often do) touch different bins at each step.
+// The tap weight 1/(s-k) uses the hardware reciprocal (1 ulp).
 __device__ inline void flush_events(bool has, const Event &e, int ch, double *tile, int N, int C, int lane) {
     double *re = tile + ch * N, *im = tile + N * C + ch * N;
-    const bool spread = has && !e.delta;
-    if (has) {
-        const int k = e.delta ? 0 : 16;
+    if (!has) return;
+    if (e.delta) {
+        const int b = wrap_bin(e.center, N);
+        atomicAdd(&re[b], (double)e.A.re);
+        atomicAdd(&im[b], (double)e.A.im);
+        return;
+    }
+    int k = lane % 33 - 16;
+    for (int it = 0; it < 33; ++it) {
         const int b = wrap_bin(e.center + k, N);
-        const float coef = e.delta ? 1.0f : 1.0f / (e.s - (float)k);
+        const float coef = __builtin_amdgcn_rcpf(e.s - (float)k);
         atomicAdd(&re[b], (double)(e.A.re * coef));
         atomicAdd(&im[b], (double)(e.A.im * coef));
-    }
-    uint64_t m = __ballot(spread);
-    const bool hi = lane >= 32;
-    const int k = (lane & 31) - 16;
-    while (m) {
-        const int sa = __builtin_ctzll(m);
-        m &= m - 1;
-        const int sb = m ? __builtin_ctzll(m) : sa;
-        const bool two = m != 0;
-        m &= m - 1;
-        const int ca = __builtin_amdgcn_readlane(e.center, sa), cb = __builtin_amdgcn_readlane(e.center, sb);
-        const int fa = __builtin_amdgcn_readlane(__float_as_int(e.s), sa);
-        const int fb = __builtin_amdgcn_readlane(__float_as_int(e.s), sb);
-        const int ra = __builtin_amdgcn_readlane(__float_as_int(e.A.re), sa);
-        const int rb = __builtin_amdgcn_readlane(__float_as_int(e.A.re), sb);
-        const int ia = __builtin_amdgcn_readlane(__float_as_int(e.A.im), sa);
-        const int ib = __builtin_amdgcn_readlane(__float_as_int(e.A.im), sb);
-        if (!hi || two) {
-            const int center = hi ? cb : ca;
-            const float s = __int_as_float(hi ? fb : fa);
-            const float coef = 1.0f / (s - (float)k);
-            const int b = wrap_bin(center + k, N);
-            atomicAdd(&re[b], (double)(__int_as_float(hi ? rb : ra) * coef));
-            atomicAdd(&im[b], (double)(__int_as_float(hi ? ib : ia) * coef));
-        }
+        k = k == 16 ? -16 : k + 1;
     }
 }
 
