@@ -629,6 +629,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
     WfCounts *next = &w.cnt[depth + 1];
     const int nrx = min(a.rxCount, a.C);
     const int groups = (nrx + kPacket - 1) / kPacket;
+    const bool recording = a.doGuiding && a.guideOn && !a.isFinal;  // finish_record's condition
     for (int64_t r0 = (int64_t)blockIdx.x * kRange; r0 < n; r0 += (int64_t)gridDim.x * kRange) {
         for (int r = 0; r < kStageIters; ++r) {
             const int64_t k = r0 + r * kWfBlock + threadIdx.x;
@@ -637,14 +638,20 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
             PathState p;
             bool alive = false;
             if (valid) {
-                load_chain(w, i, p);
-                load_field(w, i, p);
-                if (a.motion) load_motion(w, i, p);
+                // a missed path needs none of the chain and field (most rays of a batch's
+                // first bounce miss): only its depth and index, and its record when training
                 const int2 h = w.hit[i];
-                const int32_t id = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
-                if (a.recHits && p.local < a.recPaths) a.recHits[p.local * a.maxDepth + p.depth] = id;
-                alive = h.x >= 0 && apply_hit(a, p, h.x);
-                if (!alive) load_record(w, i, p);
+                p.depth = w.si[I_DEPTH * w.P + i];
+                p.local = w.base + w.si[I_LOCAL * w.P + i];
+                if (a.recHits && p.local < a.recPaths)
+                    a.recHits[p.local * a.maxDepth + p.depth] = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
+                if (h.x >= 0) {
+                    load_chain(w, i, p);
+                    load_field(w, i, p);
+                    if (a.motion) load_motion(w, i, p);
+                    alive = apply_hit(a, p, h.x);
+                }
+                if (!alive && recording) load_record(w, i, p);
             }
             stage_append(l_gat, &sn[0], alive, (int32_t)i, lane);
             const bool ends = alive && path_ends(a, p);
