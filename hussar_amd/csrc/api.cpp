@@ -25,6 +25,12 @@
 #endif
 // gather workgroups per CU (each holds a frame tile in LDS and adds a partial frame)
 static constexpr int kGatherPerCU = HSR_GATHER_PER_CU;
+// cube frames: at most this many doubles of partial cubes for the tile splat (1 GiB), and
+// the partial cubes a serving slot reserves (frames' worth)
+static constexpr int64_t kCubePartDoubles = int64_t(1) << 27;
+namespace hsr {
+constexpr int64_t kCubeReserveParts = 4;
+}
 
 namespace hsr {
 
@@ -143,6 +149,9 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_stage_sum);
     dfree(b->d_stage_comp);
     dfree(b->d_counters);
+    dfree(b->d_events);
+    dfree(b->d_evCount);
+    dfree(b->d_cubeParts);
     dfree(b->d_recHits);
     dfree(b->d_recNee);
     dfree(b->d_vel);
@@ -160,6 +169,27 @@ void hsr_hip_destroy(hsr_backend *b) {
 }  // extern "C"
 
 namespace hsr {
+hsr_status backend_events(hsr_backend *b, int64_t partDoubles, unsigned events) {
+    hsr_status s;
+    if (!b->d_evCount) {
+        if ((s = dalloc(&b->d_evCount, 64)) != HSR_OK) return s;
+        HSR_HIP(hipMemset(b->d_evCount, 0, 64 * sizeof(unsigned)));
+    }
+    if (events > b->evCap) {
+        dfree(b->d_events);
+        b->evCap = 0;
+        if ((s = dalloc(&b->d_events, events)) != HSR_OK) return s;
+        b->evCap = events;
+    }
+    if (partDoubles > b->cubePartsCap) {
+        dfree(b->d_cubeParts);
+        b->cubePartsCap = 0;
+        if ((s = dalloc(&b->d_cubeParts, (size_t)partDoubles)) != HSR_OK) return s;
+        b->cubePartsCap = partDoubles;
+    }
+    return HSR_OK;
+}
+
 hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64_t frameDoubles) {
     HSR_HIP(hipSetDevice(b->device));
     const size_t need = wavefront_bytes(P, C, maxDepth);
@@ -199,7 +229,10 @@ hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64
         if ((s = dalloc(&b->d_frame_comp, frameDoubles)) != HSR_OK) return s;
         b->frameCap = frameDoubles;
     }
-    return HSR_OK;
+    // the tile splat: partial cubes of kCubeReserveParts frames, and an event list of four
+    // events per path of a batch (more go straight to the frame with device atomics)
+    return backend_events(b, kCubeReserveParts * frameDoubles,
+                          (unsigned)std::min<int64_t>(kCubeEvents, 4 * P));
 }
 
 int64_t backend_mesh_bytes(const hsr_backend *b) {
@@ -411,6 +444,23 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.vel = mesh->d_vel;
     a.frameAcc = cube ? b->d_frame_sum : nullptr;
     a.cube = cube;
+    // cube: events go through LDS tiles of cubeMt chirps of one channel when a chirp fits,
+    // cubeG workgroups (and partial cubes) per tile, at most kCubePartDoubles of partials
+    const int cubeMt = cube ? hsr::cube_tile_chirps(chirps, N, (size_t)b->ldsPerBlock) : 0;
+    const int cubeTiles = cubeMt > 0 ? C * ((chirps + cubeMt - 1) / cubeMt) : 1;
+    const int64_t cubeCells = (int64_t)chirps * N * C;
+    int cubeG = (int)std::min<int64_t>(std::max(1, (2 * b->cu + cubeTiles - 1) / cubeTiles),
+                                       kCubePartDoubles / (2 * cubeCells));
+    if (cubeMt > 0 && cubeG >= 1 && b->optPipeline == 1) {
+        // a slot that reserved less room for partial cubes uses fewer of them
+        if (b->cubePartsCap >= 2 * cubeCells)
+            cubeG = (int)std::min<int64_t>(cubeG, b->cubePartsCap / (2 * cubeCells));
+        hsr_status s = hsr::backend_events(b, (int64_t)cubeG * 2 * cubeCells, b->evCap ? b->evCap : hsr::kCubeEvents);
+        if (s != HSR_OK) return s;
+        a.events = b->d_events;
+        a.evCount = b->d_evCount;
+        a.evCap = b->evCap;
+    }
 
     // grid: as many workgroups as fit at once, fewer for small budgets (>= 8 paths per lane)
     if (b->maxGrid == 0) {
@@ -515,6 +565,9 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         hipEvent_t *evC = b->evTrav.data(), *evS = b->evTrav.data() + nev / 2;
         HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
                                       b->stream, evC, &nClosest, evS, &nShadow));
+        if (a.events)
+            HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
+                                           (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));
         HSR_HIP(hipEventRecord(b->ev[1], b->stream));
         // LDS tiles: the workgroup tiles are reduced into the frame; cube: the gather kernel
         // has added into the frame directly

@@ -83,6 +83,12 @@ struct hsr_backend {
     double *d_stage_sum = nullptr, *d_stage_comp = nullptr;
     size_t stageCap = 0;
     unsigned long long *d_counters = nullptr;
+    // cube frames: the event list of the LDS tile splat (evCap entries) and its count
+    hsr::CubeEvent *d_events = nullptr;
+    unsigned evCap = 0;
+    unsigned *d_evCount = nullptr;
+    double *d_cubeParts = nullptr;  // partial cubes of the tile splat (grow-only)
+    int64_t cubePartsCap = 0;       // doubles
     int32_t *d_recHits = nullptr;
     int8_t *d_recNee = nullptr;
     int64_t recCap = 0, recPathsLast = 0;
@@ -114,6 +120,9 @@ namespace hsr {
 // bounces, and a frame of frameDoubles doubles, ahead of time (serving slots allocate nothing
 // while requests run).
 hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64_t frameDoubles);
+// Allocates a cube event list of at least `events` entries (its count starts at zero) and room
+// for partial cubes of partDoubles doubles in all (both grow-only).
+hsr_status backend_events(hsr_backend *b, int64_t partDoubles, unsigned events);
 // Device bytes of a backend's mesh (BVH nodes, triangle slots, velocities).
 int64_t backend_mesh_bytes(const hsr_backend *b);
 }  // namespace hsr

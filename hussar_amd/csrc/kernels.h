@@ -12,6 +12,19 @@ constexpr int kStackShort = 16;       // LDS-staged traversal stack entries per 
 constexpr int kStackSpill = 128;      // further entries per thread in private memory (16 + 128 =
                                       // kStackEntries of bvh.h: three per level of the 4-wide tree)
 
+// One splat event of a cube frame, listed by the gather kernel for the LDS tile splat
+// (k_cube_splat): the fields of Event plus the frame channel.
+struct CubeEvent {
+    int32_t center;  // sample bin of the centre tap
+    float s;         // fractional sample offset (0 with delta)
+    float are, aim;  // amplitude
+    int32_t cc;      // chirp of the centre tap
+    float sc;        // fractional chirp offset (0 with deltaC)
+    int32_t flags;   // bit 0 delta, bit 1 deltaC, bits 2.. frame channel
+    int32_t pad;
+};
+constexpr unsigned kCubeEvents = 1u << 22;  // capacity of a backend's event list (128 MiB)
+
 struct DevAntenna {
     float pos[3];
     float rot[9];
@@ -65,6 +78,11 @@ struct TraceArgs {
     const float4 *vel;  // per triangle slot velocity (xyz) or null
     double *frameAcc;   // cube: the frame sum [M][N][C] complex, accumulated with atomics
     int32_t cube;       // 1: splat into frameAcc (M > 1, or one chirp too large for an LDS tile)
+    // cube: the event list for k_cube_splat (null: every event splats with device atomics);
+    // events past evCap are splatted with device atomics at once
+    CubeEvent *events;
+    unsigned *evCount;
+    unsigned evCap;
 };
 
 hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t stream);
@@ -137,6 +155,13 @@ size_t gather_lds_bytes(int N, int C);
 hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum, double *stage_comp, int S,
                          double *frame_sum, double *frame_comp, hipStream_t stream);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t stream);
+// Cube frames: chirps per LDS tile of one channel (0: a chirp of N samples does not fit).
+int cube_tile_chirps(int M, int N, size_t ldsBytes);
+// Adds the listed events (count on the device) into the frame cube [M][N][C] through LDS
+// tiles of Mt chirps of one channel, G workgroups per tile and G partial cubes in parts
+// (G * M * N * C complex), then empties the list.
+hipError_t launch_cube_splat(const CubeEvent *events, unsigned *count, unsigned cap, int M, int N, int C, int Mt,
+                             int G, double2 *parts, double *frame, hipStream_t stream);
 size_t trace_lds_bytes(int N, int C);
 
 // ---------------------------------------------------------------- frame post-processing

@@ -199,7 +199,9 @@ hsr_status hsr_server_create(const hsr_server_config *config, hsr_server **out) 
     s->cfg = c;
     s->frameReserveDoubles = c.frameReserveBytes / 16;  // sum + compensation
     const int64_t wf = (int64_t)hsr::wavefront_bytes(c.pathsPerBatch, HSR_MAX_CHANNELS, c.maxDepth);
-    s->slotBytes = wf + 2 * s->frameReserveDoubles * 8 + (int64_t(1) << 26);  // + tiles and small buffers
+    const int64_t events = std::min<int64_t>(hsr::kCubeEvents, 4 * c.pathsPerBatch);
+    s->slotBytes = wf + (2 + 4) * s->frameReserveDoubles * 8 + events * (int64_t)sizeof(hsr::CubeEvent) +
+                   (int64_t(1) << 26);  // + partial cubes, the cube event list, tiles and small buffers
     if ((int64_t)c.slots * s->slotBytes > c.hbmBudget)
         return hsr::fail(HSR_ERR_OOM, "the slots' reservations exceed the HBM budget");
     s->slots.resize(c.slots);

@@ -827,6 +827,37 @@ __device__ inline void flush_events_cube(bool has, const Event &e, int ch, doubl
     }
 }
 
+// A cube event of this lane (if any) into the event list for the LDS tile splat, or, without
+// a list or past its capacity, straight into the frame cube with device atomics. All lanes
+// of the wave call it.
+__device__ inline void cube_event(bool has, const Event &e, int ch, const TraceArgs &a, int lane) {
+    bool direct = has;
+    if (a.events) {
+        const uint64_t m = __ballot(has);
+        if (m) {
+            const int leader = __builtin_ctzll(m);
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(a.evCount, (unsigned)__popcll(m));
+            base = __shfl(base, leader, 64);
+            const unsigned pos = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+            if (has && pos < a.evCap) {
+                CubeEvent c;
+                c.center = e.center;
+                c.s = e.s;
+                c.are = e.A.re;
+                c.aim = e.A.im;
+                c.cc = e.cc;
+                c.sc = e.sc;
+                c.flags = (e.delta ? 1 : 0) | (e.deltaC ? 2 : 0) | (ch << 2);
+                c.pad = 0;
+                a.events[pos] = c;
+                direct = false;
+            }
+        }
+    }
+    flush_events_cube(direct, e, ch, a.frameAcc, a.M, a.N, a.C, lane);
+}
+
 // Wave-aggregated FP64 add into the training statistics: lanes with the same leaf are summed
 // with a butterfly and one lane issues the global atomic.
 __device__ inline void record_stats(bool pending, int leaf, double val, double *stat, int lane) {

@@ -782,7 +782,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
                 if (rec) a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + a.chBase + ch] = flag;
             }
             if (cube)
-                flush_events_cube(has, ev, a.chBase + ch, a.frameAcc, a.M, a.N, a.C, lane);
+                cube_event(has, ev, a.chBase + ch, a, lane);
             else
                 flush_events(has, ev, a.chBase + ch, tile, a.N, a.C, lane);
         }
@@ -797,6 +797,87 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
     if (cube) return;
     __syncthreads();
     tile_flush(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
+}
+
+// ------------------------------------------------------------------------ cube splat
+// The listed events of a cube frame into the frame through LDS: workgroup blockIdx / G owns
+// the tile of channel c and chirps [m0, m0 + Mt) (all samples), blockIdx % G = g its share
+// of the list. A wave loads 64 events, keeps those of its channel, and adds each one's taps
+// inside the tile with LDS atomics, spread over its lanes (the taps and their weights are
+// those of flush_events_cube). The tile is then stored into partial cube g, of which every
+// cell is written by exactly one workgroup, and k_cube_reduce adds the G partial cubes into
+// the frame in a fixed order: no device atomics at all.
+__global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__restrict__ ev, const unsigned *count,
+                                                         unsigned cap, int M, int N, int C, int Mt, int G,
+                                                         double2 *__restrict__ parts) {
+    extern __shared__ double t[];  // re [Mt * N], then im [Mt * N]
+    const int mTiles = (M + Mt - 1) / Mt;
+    const int tile = blockIdx.x / G, part = blockIdx.x % G;
+    const int c = tile / mTiles, m0 = (tile % mTiles) * Mt;
+    const int mt = min(Mt, M - m0);
+    const int TN = Mt * N;
+    for (int j = threadIdx.x; j < 2 * TN; j += kWfBlock) t[j] = 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned n = min(*count, cap);
+    const unsigned e0 = (unsigned)(((uint64_t)n * part) / G), e1 = (unsigned)(((uint64_t)n * (part + 1)) / G);
+    for (unsigned b = e0 + wave * 64; b < e1; b += kWfBlock) {
+        const unsigned i = b + lane;
+        CubeEvent e{};
+        bool mine = false;
+        if (i < e1) {
+            e = ev[i];
+            mine = (e.flags >> 2) == c;
+        }
+        uint64_t m = __ballot(mine);
+        while (m) {
+            const int src = __builtin_ctzll(m);
+            m &= m - 1;
+            const int center = __builtin_amdgcn_readlane(e.center, src);
+            const float s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.s), src));
+            const float Are = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.are), src));
+            const float Aim = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.aim), src));
+            const int cc = __builtin_amdgcn_readlane(e.cc, src);
+            const float sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.sc), src));
+            const int flags = __builtin_amdgcn_readlane(e.flags, src);
+            const bool delta = flags & 1, deltaC = flags & 2;
+            const int ns = delta ? 1 : 33, nc = deltaC ? 1 : 33;
+            for (int tt = lane; tt < ns * nc; tt += 64) {
+                const int l = deltaC ? 0 : tt / ns - 16;
+                const int k = delta ? 0 : tt % ns - 16;
+                const int rel = wrap_bin(cc + l, M) - m0;
+                if (rel < 0 || rel >= mt) continue;
+                float coef = 1.0f;
+                if (!deltaC) coef = 1.0f / (sc - (float)l);
+                if (!delta) coef = coef * (1.0f / (s - (float)k));
+                const int j = rel * N + wrap_bin(center + k, N);
+                atomicAdd(&t[j], (double)(Are * coef));
+                atomicAdd(&t[TN + j], (double)(Aim * coef));
+            }
+        }
+    }
+    __syncthreads();
+    double2 *dst = parts + (int64_t)part * M * N * C;
+    for (int j = threadIdx.x; j < mt * N; j += kWfBlock)
+        dst[((int64_t)(m0 + j / N) * N + j % N) * C + c] = make_double2(t[j], t[TN + j]);
+}
+
+// frame += the G partial cubes (in order), and the event list emptied
+__global__ void k_cube_reduce(double2 *__restrict__ frame, const double2 *__restrict__ parts, int G, int64_t cells,
+                              unsigned *count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = 0u;
+    if (i >= cells) return;
+    double2 s = make_double2(0.0, 0.0);
+    for (int g = 0; g < G; ++g) {
+        const double2 v = parts[(int64_t)g * cells + i];
+        s.x += v.x;
+        s.y += v.y;
+    }
+    double2 f = frame[i];
+    f.x += s.x;
+    f.y += s.y;
+    frame[i] = f;
 }
 
 __global__ void k_wf_zero_counts(WfCounts *c, int n, uint32_t *bins) {
@@ -868,6 +949,29 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
 }
 
 size_t gather_lds_bytes(int N, int C) { return (size_t)N * C * 2 * sizeof(double); }
+
+int cube_tile_chirps(int M, int N, size_t ldsBytes) {
+    const int64_t rows = (int64_t)(ldsBytes / (2 * sizeof(double))) / N;
+    return (int)std::min<int64_t>(M, rows);
+}
+
+hipError_t launch_cube_splat(const CubeEvent *events, unsigned *count, unsigned cap, int M, int N, int C, int Mt,
+                             int G, double2 *parts, double *frame, hipStream_t s) {
+    (void)hipGetLastError();
+    const int tiles = C * ((M + Mt - 1) / Mt);
+    const size_t lds = (size_t)2 * Mt * N * sizeof(double);
+    if (lds > 65536) {
+        const hipError_t e =
+            hipFuncSetAttribute((const void *)k_cube_splat, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_cube_splat, dim3(tiles * G), dim3(kWfBlock), lds, s, events, count, cap, M, N, C, Mt, G,
+                       parts);
+    const int64_t cells = (int64_t)M * N * C;
+    hipLaunchKernelGGL(k_cube_reduce, dim3((unsigned)((cells + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s,
+                       (double2 *)frame, (const double2 *)parts, G, cells, count);
+    return hipGetLastError();
+}
 
 int wavefront_trav_blocks_per_cu(bool count) {
     int n = 0;
