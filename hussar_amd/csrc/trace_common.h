@@ -796,7 +796,8 @@ __device__ inline void tile_flush(const double *tile, double2 *dst, int N, int C
 }
 
 // Wave-cooperative splat of this lane's event (if any) into the frame cube in device memory
-// [M][N][C] with FP64 atomics: up to 33 x 33 taps per event, spread over the wave's lanes.
+// [M][N][C] with FP64 atomics: up to 33 x 33 taps per event, spread over the wave's lanes
+// (tap weights with the hardware reciprocal, 1 ulp).
 __device__ inline void flush_events_cube(bool has, const Event &e, int ch, double *frame, int M, int N, int C,
                                          int lane) {
     uint64_t m = __ballot(has);
@@ -816,8 +817,8 @@ __device__ inline void flush_events_cube(bool has, const Event &e, int ch, doubl
             const int l = deltaC ? 0 : t / ns - 16;
             const int k = delta ? 0 : t % ns - 16;
             float coef = 1.0f;
-            if (!deltaC) coef = 1.0f / (sc - (float)l);
-            if (!delta) coef = coef * (1.0f / (s - (float)k));
+            if (!deltaC) coef = __builtin_amdgcn_rcpf(sc - (float)l);
+            if (!delta) coef = coef * __builtin_amdgcn_rcpf(s - (float)k);
             const int bc = wrap_bin(cc + l, M);
             const int bs = wrap_bin(center + k, N);
             double *dst = frame + 2 * (((int64_t)bc * N + bs) * C + ch);

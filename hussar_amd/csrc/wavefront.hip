@@ -848,8 +848,8 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__rest
                 const int rel = wrap_bin(cc + l, M) - m0;
                 if (rel < 0 || rel >= mt) continue;
                 float coef = 1.0f;
-                if (!deltaC) coef = 1.0f / (sc - (float)l);
-                if (!delta) coef = coef * (1.0f / (s - (float)k));
+                if (!deltaC) coef = __builtin_amdgcn_rcpf(sc - (float)l);
+                if (!delta) coef = coef * __builtin_amdgcn_rcpf(s - (float)k);
                 const int j = rel * N + wrap_bin(center + k, N);
                 atomicAdd(&t[j], (double)(Are * coef));
                 atomicAdd(&t[TN + j], (double)(Aim * coef));
