@@ -87,13 +87,6 @@ struct Node4 {
     int4 ch;
 };
 
-// q * 2^k is exact for q < 256, so one fused multiply-add rounds exactly as the product and
-// sum do separately (the form the builder checked its boxes with).
-__device__ inline float4 dequant4(float o, uint32_t w, float s) {
-    return make_float4(__builtin_fmaf((float)(w & 0xFFu), s, o), __builtin_fmaf((float)((w >> 8) & 0xFFu), s, o),
-                       __builtin_fmaf((float)((w >> 16) & 0xFFu), s, o), __builtin_fmaf((float)(w >> 24), s, o));
-}
-
 // The float layout of the same node (128 B = 8 float4: lo.x[4] | hi.x[4] | lo.y[4] | hi.y[4] |
 // lo.z[4] | hi.z[4] | child[4] | -), used by the any-hit traversals: their packets test every
 // child against several rays, where exact boxes save more visits than the smaller fetch.
@@ -107,26 +100,6 @@ __device__ inline Node4 load_node4f(const float4 *__restrict__ nodes, int32_t n)
     q.lz = p[4];
     q.hz = p[5];
     q.ch = __builtin_bit_cast(int4, p[6]);
-    return q;
-}
-
-__device__ inline Node4 load_node4(const float4 *__restrict__ nodes, int32_t n) {
-    const float4 *p = nodes + 4 * (int64_t)n;
-    const float4 h = p[0];
-    const uint4 a = __builtin_bit_cast(uint4, p[1]);
-    const uint4 b = __builtin_bit_cast(uint4, p[2]);
-    const uint32_t ex = __float_as_uint(h.w);
-    const float sx = __uint_as_float((ex & 0xFFu) << 23);
-    const float sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23);
-    const float sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
-    Node4 q;
-    q.lx = dequant4(h.x, a.x, sx);
-    q.hx = dequant4(h.x, a.y, sx);
-    q.ly = dequant4(h.y, a.z, sy);
-    q.hy = dequant4(h.y, a.w, sy);
-    q.lz = dequant4(h.z, b.x, sz);
-    q.hz = dequant4(h.z, b.y, sz);
-    q.ch = __builtin_bit_cast(int4, p[3]);
     return q;
 }
 
@@ -153,6 +126,70 @@ __device__ inline uint32_t slab4(const Node4 &q, const SlabRay &r, float tb, flo
     return m;
 }
 
+// The closest-hit box test of a quantised node's four children, in the node's frame: a child
+// bound origin + q*2^e is at ray distance t = q*a + b, with a = 2^e * invd (exact: a power of
+// two) and b = (origin - o) * invd, so each plane costs one fused multiply-add instead of a
+// decode, a subtraction and a product, and the ray's direction signs pick each axis' near and
+// far planes (no min/max). b carries two roundings, so t is exact to within one rounding of
+// itself plus 2.00001 u |b| (u = 2^-24): the near planes use b - err and the far planes
+// b + err, err = 2^-20 |b| per axis, and the exit keeps the robust factor 1 + 2 gamma_3 (Ize
+// 2013), which makes the test a superset of the exact one. The real boxes origin + q*2^e lie
+// within one rounding of the decoded boxes the builder checked, and it widened every box by
+// far more. invd is finite (safe_rcp), so no product is NaN.
+struct QuadHit {
+    uint32_t m;     // bit i: child i is entered
+    float te[4];    // entry distances
+    int32_t c[4];   // child refs
+};
+
+__device__ inline QuadHit slab4q(const float4 *__restrict__ nodes, int32_t n, const SlabRay &r, float tb) {
+    const float4 *p = nodes + 4 * (int64_t)n;
+    const float4 h = p[0];
+    const uint4 qa = __builtin_bit_cast(uint4, p[1]);
+    const float4 pb = p[2];
+    const int4 ch = __builtin_bit_cast(int4, p[3]);
+    QuadHit qh;
+    qh.c[0] = ch.x;
+    qh.c[1] = ch.y;
+    qh.c[2] = ch.z;
+    qh.c[3] = ch.w;
+    const uint32_t ex = __float_as_uint(h.w);
+    const float ax = __uint_as_float((ex & 0xFFu) << 23) * r.invd.x;
+    const float ay = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.invd.y;
+    const float az = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.invd.z;
+    const float bx = (h.x - r.o.x) * r.invd.x, by = (h.y - r.o.y) * r.invd.y, bz = (h.z - r.o.z) * r.invd.z;
+    constexpr float kErr = 9.5367431640625e-07f;  // 2^-20
+    const float ex_ = fabsf(bx) * kErr, ey_ = fabsf(by) * kErr, ez_ = fabsf(bz) * kErr;
+    const float bnx = bx - ex_, bfx = bx + ex_, bny = by - ey_, bfy = by + ey_, bnz = bz - ez_, bfz = bz + ez_;
+    // near and far plane bytes per axis: the lower bounds are near for a positive direction
+    const bool sx = r.invd.x >= 0.0f, sy = r.invd.y >= 0.0f, sz = r.invd.z >= 0.0f;
+    const uint32_t qz0 = __float_as_uint(pb.x), qz1 = __float_as_uint(pb.y);
+    const uint32_t nxw = sx ? qa.x : qa.y, fxw = sx ? qa.y : qa.x;
+    const uint32_t nyw = sy ? qa.z : qa.w, fyw = sy ? qa.w : qa.z;
+    const uint32_t nzw = sz ? qz0 : qz1, fzw = sz ? qz1 : qz0;
+    qh.m = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float nx = __builtin_fmaf((float)((nxw >> (8 * i)) & 0xFFu), ax, bnx);
+        const float fx = __builtin_fmaf((float)((fxw >> (8 * i)) & 0xFFu), ax, bfx);
+        const float ny = __builtin_fmaf((float)((nyw >> (8 * i)) & 0xFFu), ay, bny);
+        const float fy = __builtin_fmaf((float)((fyw >> (8 * i)) & 0xFFu), ay, bfy);
+        const float nz = __builtin_fmaf((float)((nzw >> (8 * i)) & 0xFFu), az, bnz);
+        const float fz = __builtin_fmaf((float)((fzw >> (8 * i)) & 0xFFu), az, bfz);
+        const float t0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tnear));
+        const float t1 = fminf(fminf(fx, fy), fz) * 1.0000004f;
+        qh.te[i] = t0;
+        if (t0 <= t1 && t0 <= tb && qh.c[i] != kNoChild) qh.m |= 1u << i;
+    }
+    return qh;
+}
+
+// Reciprocal direction for the slab tests, finite: a zero (or subnormal) component becomes
+// +-2^100, which keeps every box test conservative (the products stay finite).
+__device__ inline float safe_rcp(float v) {
+    return fabsf(v) > 7.888609052210118e-31f ? 1.0f / v : copysignf(1.2676506002282294e30f, v);
+}
+
 __device__ inline int32_t child_ref(const int4 &c, int i) {
     return i == 0 ? c.x : (i == 1 ? c.y : (i == 2 ? c.z : c.w));
 }
@@ -171,12 +208,16 @@ __device__ inline void cswap(float &ka, int32_t &va, float &kb, int32_t &vb) {
 // Closest-hit order from a visited inner node with hit mask m != 0: the hit children sorted by
 // entry distance (a five-comparator network over four keys, misses keyed +inf), the nearest
 // returned as the next node, the others pushed farthest first so that they pop nearest first.
-__device__ inline int32_t order_children(const Node4 &q, uint32_t m, const float te[4], Stack &st) {
-    if ((m & (m - 1)) == 0) return child_ref(q.ch, __builtin_ctz(m));
+__device__ inline int32_t order_children(const QuadHit &q, Stack &st) {
+    const uint32_t m = q.m;
+    if ((m & (m - 1)) == 0) {
+        const int i = __builtin_ctz(m);
+        return i == 0 ? q.c[0] : (i == 1 ? q.c[1] : (i == 2 ? q.c[2] : q.c[3]));
+    }
     const float inf = __builtin_inff();
-    float k0 = (m & 1u) ? te[0] : inf, k1 = (m & 2u) ? te[1] : inf;
-    float k2 = (m & 4u) ? te[2] : inf, k3 = (m & 8u) ? te[3] : inf;
-    int32_t v0 = q.ch.x, v1 = q.ch.y, v2 = q.ch.z, v3 = q.ch.w;
+    float k0 = (m & 1u) ? q.te[0] : inf, k1 = (m & 2u) ? q.te[1] : inf;
+    float k2 = (m & 4u) ? q.te[2] : inf, k3 = (m & 8u) ? q.te[3] : inf;
+    int32_t v0 = q.c[0], v1 = q.c[1], v2 = q.c[2], v3 = q.c[3];
     cswap(k0, v0, k1, v1);
     cswap(k2, v2, k3, v3);
     cswap(k0, v0, k2, v2);
@@ -204,7 +245,7 @@ __device__ inline int32_t any_children(const Node4 &q, uint32_t m, Stack &st) {
 template <bool kCount>
 __device__ inline int32_t closest_hit(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
                                       f3 d, Stack &st, float &tout, int32_t &slot, uint32_t &cn, uint32_t &ct) {
-    SlabRay r{o, {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}, kEps};
+    SlabRay r{o, {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)}, kEps};
     float best = __builtin_inff();
     int32_t best_id = -1, best_slot = -1;
     st.sp = 0;
@@ -212,11 +253,9 @@ __device__ inline int32_t closest_hit(const float4 *__restrict__ nodes, const fl
     while (true) {
         if (node >= 0) {
             if (kCount) ++cn;
-            const Node4 q = load_node4(nodes, node);
-            float te[4];
-            const uint32_t m = slab4(q, r, best, te);
-            if (m) {
-                node = order_children(q, m, te, st);
+            const QuadHit qh = slab4q(nodes, node, r, best);
+            if (qh.m) {
+                node = order_children(qh, st);
                 continue;
             }
         } else {
@@ -593,7 +632,7 @@ struct Trav {
 __device__ inline void trav_begin(Trav &t, f3 o, f3 d, float tfar, Stack &st) {
     t.o = o;
     t.d = d;
-    t.invd = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    t.invd = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
     t.tfar = tfar;
     t.node = 0;
     t.best_id = -1;
@@ -608,11 +647,18 @@ __device__ inline void trav_run(const float4 *__restrict__ nodes, const float4 *
     for (int s = 0; s < steps && t.node != kDone; ++s) {
         if (t.node >= 0) {
             if (kCount) ++cn;
-            const Node4 q = kAny ? load_node4f(nodes, t.node) : load_node4(nodes, t.node);
-            float te[4];
-            const uint32_t m = slab4(q, r, t.tfar, te);
-            if (m) t.node = kAny ? any_children(q, m, st) : order_children(q, m, te, st);
-            else t.node = st.sp ? st.pop() : kDone;
+            uint32_t m;
+            if (kAny) {
+                const Node4 q = load_node4f(nodes, t.node);
+                float te[4];
+                m = slab4(q, r, t.tfar, te);
+                if (m) t.node = any_children(q, m, st);
+            } else {
+                const QuadHit qh = slab4q(nodes, t.node, r, t.tfar);
+                m = qh.m;
+                if (m) t.node = order_children(qh, st);
+            }
+            if (!m) t.node = st.sp ? st.pop() : kDone;
         } else {
             int first, count;
             leaf_decode(t.node, first, count);

@@ -381,8 +381,12 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
 
 // ------------------------------------------------------------------------ traversal
 // Persistent: each wave keeps its lanes on rays, refilling idle lanes from the queue.
+#ifndef HSR_TRAV_OCC
+#define HSR_TRAV_OCC 8
+#endif
+// resident waves per SIMD the register allocator must allow (workgroups of 4 waves per CU)
 template <bool kAny, bool kCount>
-__global__ __launch_bounds__(kWfBlock) void k_wf_trav(TraceArgs a, WfArgs w, int depth) {
+__global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a, WfArgs w, int depth) {
     __shared__ int32_t lds_stack[kStackShort * kWfBlock];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
