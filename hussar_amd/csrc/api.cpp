@@ -434,6 +434,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         a.gsP0 = guide->d_sP0;
         a.gtChild = guide->d_tChild;
         a.gtStat = guide->d_tStat;
+        a.gtStride = guide->capacity;
         a.uniformProb = guide->host.uniformProb;
         a.secondMoment = guide->host.secondMoment;
         a.guideOn = 1;
@@ -804,14 +805,19 @@ static hsr_status guide_upload(hsr_guide *g) {
     HSR_HIP(hipMemcpy(g->d_sChild, s.child.data(), s.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HSR_HIP(hipMemcpy(g->d_sP0, s.p0.data(), s.size() * sizeof(float), hipMemcpyHostToDevice));
     HSR_HIP(hipMemcpy(g->d_tChild, t.child.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HSR_HIP(hipMemset(g->d_tStat, 0, (size_t)hsr::kStatCopies * g->capacity * sizeof(double)));
     HSR_HIP(hipMemcpy(g->d_tStat, t.stat.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
     g->dirty = false;
     return HSR_OK;
 }
 
+// the training statistics: the kStatCopies device copies folded into the first, in order
 static hsr_status guide_download_stats(hsr_guide *g) {
     hsr::GuideTree &t = g->host.training;
-    HSR_HIP(hipMemcpy(t.stat.data(), g->d_tStat, t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    hipStream_t s = g->backend->stream;
+    HSR_HIP(hsr::launch_stat_fold(g->d_tStat, g->capacity, (int64_t)t.size(), s));
+    HSR_HIP(hipMemcpyAsync(t.stat.data(), g->d_tStat, t.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HSR_HIP(hipStreamSynchronize(s));
     return HSR_OK;
 }
 
@@ -831,7 +837,7 @@ hsr_status hsr_guide_create(hsr_backend *b, float uniformProb, float splitThresh
     if ((s = dalloc(&g->d_sChild, g->capacity)) != HSR_OK) return s;
     if ((s = dalloc(&g->d_sP0, g->capacity)) != HSR_OK) return s;
     if ((s = dalloc(&g->d_tChild, g->capacity)) != HSR_OK) return s;
-    if ((s = dalloc(&g->d_tStat, g->capacity)) != HSR_OK) return s;
+    if ((s = dalloc(&g->d_tStat, (size_t)hsr::kStatCopies * g->capacity)) != HSR_OK) return s;
     g->host.reset();
     if ((s = guide_upload(g.get())) != HSR_OK) return s;
     *out = g.release();

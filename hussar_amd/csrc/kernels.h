@@ -25,6 +25,11 @@ struct CubeEvent {
 };
 constexpr unsigned kCubeEvents = 1u << 22;  // capacity of a backend's event list (128 MiB)
 
+// Copies of the guiding tree's training statistics: records of one leaf from every workgroup
+// would otherwise queue on one address (device atomics on one address are serialised); the
+// copies are summed when the statistics are read back.
+constexpr int kStatCopies = 8;
+
 struct DevAntenna {
     float pos[3];
     float rot[9];
@@ -59,7 +64,8 @@ struct TraceArgs {
     const int32_t *gsChild;
     const float *gsP0;
     const int32_t *gtChild;
-    double *gtStat;
+    double *gtStat;     // kStatCopies copies, gtStride apart: workgroup g adds into copy g % kStatCopies
+    int64_t gtStride;
     float uniformProb;
     int32_t secondMoment;
     int32_t guideOn;
@@ -155,6 +161,8 @@ size_t gather_lds_bytes(int N, int C);
 hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum, double *stage_comp, int S,
                          double *frame_sum, double *frame_comp, hipStream_t stream);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t stream);
+// Folds the kStatCopies copies of n training statistics (stride apart) into the first.
+hipError_t launch_stat_fold(double *stat, int64_t stride, int64_t n, hipStream_t stream);
 // Cube frames: chirps per LDS tile of one channel (0: a chirp of N samples does not fit).
 int cube_tile_chirps(int M, int N, size_t ldsBytes);
 // Adds the listed events (count on the device) into the frame cube [M][N][C] through LDS

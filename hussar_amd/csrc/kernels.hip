@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                 pend = true;
             }
         }
-        if (a.doGuiding && a.guideOn && !a.isFinal) record_stats(pend, leaf, val, a.gtStat, lane);
+        if (a.doGuiding && a.guideOn && !a.isFinal) record_stats(pend, leaf, val, a.gtStat + (blockIdx.x % kStatCopies) * a.gtStride, lane);
     }
 
     // ---- store the workgroup's frame tile
@@ -217,6 +217,26 @@ hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum
                        stage_sum, stage_comp);
     hipLaunchKernelGGL(k_reduce_final, dim3((M + bs - 1) / bs), dim3(bs), 0, stream, stage_sum, stage_comp, S, M,
                        frame_sum, frame_comp);
+    return hipGetLastError();
+}
+
+// stat[i] += stat[k * stride + i] for the copies k = 1 .. kStatCopies-1, in order; the
+// copies are cleared
+__global__ void k_stat_fold(double *stat, int64_t stride, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = stat[i];
+    for (int k = 1; k < kStatCopies; ++k) {
+        s += stat[k * stride + i];
+        stat[k * stride + i] = 0.0;
+    }
+    stat[i] = s;
+}
+
+hipError_t launch_stat_fold(double *stat, int64_t stride, int64_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_stat_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, stat, stride, n);
     return hipGetLastError();
 }
 

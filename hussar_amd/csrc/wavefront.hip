@@ -243,7 +243,7 @@ __device__ inline void finish_record(const TraceArgs &a, bool finishing, const P
         val = (a.secondMoment ? dd * dd : dd) * (double)weight;
         pend = true;
     }
-    record_stats(pend, leaf, val, a.gtStat, lane);
+    record_stats(pend, leaf, val, a.gtStat + (blockIdx.x % kStatCopies) * a.gtStride, lane);
 }
 
 // The extension ray of a path that goes on, and the flag of a path that ends after its NEE.
