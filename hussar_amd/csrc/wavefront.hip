@@ -447,7 +447,10 @@ struct Packet {
     float tfar[kPacket];
     uint32_t alive;  // rays not yet occluded
     int32_t node;
+    bool same;       // all rays share their direction signs, given by sgn (bit k: axis k >= 0)
+    uint32_t sgn;
 };
+
 
 __device__ inline bool slab_diff(float dlx, float dhx, float dly, float dhy, float dlz, float dhz, f3 invd,
                                  float tfar, float &tentry) {
@@ -487,15 +490,34 @@ __device__ inline void packet_run(const float4 *__restrict__ nodes, const float4
                 dl[i][5] = hz - pk.o.z;
             }
             uint32_t m = 0;
+            if (pk.same) {
+                // the direction signs pick each axis' near and far face for every ray: the
+                // same products as the min/max form, without the min/max
+                const bool sx = pk.sgn & 1u, sy = pk.sgn & 2u, sz = pk.sgn & 4u;
 #pragma unroll
-            for (int r = 0; r < kPacket; ++r) {
-                if (pk.alive & (1u << r)) {
+                for (int i = 0; i < 4; ++i) {
+                    const float nx = sx ? dl[i][0] : dl[i][1], fx = sx ? dl[i][1] : dl[i][0];
+                    const float ny = sy ? dl[i][2] : dl[i][3], fy = sy ? dl[i][3] : dl[i][2];
+                    const float nz = sz ? dl[i][4] : dl[i][5], fz = sz ? dl[i][5] : dl[i][4];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        float e;
-                        if (slab_diff(dl[i][0], dl[i][1], dl[i][2], dl[i][3], dl[i][4], dl[i][5], pk.invd[r],
-                                      pk.tfar[r], e))
-                            m |= 1u << i;
+                    for (int r = 0; r < kPacket; ++r) {
+                        const f3 iv = pk.invd[r];
+                        const float t0 = fmaxf(fmaxf(nx * iv.x, ny * iv.y), fmaxf(nz * iv.z, kEps));
+                        const float t1 = fminf(fminf(fx * iv.x, fy * iv.y), fz * iv.z) * 1.0000004f;
+                        if ((pk.alive & (1u << r)) && t0 <= t1 && t0 <= pk.tfar[r]) m |= 1u << i;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < kPacket; ++r) {
+                    if (pk.alive & (1u << r)) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            float e;
+                            if (slab_diff(dl[i][0], dl[i][1], dl[i][2], dl[i][3], dl[i][4], dl[i][5], pk.invd[r],
+                                          pk.tfar[r], e))
+                                m |= 1u << i;
+                        }
                     }
                 }
             }
@@ -557,6 +579,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
                 rays = min(kPacket, nrx - c0);
                 slot0 = i * a.C + c0;
                 pk.alive = 0;
+                uint32_t sg[kPacket];
 #pragma unroll
                 for (int r = 0; r < kPacket; ++r) {
                     if (r < rays) {
@@ -564,7 +587,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
                         const float4 d = w.s_d[slot0 + r];
                         pk.o = {o.x, o.y, o.z};
                         pk.d[r] = {d.x, d.y, d.z};
-                        pk.invd[r] = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+                        pk.invd[r] = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
                         pk.tfar[r] = o.w;
                         pk.alive |= 1u << r;
                     } else {
@@ -572,7 +595,14 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
                         pk.invd[r] = {0.0f, 0.0f, 1.0f};
                         pk.tfar[r] = 0.0f;
                     }
+                    sg[r] = (pk.invd[r].x >= 0.0f ? 1u : 0u) | (pk.invd[r].y >= 0.0f ? 2u : 0u) |
+                            (pk.invd[r].z >= 0.0f ? 4u : 0u);
                 }
+                pk.sgn = sg[0];
+                pk.same = true;
+#pragma unroll
+                for (int r = 1; r < kPacket; ++r)
+                    if (r < rays && sg[r] != sg[0]) pk.same = false;
                 pk.node = 0;
                 st.sp = 0;
                 has = true;
