@@ -113,9 +113,12 @@ struct WfCounts {
 //  sf: 15 float fields x P   (H 6, guidingWeight 2, x0, x1, pdf, freq, vertex velocity 3)
 //  si: 3 int fields x P      (depth, flags: bit 0 = the path ends after its pending NEE,
 //                            the path's index within the batch)
-//  shadow slots [P * C]: ray (o.xyz, tfar | d.xyz), NEE payload (tn, v, plane-filter guiding
-//  term), status bits (1 set-up, 2 needs a visibility query, 4 add v to the guiding weight,
-//  8 splat, 16 add the plane-filter term) and the visibility (1 occluded, 2 visible).
+//  ray_o / ray_d [P]: the path's current vertex rounded to float (origin of its shadow and
+//  extension rays) and its extension direction
+//  shadow slots [P * C]: ray direction and extent (d.xyz, tfar), NEE payload (tn, v,
+//  plane-filter guiding term), status bits (1 set-up, 2 needs a visibility query, 4 add v to
+//  the guiding weight, 8 splat, 16 add the plane-filter term) and the visibility (1 occluded,
+//  2 visible).
 struct WfArgs {
     int64_t P;
     int64_t base;   // index (relative to the run's first path) of slot 0
@@ -125,7 +128,7 @@ struct WfArgs {
     int32_t *si;
     float4 *ray_o, *ray_d;
     int2 *hit;
-    float4 *s_o, *s_d;
+    float4 *s_d;
     double *s_tn;
     double *s_dv;  // radial velocity of the contribution (motion only)
     float2 *s_v, *s_g1;

@@ -246,14 +246,19 @@ __device__ inline void finish_record(const TraceArgs &a, bool finishing, const P
     record_stats(pend, leaf, val, a.gtStat + (blockIdx.x % kStatCopies) * a.gtStride, lane);
 }
 
-// The extension ray of a path that goes on, and the flag of a path that ends after its NEE.
+// A live path's vertex rounded to float (the origin of its shadow rays and of its extension
+// ray), the extension direction of a path that goes on, and the flag of a path that ends
+// after its NEE.
 __device__ inline void write_extend(const WfArgs &w, bool alive, bool ends, int64_t i, const PathState &p) {
-    if (alive && !ends) {
-        const f3 o = cast3<float>(p.o), d = cast3<float>(p.d);
+    if (alive) {
+        const f3 o = cast3<float>(p.o);
         w.ray_o[i] = make_float4(o.x, o.y, o.z, 0.0f);
+        w.si[I_FLAGS * w.P + i] = ends ? 1 : 0;
+    }
+    if (alive && !ends) {
+        const f3 d = cast3<float>(p.d);
         w.ray_d[i] = make_float4(d.x, d.y, d.z, 0.0f);
     }
-    if (alive) w.si[I_FLAGS * w.P + i] = ends ? 1 : 0;
 }
 
 // ------------------------------------------------------------------------ coherence order
@@ -380,12 +385,13 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
 }
 
 // ------------------------------------------------------------------------ traversal
-// Persistent: each wave keeps its lanes on rays, refilling idle lanes from the queue.
+// Closest-hit traversal of the extension queue. Persistent: each wave keeps its lanes on
+// rays, refilling idle lanes from its claim on the queue.
 #ifndef HSR_TRAV_OCC
 #define HSR_TRAV_OCC 8
 #endif
 // resident waves per SIMD the register allocator must allow (workgroups of 4 waves per CU)
-template <bool kAny, bool kCount>
+template <bool kCount>
 __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a, WfArgs w, int depth) {
     __shared__ int32_t lds_stack[kStackShort * kWfBlock];
     const int tid = threadIdx.x;
@@ -396,9 +402,9 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
     st.lds = lds_stack + tid;
     st.sp = 0;
     WfCounts *cnt = &w.cnt[depth];
-    const unsigned int n = kAny ? cnt->n_shd.v : cnt->n_ext.v;
-    WfCounter *heads = kAny ? cnt->head_shd : cnt->head_ext;
-    const int32_t *queue = kAny ? w.q_shd : ((depth & 1) ? w.q_ext1 : w.q_ext0);
+    const unsigned int n = cnt->n_ext.v;
+    WfCounter *heads = cnt->head_ext;
+    const int32_t *queue = (depth & 1) ? w.q_ext1 : w.q_ext0;
     uint32_t cn = 0, ct = 0, cr = 0;
     Trav t;
     int32_t item = -1;
@@ -410,9 +416,9 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
             const unsigned int my = claim_items(heads, n, idle, lane, cl);
             if (my != ~0u) {
                 item = queue[my];
-                const float4 o = kAny ? w.s_o[item] : w.ray_o[item];
-                const float4 d = kAny ? w.s_d[item] : w.ray_d[item];
-                trav_begin(t, {o.x, o.y, o.z}, {d.x, d.y, d.z}, kAny ? o.w : __builtin_inff(), st);
+                const float4 o = w.ray_o[item];
+                const float4 d = w.ray_d[item];
+                trav_begin(t, {o.x, o.y, o.z}, {d.x, d.y, d.z}, __builtin_inff(), st);
                 has = true;
                 if (kCount) ++cr;
             }
@@ -422,10 +428,9 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
             continue;
         }
         if (has) {
-            trav_run<kAny, kCount>(kAny ? a.nodesF : a.nodes, a.tris, t, st, kTravSteps, cn, ct);
+            trav_run<false, kCount>(a.nodes, a.tris, t, st, kTravSteps, cn, ct);
             if (t.node == kDone) {
-                if (kAny) w.s_vis[item] = t.best_id >= 0 ? 1 : 2;
-                else w.hit[item] = make_int2(t.best_slot, __float_as_int(t.tfar));
+                w.hit[item] = make_int2(t.best_slot, __float_as_int(t.tfar));
                 has = false;
             }
         }
@@ -433,7 +438,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
     if (kCount) {
         wave_count(&a.counters[0], cn, lane);
         wave_count(&a.counters[1], ct, lane);
-        wave_count(&a.counters[kAny ? 3 : 2], cr, lane);
+        wave_count(&a.counters[2], cr, lane);
     }
 }
 
@@ -580,15 +585,15 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
                 slot0 = i * a.C + c0;
                 pk.alive = 0;
                 uint32_t sg[kPacket];
+                const float4 o = w.ray_o[i];
+                pk.o = {o.x, o.y, o.z};
 #pragma unroll
                 for (int r = 0; r < kPacket; ++r) {
                     if (r < rays) {
-                        const float4 o = w.s_o[slot0 + r];
                         const float4 d = w.s_d[slot0 + r];
-                        pk.o = {o.x, o.y, o.z};
                         pk.d[r] = {d.x, d.y, d.z};
                         pk.invd[r] = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
-                        pk.tfar[r] = o.w;
+                        pk.tfar[r] = d.w;
                         pk.alive |= 1u << r;
                     } else {
                         pk.d[r] = {0.0f, 0.0f, 1.0f};
@@ -630,18 +635,18 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
 
 // ------------------------------------------------------------------------ hit
 // One shadow slot of the NEE set-up ns in slot j; returns whether it needs a visibility query.
+// The ray's origin is the path's vertex in ray_o (write_extend); s_g1 is written only when
+// set (S_GW1) and s_vis by the shadow traversal of the slots it tests (S_TEST).
 __device__ inline bool write_slot(const TraceArgs &a, const WfArgs &w, int64_t j, const NeeSetup &ns) {
     if (a.motion) w.s_dv[j] = ns.dv;
-    w.s_o[j] = make_float4(ns.o.x, ns.o.y, ns.o.z, ns.tfar);
-    w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, 0.0f);
+    w.s_d[j] = make_float4(ns.d.x, ns.d.y, ns.d.z, ns.tfar);
     w.s_tn[j] = ns.tn;
     w.s_v[j] = make_float2(ns.v.re, ns.v.im);
-    w.s_g1[j] = make_float2(ns.gw1.re, ns.gw1.im);
+    const bool g1 = ns.gw1.re != 0.0f || ns.gw1.im != 0.0f;
+    if (g1) w.s_g1[j] = make_float2(ns.gw1.re, ns.gw1.im);
     const bool test = !a.poDiffraction;
-    const int st = S_SETUP | (test ? S_TEST : 0) | (ns.gwAdd ? S_GW : 0) | (ns.splat ? S_SPLAT : 0) |
-                   ((ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) ? S_GW1 : 0);
+    const int st = S_SETUP | (test ? S_TEST : 0) | (ns.gwAdd ? S_GW : 0) | (ns.splat ? S_SPLAT : 0) | (g1 ? S_GW1 : 0);
     w.s_st[j] = (int8_t)st;
-    w.s_vis[j] = 0;
     return test;
 }
 
@@ -939,7 +944,7 @@ size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
     b += 2 * align256(sizeof(float4) * P);
     b += align256(sizeof(int2) * P);
     const int64_t S = P * C;
-    b += 2 * align256(sizeof(float4) * S);
+    b += align256(sizeof(float4) * S);
     b += 2 * align256(sizeof(double) * S);
     b += 2 * align256(sizeof(float2) * S);
     b += 2 * align256(sizeof(int8_t) * S);
@@ -964,7 +969,6 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.ray_o = (float4 *)take(sizeof(float4) * P);
     w.ray_d = (float4 *)take(sizeof(float4) * P);
     w.hit = (int2 *)take(sizeof(int2) * P);
-    w.s_o = (float4 *)take(sizeof(float4) * S);
     w.s_d = (float4 *)take(sizeof(float4) * S);
     w.s_tn = (double *)take(sizeof(double) * S);
     w.s_dv = (double *)take(sizeof(double) * S);
@@ -1009,8 +1013,8 @@ hipError_t launch_cube_splat(const CubeEvent *events, unsigned *count, unsigned 
 
 int wavefront_trav_blocks_per_cu(bool count) {
     int n = 0;
-    hipError_t e = count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trav<false, true>, kWfBlock, 0)
-                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trav<false, false>, kWfBlock, 0);
+    hipError_t e = count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trav<true>, kWfBlock, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wf_trav<false>, kWfBlock, 0);
     return e == hipSuccess && n > 0 ? n : 1;
 }
 
@@ -1062,9 +1066,9 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
         if (k < maxDepth) {
             (void)hipEventRecord(evClosest[2 * *nClosest], s);
             if (count)
-                hipLaunchKernelGGL((k_wf_trav<false, true>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
+                hipLaunchKernelGGL((k_wf_trav<true>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
             else
-                hipLaunchKernelGGL((k_wf_trav<false, false>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
+                hipLaunchKernelGGL((k_wf_trav<false>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
             (void)hipEventRecord(evClosest[2 * *nClosest + 1], s);
             ++*nClosest;
             hipLaunchKernelGGL(k_wf_shade, dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
