@@ -110,6 +110,7 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     b->ldsPerBlock = (int)prop.sharedMemPerBlock;
     b->ldsPerCU = (int)prop.maxSharedMemoryPerMultiProcessor;
     HSR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HSR_HIP(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
     for (auto &e : b->ev) HSR_HIP(hipEventCreate(&e));
 
     hsr::FlatBVH bvh;
@@ -140,6 +141,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
     dfree(b->d_nodes);
     dfree(b->d_nodesF);
     dfree(b->d_tris);
@@ -162,6 +164,9 @@ void hsr_hip_destroy(hsr_backend *b) {
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : b->evTrav) (void)hipEventDestroy(e);
+    for (auto &e : b->evFork) (void)hipEventDestroy(e);
+    for (auto &e : b->evJoin) (void)hipEventDestroy(e);
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -564,8 +569,16 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         }
         int nClosest = 0, nShadow = 0;
         hipEvent_t *evC = b->evTrav.data(), *evS = b->evTrav.data() + nev / 2;
+        while ((int)b->evFork.size() < knobs->maxDepth + 1) {
+            hipEvent_t f, j;
+            HSR_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
+            HSR_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+            b->evFork.push_back(f);
+            b->evJoin.push_back(j);
+        }
         HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
-                                      b->stream, evC, &nClosest, evS, &nShadow));
+                                      b->stream, b->stream2, b->evFork.data(), b->evJoin.data(), evC, &nClosest, evS,
+                                      &nShadow));
         if (a.events)
             HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
                                            (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));

@@ -112,6 +112,10 @@ struct hsr_backend {
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> evTrav;  // pairs around traversal launches (wavefront)
+    // the closest-hit traversal of a depth runs on stream2, beside that depth's shadow and
+    // gather stages on stream (default priority both); evFork/evJoin order them per depth
+    hipStream_t stream2 = nullptr;
+    std::vector<hipEvent_t> evFork, evJoin;
     hsr_run_stats stats{};
 };
 

@@ -153,11 +153,14 @@ size_t wavefront_bytes(int64_t P, int C, int maxDepth);
 void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth);
 // One batch: generation, then per depth shadow / gather / extend / shade, with all queue
 // sizes on the device (no host synchronisation). partials: [gatherGrid][N*C], accumulated.
-// evClosest / evShadow: room for 2 * (maxDepth + 1) events each; the launcher records a pair
-// around every closest-hit / shadow traversal launch and returns the pair counts.
+// The closest-hit traversal of each depth runs on stream2 beside the depth's shadow and
+// gather stages on stream, ordered by evFork/evJoin (maxDepth + 1 each); the batch ends
+// on stream. evClosest / evShadow: room for 2 * (maxDepth + 1) events each; the launcher
+// records a pair around every closest-hit / shadow traversal launch and returns the counts.
 hipError_t launch_wavefront(const TraceArgs &args, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
-                            double2 *partials, bool count, hipStream_t stream, hipEvent_t *evClosest,
-                            int *nClosest, hipEvent_t *evShadow, int *nShadow);
+                            double2 *partials, bool count, hipStream_t stream, hipStream_t stream2,
+                            hipEvent_t *evFork, hipEvent_t *evJoin, hipEvent_t *evClosest, int *nClosest,
+                            hipEvent_t *evShadow, int *nShadow);
 int wavefront_trav_blocks_per_cu(bool count);
 size_t gather_lds_bytes(int N, int C);
 // partials [W][M] doubles -> stage [S][M] (sum, comp) -> frame (sum, comp) += Kahan

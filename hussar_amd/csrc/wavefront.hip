@@ -1019,8 +1019,9 @@ int wavefront_trav_blocks_per_cu(bool count) {
 }
 
 hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
-                            double2 *partials, bool count, hipStream_t s, hipEvent_t *evClosest, int *nClosest,
-                            hipEvent_t *evShadow, int *nShadow) {
+                            double2 *partials, bool count, hipStream_t s, hipStream_t s2, hipEvent_t *evFork,
+                            hipEvent_t *evJoin, hipEvent_t *evClosest, int *nClosest, hipEvent_t *evShadow,
+                            int *nShadow) {
     *nClosest = 0;
     *nShadow = 0;
     (void)hipGetLastError();
@@ -1051,6 +1052,21 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     }
     if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee0, dim3(gNee), dim3(kWfBlock), 0, s, a, w);
     for (int k = 0; k <= maxDepth; ++k) {
+        // the depth's closest-hit traversal on s2, beside its shadow and gather stages: it
+        // reads the extension queue and rays of depth k and writes only the hits, which the
+        // shading stage reads after the join
+        if (k < maxDepth) {
+            (void)hipEventRecord(evFork[k], s);
+            (void)hipStreamWaitEvent(s2, evFork[k], 0);
+            (void)hipEventRecord(evClosest[2 * *nClosest], s2);
+            if (count)
+                hipLaunchKernelGGL((k_wf_trav<true>), dim3(gClosest), dim3(kWfBlock), 0, s2, a, w, k);
+            else
+                hipLaunchKernelGGL((k_wf_trav<false>), dim3(gClosest), dim3(kWfBlock), 0, s2, a, w, k);
+            (void)hipEventRecord(evClosest[2 * *nClosest + 1], s2);
+            ++*nClosest;
+            (void)hipEventRecord(evJoin[k], s2);
+        }
         if (k >= 1 || !a.onlyIndirect) {
             if (!a.poDiffraction) {
                 (void)hipEventRecord(evShadow[2 * *nShadow], s);
@@ -1064,13 +1080,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
             hipLaunchKernelGGL(k_wf_gather, dim3(gridGather), dim3(kWfBlock), glds, s, a, w, k, partials, count);
         }
         if (k < maxDepth) {
-            (void)hipEventRecord(evClosest[2 * *nClosest], s);
-            if (count)
-                hipLaunchKernelGGL((k_wf_trav<true>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
-            else
-                hipLaunchKernelGGL((k_wf_trav<false>), dim3(gClosest), dim3(kWfBlock), 0, s, a, w, k);
-            (void)hipEventRecord(evClosest[2 * *nClosest + 1], s);
-            ++*nClosest;
+            (void)hipStreamWaitEvent(s, evJoin[k], 0);
             hipLaunchKernelGGL(k_wf_shade, dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
         }
     }
