@@ -577,8 +577,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             b->evJoin.push_back(j);
         }
         HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
-                                      b->stream, b->stream2, b->evFork.data(), b->evJoin.data(), evC, &nClosest, evS,
-                                      &nShadow));
+                                      b->stream, b->useStream2 ? b->stream2 : b->stream, b->evFork.data(),
+                                      b->evJoin.data(), evC, &nClosest, evS, &nShadow));
         if (a.events)
             HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
                                            (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));

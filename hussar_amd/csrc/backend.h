@@ -116,6 +116,9 @@ struct hsr_backend {
     // gather stages on stream (default priority both); evFork/evJoin order them per depth
     hipStream_t stream2 = nullptr;
     std::vector<hipEvent_t> evFork, evJoin;
+    // off for the slots of a server with more than two: concurrent slots already share the
+    // device, and a second stream per slot would make more streams than hardware queues
+    bool useStream2 = true;
     hsr_run_stats stats{};
 };
 

@@ -212,6 +212,9 @@ hsr_status hsr_server_create(const hsr_server_config *config, hsr_server **out) 
             hsr_server_destroy(s.release());
             return st;
         }
+        // a slot's second stream only while every stream can have a hardware queue of its own
+        // (HIP gives a process 4 by default); beyond that the slots' streams would share queues
+        sl.b->useStream2 = 2 * c.slots <= 4;
         if ((st = hsr::backend_reserve(sl.b, c.pathsPerBatch, HSR_MAX_CHANNELS, c.maxDepth,
                                        s->frameReserveDoubles)) != HSR_OK ||
             (st = hsr_guide_create(sl.b, 0.1f, 0.005f, 1, &sl.g)) != HSR_OK) {
