@@ -437,6 +437,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     if (guide) {
         a.gsChild = guide->d_sChild;
         a.gsP0 = guide->d_sP0;
+        a.gsSize = guide->host.sampling.size();
         a.gtChild = guide->d_tChild;
         a.gtStat = guide->d_tStat;
         a.gtStride = guide->capacity;

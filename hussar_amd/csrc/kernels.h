@@ -63,6 +63,7 @@ struct TraceArgs {
     // guiding: sampling tree (child, p0) and training tree (child, stat)
     const int32_t *gsChild;
     const float *gsP0;
+    int32_t gsSize;     // nodes of the sampling tree
     const int32_t *gtChild;
     double *gtStat;     // kStatCopies copies, gtStride apart: workgroup g adds into copy g % kStatCopies
     int64_t gtStride;

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
             const int64_t mine = next + (int64_t)__popcll(below);
             if (!active && mine < end) {
                 p.local = mine;
-                generate(a, (uint64_t)(a.offset + mine), p);
+                generate(a, (uint64_t)(a.offset + mine), p, a.gsChild, a.gsP0);
                 active = true;
             }
             next = min(end, next + (int64_t)__popcll(idle));

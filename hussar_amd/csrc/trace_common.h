@@ -401,18 +401,20 @@ __device__ inline int32_t tree_leaf(const int32_t *__restrict__ child, float x0,
     return node;
 }
 
-// GuidingWrapper::sample (path.h:53-82)
-__device__ inline float guide_sample(const TraceArgs &a, float &x0, float &x1) {
+// GuidingWrapper::sample (path.h:53-82), over the sampling tree (child, p0), which may be a
+// copy in LDS
+__device__ inline float guide_sample(const TraceArgs &a, const int32_t *child, const float *p0, float &x0,
+                                     float &x1) {
     const float u = a.uniformProb;
     if (u == 1.0f) return 1.0f;
     float pdf = 1.0f - u;
     if (x0 < u) {
         x0 /= u;
-        pdf *= tree_pdf(a.gsChild, a.gsP0, x0, x1);
+        pdf *= tree_pdf(child, p0, x0, x1);
     } else {
         x0 -= u;
         x0 /= 1.0f - u;
-        const float gpdf = tree_sample(a.gsChild, a.gsP0, x0, x1);
+        const float gpdf = tree_sample(child, p0, x0, x1);
         pdf *= gpdf;
     }
     pdf += u;
@@ -446,12 +448,13 @@ __device__ inline void primary_sample(const TraceArgs &a, uint64_t index, float 
     x1 = a.primaryOrder == 0 ? u_a : u_b;
 }
 
-__device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p) {
+__device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p, const int32_t *gsChild,
+                                const float *gsP0) {
     const float u_f = radical_inverse_2(index);
     float x0, x1;
     primary_sample(a, index, x0, x1);
     float pdf = 1.0f;
-    if (a.doGuiding && a.guideOn) pdf = guide_sample(a, x0, x1);
+    if (a.doGuiding && a.guideOn) pdf = guide_sample(a, gsChild, gsP0, x0, x1);
     const d3 dl = uniform_sample_sphere((double)x0, (double)x1);
     cf3 Hl = awr_evaluate(cast3<float>(dl));
     Hl = Hl / kInv4PiF;

@@ -347,12 +347,26 @@ __global__ __launch_bounds__(kWfBlock) void k_sort_scatter(WfArgs w) {
 
 // ------------------------------------------------------------------------ generation
 // Thread s generates the path of batch index inv[s] into slot s; a workgroup generates
-// ranges of kRange slots and stages their depth-0 gather and extension entries.
+// ranges of kRange slots and stages their depth-0 gather and extension entries. A sampling
+// tree of up to kGuideLds nodes is copied into LDS first: the guiding warp descends it with
+// one dependent load per level.
+constexpr int kGuideLds = 2048;
 __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
     __shared__ int32_t l_gat[kRange], l_ext[kRange];
     __shared__ unsigned sn[2], sb[2], sc[2];
+    __shared__ int32_t g_child[kGuideLds];
+    __shared__ float g_p0[kGuideLds];
     const int lane = threadIdx.x & 63;
     if (threadIdx.x < 2) sn[threadIdx.x] = 0;
+    const bool guideLds = a.doGuiding && a.guideOn && a.gsSize <= kGuideLds;
+    if (guideLds) {
+        for (int j = threadIdx.x; j < a.gsSize; j += kWfBlock) {
+            g_child[j] = a.gsChild[j];
+            g_p0[j] = a.gsP0[j];
+        }
+    }
+    const int32_t *gsChild = guideLds ? g_child : a.gsChild;
+    const float *gsP0 = guideLds ? g_p0 : a.gsP0;
     __syncthreads();
     WfCounts *cnt = &w.cnt[0];
     for (int64_t r0 = (int64_t)blockIdx.x * kRange; r0 < w.n; r0 += (int64_t)gridDim.x * kRange) {
@@ -363,7 +377,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
             if (valid) {
                 const int64_t m = w.inv[i];
                 p.local = w.base + m;
-                generate(a, (uint64_t)(a.offset + w.base + m), p);
+                generate(a, (uint64_t)(a.offset + w.base + m), p, gsChild, gsP0);
             }
             const bool nee = valid && !a.onlyIndirect;
             stage_append(l_gat, &sn[0], nee, (int32_t)i, lane);
