@@ -66,7 +66,7 @@ def roofline(pipeline, acc, cnt, steps):
         t = cnt["tris_tested"] - cnt["tris_shadow"]
         launches = cnt["closest_launches"]
         kms = acc["closest_ms"] / acc["closest_launches"]
-        kernel, sym = "k_wf_trav<false,false> (closest-hit traversal, one launch per bounce)", "k_wf_trav_closest"
+        kernel, sym = "k_wf_trav<false> (closest-hit traversal, one launch per bounce)", "k_wf_trav_closest"
         rays = cnt["rays_closest"]
         sh_bytes = (cnt["nodes_shadow"] * NODE_BYTES_ANY + cnt["tris_shadow"] * TRI_BYTES) / cnt["shadow_launches"]
         sh_ms = acc["shadow_ms"] / acc["shadow_launches"]
