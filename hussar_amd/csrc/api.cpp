@@ -132,6 +132,11 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     b->slotIds.resize(bvh.slotCount);
     for (int32_t s2 = 0; s2 < bvh.slotCount; ++s2) std::memcpy(&b->slotIds[s2], &bvh.tris[12 * (size_t)s2 + 3], 4);
     if ((s = dalloc(&b->d_counters, 8)) != HSR_OK) return s;
+    {
+        static const std::vector<int32_t> cells = hsr::coherence_cells();
+        if ((s = dalloc(&b->d_cells, cells.size())) != HSR_OK) return s;
+        HSR_HIP(hipMemcpy(b->d_cells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     b->stats.bvh_build_ms = b->bvh_ms;
     *out = b.release();
     return HSR_OK;
@@ -151,6 +156,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_stage_sum);
     dfree(b->d_stage_comp);
     dfree(b->d_counters);
+    dfree(b->d_cells);
     dfree(b->d_events);
     dfree(b->d_evCount);
     dfree(b->d_cubeParts);
@@ -523,6 +529,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         hsr::WfArgs w;
         std::memset(&w, 0, sizeof(w));
         hsr::wavefront_layout(w, b->d_wf, P, C, knobs->maxDepth);
+        w.cells = b->d_cells;
         w.base = done;
         w.n = (int32_t)n;
         a.offset = sampleIndexOffset;

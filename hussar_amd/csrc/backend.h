@@ -83,6 +83,7 @@ struct hsr_backend {
     double *d_stage_sum = nullptr, *d_stage_comp = nullptr;
     size_t stageCap = 0;
     unsigned long long *d_counters = nullptr;
+    int32_t *d_cells = nullptr;  // coherence order of the wavefront generation (kCells)
     // cube frames: the event list of the LDS tile splat (evCap entries) and its count
     hsr::CubeEvent *d_events = nullptr;
     unsigned evCap = 0;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace hsr {
 
 constexpr int kMaxChannels = 16;
@@ -136,17 +138,13 @@ struct WfArgs {
     int8_t *s_st, *s_vis;
     int32_t *q_ext0, *q_ext1, *q_shd, *q_gat;
     WfCounts *cnt;  // [maxDepth + 2]
-    // coherence order of the batch: slot s holds the path of batch index inv[s]
-    uint32_t *key;  // [P] cell of each batch index
-    uint32_t *bins; // [kSortBins] cell counts, then the next free slot of each cell
-    int32_t *inv;   // [P]
+    // coherence order: the residue mod kCells of the Halton indices of each cell of the
+    // primary sample square, cells in Morton order (coherence_cells)
+    const int32_t *cells;
 };
-#ifndef HSR_SORT_BITS
-#define HSR_SORT_BITS 7
-#endif
-constexpr int kSortBits = HSR_SORT_BITS;             // cells per axis: 2^kSortBits
-constexpr int kSortBins = 1 << (2 * kSortBits);      // cells of the primary sample square (Morton order)
-constexpr int kSortBlocks = 256;  // workgroups of the counting sort
+constexpr int kCells = 125 * 243;  // cells of the coherence order: 5^3 x 3^5
+// The kCells residues of the coherence order, cells in Morton order (host).
+std::vector<int32_t> coherence_cells();
 
 // Bytes of device memory the wavefront buffers need for P paths and C channels.
 size_t wavefront_bytes(int64_t P, int C, int maxDepth);

@@ -21,6 +21,10 @@
 // of kClaim items per atomic. The traversal kernels hold only the ray and stack, which keeps
 // their register use low and their occupancy high; a lane whose ray is done takes the next
 // one from its wave's claim while the others continue.
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "trace_common.h"
 
 namespace hsr {
@@ -262,100 +266,38 @@ __device__ inline void write_extend(const WfArgs &w, bool alive, bool ends, int6
 }
 
 // ------------------------------------------------------------------------ coherence order
-// The batch's paths are laid out in slot order sorted by the cell of their primary sample
-// (before the guiding warp, which maps sample-space neighbourhoods to direction-space
-// neighbourhoods at the scale of the guiding tree): neighbouring lanes then trace nearly
-// parallel rays from the same antenna, and the queues of later bounces, appended in wave
-// order, keep much of that coherence. A counting sort over kSortBins Morton-ordered cells:
-// per-workgroup counts in LDS added to global counts, an exclusive scan of the counts, then a
-// scatter in which each workgroup reserves one range per cell with one atomic and ranks its
-// elements with LDS counters. The order within a cell depends on the atomics; each path's
-// result does not.
-__device__ inline uint32_t morton_cell(float x0, float x1) {
-    constexpr uint32_t side = 1u << kSortBits;
-    const uint32_t cx = min(side - 1, (uint32_t)(x0 * (float)side)), cy = min(side - 1, (uint32_t)(x1 * (float)side));
-    uint32_t k = 0;
-#pragma unroll
-    for (int b = 0; b < kSortBits; ++b) k |= (((cx >> b) & 1u) << (2 * b)) | (((cy >> b) & 1u) << (2 * b + 1));
-    return k;
+// The batch's paths are laid out in slot order by the cell of their primary sample (before
+// the guiding warp, which maps sample-space neighbourhoods to direction-space neighbourhoods
+// at the scale of the guiding tree): neighbouring lanes then trace nearly parallel rays from
+// the same antenna, and the queues of later bounces, appended in range order, keep much of
+// that coherence. No sort is needed: the primary sample is (RI_5(h), RI_3(h)) (or swapped)
+// for Halton index h, and the first 3 base-5 digits of RI_5(h) are the last 3 base-5 digits
+// of h reversed (likewise 5 base-3 digits), so the cell of [0,1)^2 cut into 125 x 243 is a
+// function of h mod 125 and h mod 243, i.e. of the residue of h mod kCells = 30375 (Chinese
+// remainder theorem). The cells are visited in Morton order of (u, v), each with the residue
+// of its indices (kCells residues, made once per backend by coherence_cells); cell c holds
+// the indices h = B0 + t_c + j * kCells of the batch [B0, B0 + n), t_c = (r_c - B0) mod
+// kCells, and its slots follow those of the cells before it. The order is deterministic.
+__device__ inline int64_t cell_first(int32_t r, int64_t B0) {
+    int64_t t = ((int64_t)r - B0) % kCells;
+    return t < 0 ? t + kCells : t;
 }
-
-__device__ inline void sort_chunk(int64_t n, int64_t &b0, int64_t &b1) {
-    const int64_t per = (n + kSortBlocks - 1) / kSortBlocks;
-    b0 = min(n, (int64_t)blockIdx.x * per);
-    b1 = min(n, b0 + per);
-}
-
-__global__ __launch_bounds__(kWfBlock) void k_sort_count(TraceArgs a, WfArgs w) {
-    __shared__ uint32_t h[kSortBins];
-    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock) h[c] = 0;
-    __syncthreads();
-    int64_t b0, b1;
-    sort_chunk(w.n, b0, b1);
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += kWfBlock) {
-        float x0, x1;
-        primary_sample(a, (uint64_t)(a.offset + w.base + i), x0, x1);
-        const uint32_t k = morton_cell(x0, x1);
-        w.key[i] = k;
-        atomicAdd(&h[k], 1u);
-    }
-    __syncthreads();
-    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock)
-        if (h[c]) atomicAdd(&w.bins[c], h[c]);
-}
-
-// Exclusive scan of the kSortBins counts in place, by one workgroup of 1024 (4 cells each).
-__global__ __launch_bounds__(1024) void k_sort_scan(WfArgs w) {
-    __shared__ uint32_t part[1024];
-    constexpr int kPer = kSortBins / 1024;
-    uint32_t *v = w.bins + (size_t)threadIdx.x * kPer;
-    uint32_t s = 0;
-    for (int j = 0; j < kPer; ++j) s += v[j];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;  // exclusive prefix of this thread's cells
-    for (int j = 0; j < kPer; ++j) {
-        const uint32_t c = v[j];
-        v[j] = run;
-        run += c;
-    }
-}
-
-__global__ __launch_bounds__(kWfBlock) void k_sort_scatter(WfArgs w) {
-    __shared__ uint32_t h[kSortBins];
-    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock) h[c] = 0;
-    __syncthreads();
-    int64_t b0, b1;
-    sort_chunk(w.n, b0, b1);
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += kWfBlock) atomicAdd(&h[w.key[i]], 1u);
-    __syncthreads();
-    // one range per cell for this workgroup's elements
-    for (int c = threadIdx.x; c < kSortBins; c += kWfBlock)
-        if (h[c]) h[c] = atomicAdd(&w.bins[c], h[c]);
-    __syncthreads();
-    for (int64_t i = b0 + threadIdx.x; i < b1; i += kWfBlock) {
-        const uint32_t pos = atomicAdd(&h[w.key[i]], 1u);
-        w.inv[pos] = (int32_t)i;
-    }
-}
+__device__ inline int64_t cell_count(int64_t t, int64_t n) { return t < n ? (n - 1 - t) / kCells + 1 : 0; }
 
 // ------------------------------------------------------------------------ generation
-// Thread s generates the path of batch index inv[s] into slot s; a workgroup generates
-// ranges of kRange slots and stages their depth-0 gather and extension entries. A sampling
-// tree of up to kGuideLds nodes is copied into LDS first: the guiding warp descends it with
-// one dependent load per level.
+// Workgroup w generates the paths of cells [w * K, (w + 1) * K) of the coherence order into
+// their slots, in ranges of kRange paths whose depth-0 gather and extension entries it
+// stages. A sampling tree of up to kGuideLds nodes is copied into LDS first: the guiding warp
+// descends it with one dependent load per level.
 constexpr int kGuideLds = 2048;
-__global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
+constexpr int kGenCellsMax = kRange;  // cells per workgroup (K)
+__global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int K) {
     __shared__ int32_t l_gat[kRange], l_ext[kRange];
     __shared__ unsigned sn[2], sb[2], sc[2];
     __shared__ int32_t g_child[kGuideLds];
     __shared__ float g_p0[kGuideLds];
+    __shared__ int32_t cstart[kGenCellsMax + 1];  // prefix of the paths of this workgroup's cells
+    __shared__ int64_t part[kWfBlock / 64];
     const int lane = threadIdx.x & 63;
     if (threadIdx.x < 2) sn[threadIdx.x] = 0;
     const bool guideLds = a.doGuiding && a.guideOn && a.gsSize <= kGuideLds;
@@ -367,17 +309,43 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w) {
     }
     const int32_t *gsChild = guideLds ? g_child : a.gsChild;
     const float *gsP0 = guideLds ? g_p0 : a.gsP0;
+    const int64_t B0 = a.offset + w.base, n = w.n;
+    const int c0 = blockIdx.x * K, nc = min(K, kCells - c0);
+    // the first slot: the paths of all cells before c0
+    int64_t before = 0;
+    for (int c = threadIdx.x; c < c0; c += kWfBlock) before += cell_count(cell_first(w.cells[c], B0), n);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) before += __shfl_xor(before, off, 64);
+    if (lane == 0) part[threadIdx.x >> 6] = before;
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int c = 0; c < nc; ++c) {
+            cstart[c] = acc;
+            acc += (int32_t)cell_count(cell_first(w.cells[c0 + c], B0), n);
+        }
+        cstart[nc] = acc;
+    }
     __syncthreads();
+    const int64_t s0 = part[0] + part[1] + part[2] + part[3];
+    const int32_t total = cstart[nc];
     WfCounts *cnt = &w.cnt[0];
-    for (int64_t r0 = (int64_t)blockIdx.x * kRange; r0 < w.n; r0 += (int64_t)gridDim.x * kRange) {
+    for (int32_t r0 = 0; r0 < total; r0 += kRange) {
         for (int r = 0; r < kStageIters; ++r) {
-            const int64_t i = r0 + r * kWfBlock + threadIdx.x;
-            const bool valid = i < w.n;
+            const int32_t item = r0 + r * kWfBlock + threadIdx.x;
+            const bool valid = item < total;
+            const int64_t i = s0 + item;  // slot
             PathState p;
             if (valid) {
-                const int64_t m = w.inv[i];
+                // the cell of this item: the last c with cstart[c] <= item
+                int lo = 0, hi = nc - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (cstart[mid] <= item) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int64_t m = cell_first(w.cells[c0 + lo], B0) + (int64_t)(item - cstart[lo]) * kCells;
                 p.local = w.base + m;
-                generate(a, (uint64_t)(a.offset + w.base + m), p, gsChild, gsP0);
+                generate(a, (uint64_t)(B0 + m), p, gsChild, gsP0);
             }
             const bool nee = valid && !a.onlyIndirect;
             stage_append(l_gat, &sn[0], nee, (int32_t)i, lane);
@@ -933,7 +901,7 @@ __global__ void k_cube_reduce(double2 *__restrict__ frame, const double2 *__rest
     frame[i] = f;
 }
 
-__global__ void k_wf_zero_counts(WfCounts *c, int n, uint32_t *bins) {
+__global__ void k_wf_zero_counts(WfCounts *c, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         c[i].n_ext.v = 0;
@@ -944,7 +912,6 @@ __global__ void k_wf_zero_counts(WfCounts *c, int n, uint32_t *bins) {
             c[i].head_shd[k].v = 0;
         }
     }
-    if (i < kSortBins) bins[i] = 0;
 }
 
 // ------------------------------------------------------------------------ host side
@@ -964,7 +931,6 @@ size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
     b += 2 * align256(sizeof(int8_t) * S);
     b += 3 * align256(sizeof(int32_t) * P) + align256(sizeof(int32_t) * S);
     b += align256(sizeof(WfCounts) * (maxDepth + 2));
-    b += 2 * align256(sizeof(uint32_t) * P) + align256(sizeof(uint32_t) * kSortBins);
     return b;
 }
 
@@ -995,12 +961,43 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.q_gat = (int32_t *)take(sizeof(int32_t) * P);
     w.q_shd = (int32_t *)take(sizeof(int32_t) * S);
     w.cnt = (WfCounts *)take(sizeof(WfCounts) * (maxDepth + 2));
-    w.key = (uint32_t *)take(sizeof(uint32_t) * P);
-    w.inv = (int32_t *)take(sizeof(int32_t) * P);
-    w.bins = (uint32_t *)take(sizeof(uint32_t) * kSortBins);
 }
 
 size_t gather_lds_bytes(int N, int C) { return (size_t)N * C * 2 * sizeof(double); }
+
+std::vector<int32_t> coherence_cells() {
+    // cell (u, v): the first 3 base-5 digits and the first 5 base-3 digits of the primary
+    // sample; its indices h satisfy h mod 125 = the 3 digits of u reversed and h mod 243 =
+    // the 5 digits of v reversed, i.e. one residue mod 30375
+    constexpr int U = 125, V = 243;
+    static_assert(U * V == kCells, "cells");
+    auto rev = [](int x, int base, int digits) {
+        int r = 0;
+        for (int k = 0; k < digits; ++k) {
+            r = r * base + x % base;
+            x /= base;
+        }
+        return r;
+    };
+    auto spread = [](uint32_t x) {  // bits of x to the even bit positions
+        uint32_t r = 0;
+        for (int b = 0; b < 16; ++b) r |= ((x >> b) & 1u) << (2 * b);
+        return r;
+    };
+    std::vector<std::pair<uint32_t, int32_t>> cells;
+    cells.reserve(kCells);
+    for (int u = 0; u < U; ++u)
+        for (int v = 0; v < V; ++v) {
+            const int a5 = rev(u, 5, 3), a3 = rev(v, 3, 5);
+            int r = a5;  // r = a5 (mod 125), r = a3 (mod 243)
+            while (r % V != a3) r += U;
+            cells.push_back({spread((uint32_t)u) | (spread((uint32_t)v) << 1), (int32_t)r});
+        }
+    std::sort(cells.begin(), cells.end());
+    std::vector<int32_t> out(kCells);
+    for (int c = 0; c < kCells; ++c) out[c] = cells[c].second;
+    return out;
+}
 
 int cube_tile_chirps(int M, int N, size_t ldsBytes) {
     const int64_t rows = (int64_t)(ldsBytes / (2 * sizeof(double))) / N;
@@ -1051,13 +1048,11 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket), kWfBlock);
     const int gShade = fit(gridWork, n, kRange);
     const int gNee = fit(gridWork, n * nrx, kRange);
-    const int gGen = fit(1 << 30, n, kRange);
-    hipLaunchKernelGGL(k_wf_zero_counts, dim3((max(maxDepth + 2, kSortBins) + 255) / 256), dim3(256), 0, s, w.cnt,
-                       maxDepth + 2, w.bins);
-    hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(kWfBlock), 0, s, a, w);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, w);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(kWfBlock), 0, s, w);
-    hipLaunchKernelGGL(k_wf_gen, dim3(gGen), dim3(kWfBlock), 0, s, a, w);
+    hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 255) / 256), dim3(256), 0, s, w.cnt, maxDepth + 2);
+    // generation: K cells per workgroup, about kRange paths each
+    const int64_t q = n / kCells;
+    const int K = (int)std::max<int64_t>(1, std::min<int64_t>(kGenCellsMax, kRange / (q + 1)));
+    hipLaunchKernelGGL(k_wf_gen, dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
     if (glds > 65536) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_wf_gather,
