@@ -141,6 +141,7 @@ struct WfArgs {
     // coherence order: the residue mod kCells of the Halton indices of each cell of the
     // primary sample square, cells in Morton order (coherence_cells)
     const int32_t *cells;
+    int32_t *cellStart;  // [kCells + 1] first slot of each cell in this batch
 };
 constexpr int kCells = 125 * 243;  // cells of the coherence order: 5^3 x 3^5
 // The kCells residues of the coherence order, cells in Morton order (host).

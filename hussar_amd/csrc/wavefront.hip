@@ -284,9 +284,40 @@ __device__ inline int64_t cell_first(int32_t r, int64_t B0) {
 }
 __device__ inline int64_t cell_count(int64_t t, int64_t n) { return t < n ? (n - 1 - t) / kCells + 1 : 0; }
 
+// cellStart[c] = the first slot of cell c (exclusive prefix of the cells' path counts), and
+// cellStart[kCells] = n: one workgroup of 1024; the counts first go to LDS with coalesced
+// loads of the residues, then each thread sums and writes a contiguous run of cells.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_cell_scan(TraceArgs a, WfArgs w) {
+    __shared__ uint16_t cnt[kCells];  // n / kCells + 1 paths per cell at most: 2210 for a batch of 2^26
+    __shared__ int32_t part[kScanThreads];
+    constexpr int per = (kCells + kScanThreads - 1) / kScanThreads;
+    const int64_t B0 = a.offset + w.base, n = w.n;
+    for (int c = threadIdx.x; c < kCells; c += kScanThreads)
+        cnt[c] = (uint16_t)cell_count(cell_first(w.cells[c], B0), n);
+    __syncthreads();
+    const int c0 = threadIdx.x * per, c1 = min(kCells, c0 + per);
+    int32_t sum = 0;
+    for (int c = c0; c < c1; ++c) sum += cnt[c];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int32_t run = part[threadIdx.x] - sum;
+    for (int c = c0; c < c1; ++c) {
+        w.cellStart[c] = run;
+        run += cnt[c];
+    }
+    if (threadIdx.x == kScanThreads - 1) w.cellStart[kCells] = run;
+}
+
 // ------------------------------------------------------------------------ generation
 // Workgroup w generates the paths of cells [w * K, (w + 1) * K) of the coherence order into
-// their slots, in ranges of kRange paths whose depth-0 gather and extension entries it
+// their slots (cellStart), in ranges of kRange paths whose depth-0 gather and extension entries it
 // stages. A sampling tree of up to kGuideLds nodes is copied into LDS first: the guiding warp
 // descends it with one dependent load per level.
 constexpr int kGuideLds = 2048;
@@ -296,8 +327,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int 
     __shared__ unsigned sn[2], sb[2], sc[2];
     __shared__ int32_t g_child[kGuideLds];
     __shared__ float g_p0[kGuideLds];
-    __shared__ int32_t cstart[kGenCellsMax + 1];  // prefix of the paths of this workgroup's cells
-    __shared__ int64_t part[kWfBlock / 64];
+    __shared__ int32_t cstart[kGenCellsMax + 1];  // first slots of this workgroup's cells
     const int lane = threadIdx.x & 63;
     if (threadIdx.x < 2) sn[threadIdx.x] = 0;
     const bool guideLds = a.doGuiding && a.guideOn && a.gsSize <= kGuideLds;
@@ -309,24 +339,11 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int 
     }
     const int32_t *gsChild = guideLds ? g_child : a.gsChild;
     const float *gsP0 = guideLds ? g_p0 : a.gsP0;
-    const int64_t B0 = a.offset + w.base, n = w.n;
+    const int64_t B0 = a.offset + w.base;
     const int c0 = blockIdx.x * K, nc = min(K, kCells - c0);
-    // the first slot: the paths of all cells before c0
-    int64_t before = 0;
-    for (int c = threadIdx.x; c < c0; c += kWfBlock) before += cell_count(cell_first(w.cells[c], B0), n);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) before += __shfl_xor(before, off, 64);
-    if (lane == 0) part[threadIdx.x >> 6] = before;
-    if (threadIdx.x == 0) {
-        int32_t acc = 0;
-        for (int c = 0; c < nc; ++c) {
-            cstart[c] = acc;
-            acc += (int32_t)cell_count(cell_first(w.cells[c0 + c], B0), n);
-        }
-        cstart[nc] = acc;
-    }
+    const int32_t s0 = w.cellStart[c0];
+    for (int c = threadIdx.x; c <= nc; c += kWfBlock) cstart[c] = w.cellStart[c0 + c] - s0;
     __syncthreads();
-    const int64_t s0 = part[0] + part[1] + part[2] + part[3];
     const int32_t total = cstart[nc];
     WfCounts *cnt = &w.cnt[0];
     for (int32_t r0 = 0; r0 < total; r0 += kRange) {
@@ -931,6 +948,7 @@ size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
     b += 2 * align256(sizeof(int8_t) * S);
     b += 3 * align256(sizeof(int32_t) * P) + align256(sizeof(int32_t) * S);
     b += align256(sizeof(WfCounts) * (maxDepth + 2));
+    b += align256(sizeof(int32_t) * (kCells + 1));
     return b;
 }
 
@@ -961,6 +979,7 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.q_gat = (int32_t *)take(sizeof(int32_t) * P);
     w.q_shd = (int32_t *)take(sizeof(int32_t) * S);
     w.cnt = (WfCounts *)take(sizeof(WfCounts) * (maxDepth + 2));
+    w.cellStart = (int32_t *)take(sizeof(int32_t) * (kCells + 1));
 }
 
 size_t gather_lds_bytes(int N, int C) { return (size_t)N * C * 2 * sizeof(double); }
@@ -1052,6 +1071,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     // generation: K cells per workgroup, about kRange paths each
     const int64_t q = n / kCells;
     const int K = (int)std::max<int64_t>(1, std::min<int64_t>(kGenCellsMax, kRange / (q + 1)));
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(kScanThreads), 0, s, a, w);
     hipLaunchKernelGGL(k_wf_gen, dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
     if (glds > 65536) {
