@@ -114,8 +114,9 @@ struct WfCounts {
 // Structure-of-arrays path state and queues for a batch of up to P paths.
 //  sd: 9 double fields x P   (o.xyz, d.xyz, time, r, path-length rate)
 //  sf: 15 float fields x P   (H 6, guidingWeight 2, x0, x1, pdf, freq, vertex velocity 3)
-//  si: 3 int fields x P      (depth, flags: bit 0 = the path ends after its pending NEE,
-//                            the path's index within the batch)
+//  si: 2 int fields x P      (flags: bit 0 = the path ends after its pending NEE, the path's
+//                            index within the batch); a path's depth is its queue's
+//  (the origin, time, length, guiding weight and motion rows are not written at depth 0)
 //  ray_o / ray_d [P]: the path's current vertex rounded to float (origin of its shadow and
 //  extension rays) and its extension direction
 //  shadow slots [P * C]: ray direction and extent (d.xyz, tfar), NEE payload (tn, v,

@@ -33,7 +33,7 @@ namespace hsr {
 // produced and consumed inside k_wf_shade and never stored.
 enum : int { D_OX = 0, D_OY, D_OZ, D_DX, D_DY, D_DZ, D_TIME, D_R, D_RATE, D_N };
 enum : int { F_H = 0, F_GWR = 6, F_GWI, F_X0, F_X1, F_PDF, F_FREQ, F_VX, F_VY, F_VZ, F_N };
-enum : int { I_DEPTH = 0, I_FLAGS, I_LOCAL, I_N };
+enum : int { I_FLAGS = 0, I_LOCAL, I_N };
 enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
 
 constexpr int kWfBlock = 256;
@@ -157,30 +157,34 @@ __device__ inline unsigned claim_items(WfCounter *heads, unsigned n, uint64_t id
     return my;
 }
 
+// A new path's state. What every path has at depth 0 is not stored: the origin (the Tx),
+// time, length, guiding weight and motion terms are zero or the scene's, and the depth of a
+// path is that of the queue it is in (the stages take it as their depth argument); the
+// loaders below supply them at depth 0.
 __device__ inline void store_state(const WfArgs &w, int64_t i, const PathState &p) {
     double *d = w.sd;
     float *f = w.sf;
     const int64_t P = w.P;
-    d[D_OX * P + i] = p.o.x; d[D_OY * P + i] = p.o.y; d[D_OZ * P + i] = p.o.z;
     d[D_DX * P + i] = p.d.x; d[D_DY * P + i] = p.d.y; d[D_DZ * P + i] = p.d.z;
-    d[D_TIME * P + i] = p.time;
-    d[D_R * P + i] = p.r;
     f[(F_H + 0) * P + i] = p.H.x.re; f[(F_H + 1) * P + i] = p.H.x.im;
     f[(F_H + 2) * P + i] = p.H.y.re; f[(F_H + 3) * P + i] = p.H.y.im;
     f[(F_H + 4) * P + i] = p.H.z.re; f[(F_H + 5) * P + i] = p.H.z.im;
-    f[F_GWR * P + i] = p.gw.re;
-    f[F_GWI * P + i] = p.gw.im;
     f[F_X0 * P + i] = p.x0;
     f[F_X1 * P + i] = p.x1;
     f[F_PDF * P + i] = p.pdf;
     f[F_FREQ * P + i] = p.freq;
-    w.si[I_DEPTH * P + i] = p.depth;
     w.si[I_LOCAL * P + i] = (int32_t)(p.local - w.base);
 }
 
-// motion: the path-length rate and the current vertex's velocity
-__device__ inline void load_motion(const WfArgs &w, int64_t i, PathState &p) {
+// motion: the path-length rate and the current vertex's velocity (0 and the platform's at
+// depth 0)
+__device__ inline void load_motion(const TraceArgs &a, const WfArgs &w, int64_t i, PathState &p, int depth) {
     const int64_t P = w.P;
+    if (depth == 0) {
+        p.rate = 0.0;
+        p.vprev = {a.ego[0], a.ego[1], a.ego[2]};
+        return;
+    }
     p.rate = w.sd[D_RATE * P + i];
     p.vprev = {w.sf[F_VX * P + i], w.sf[F_VY * P + i], w.sf[F_VZ * P + i]};
 }
@@ -193,14 +197,20 @@ __device__ inline void store_motion(const WfArgs &w, int64_t i, const PathState 
 }
 
 // The fields a stage needs, loaded separately to keep register use and traffic down.
-__device__ inline void load_chain(const WfArgs &w, int64_t i, PathState &p) {
+__device__ inline void load_chain(const TraceArgs &a, const WfArgs &w, int64_t i, PathState &p, int depth) {
     const double *d = w.sd;
     const int64_t P = w.P;
-    p.o = {d[D_OX * P + i], d[D_OY * P + i], d[D_OZ * P + i]};
+    if (depth == 0) {
+        p.o = posd(a.tx);
+        p.time = 0.0;
+        p.r = 0.0;
+    } else {
+        p.o = {d[D_OX * P + i], d[D_OY * P + i], d[D_OZ * P + i]};
+        p.time = d[D_TIME * P + i];
+        p.r = d[D_R * P + i];
+    }
     p.d = {d[D_DX * P + i], d[D_DY * P + i], d[D_DZ * P + i]};
-    p.time = d[D_TIME * P + i];
-    p.r = d[D_R * P + i];
-    p.depth = w.si[I_DEPTH * P + i];
+    p.depth = depth;
     p.local = w.base + w.si[I_LOCAL * P + i];
 }
 __device__ inline void load_field(const WfArgs &w, int64_t i, PathState &p) {
@@ -210,10 +220,11 @@ __device__ inline void load_field(const WfArgs &w, int64_t i, PathState &p) {
            {f[(F_H + 2) * P + i], f[(F_H + 3) * P + i]},
            {f[(F_H + 4) * P + i], f[(F_H + 5) * P + i]}};
 }
-__device__ inline void load_record(const WfArgs &w, int64_t i, PathState &p) {
+// gwZero: the guiding weight is still zero (never stored: depth 0 before any NEE)
+__device__ inline void load_record(const WfArgs &w, int64_t i, PathState &p, bool gwZero) {
     const float *f = w.sf;
     const int64_t P = w.P;
-    p.gw = {f[F_GWR * P + i], f[F_GWI * P + i]};
+    p.gw = gwZero ? cf{0.0f, 0.0f} : cf{f[F_GWR * P + i], f[F_GWI * P + i]};
     p.x0 = f[F_X0 * P + i];
     p.x1 = f[F_X1 * P + i];
     p.pdf = f[F_PDF * P + i];
@@ -230,7 +241,6 @@ __device__ inline void store_hit(const WfArgs &w, int64_t i, const PathState &p)
     f[(F_H + 0) * P + i] = p.H.x.re; f[(F_H + 1) * P + i] = p.H.x.im;
     f[(F_H + 2) * P + i] = p.H.y.re; f[(F_H + 3) * P + i] = p.H.y.im;
     f[(F_H + 4) * P + i] = p.H.z.re; f[(F_H + 5) * P + i] = p.H.z.im;
-    w.si[I_DEPTH * P + i] = p.depth;
 }
 
 // Guiding record of a finished path (path.h:448-450), wave-aggregated.
@@ -370,7 +380,6 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int 
             stage_append(l_ext, &sn[1], valid && !ends, (int32_t)i, lane);
             write_extend(w, valid, ends, i, p);
             if (valid) store_state(w, i, p);
-            if (valid && a.motion) store_motion(w, i, p);
             finish_record(a, valid && ends && !nee, p, lane);
         }
         __syncthreads();
@@ -679,17 +688,17 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
                 // a missed path needs none of the chain and field (most rays of a batch's
                 // first bounce miss): only its depth and index, and its record when training
                 const int2 h = w.hit[i];
-                p.depth = w.si[I_DEPTH * w.P + i];
+                p.depth = depth;
                 p.local = w.base + w.si[I_LOCAL * w.P + i];
                 if (a.recHits && p.local < a.recPaths)
                     a.recHits[p.local * a.maxDepth + p.depth] = h.x >= 0 ? __float_as_int(a.tris[3 * h.x].w) : -1;
                 if (h.x >= 0) {
-                    load_chain(w, i, p);
+                    load_chain(a, w, i, p, depth);
                     load_field(w, i, p);
-                    if (a.motion) load_motion(w, i, p);
+                    if (a.motion) load_motion(a, w, i, p, depth);
                     alive = apply_hit(a, p, h.x);
                 }
-                if (!alive && recording) load_record(w, i, p);
+                if (!alive && recording) load_record(w, i, p, depth == 0 && a.onlyIndirect);
             }
             stage_append(l_gat, &sn[0], alive, (int32_t)i, lane);
             const bool ends = alive && path_ends(a, p);
@@ -743,9 +752,9 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a
                 const int64_t i = w.q_gat[k / nrx];
                 const int ch = (int)(k % nrx);
                 PathState p;
-                load_chain(w, i, p);
+                load_chain(a, w, i, p, 0);
                 p.freq = w.sf[F_FREQ * w.P + i];
-                if (a.motion) load_motion(w, i, p);
+                if (a.motion) load_motion(a, w, i, p, 0);
                 j = i * a.C + ch;
                 test = write_slot(a, w, j, nee_setup(a, p, ch));
             }
@@ -785,11 +794,14 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
         const int64_t i = valid ? w.q_gat[k] : 0;
         PathState p;
         if (valid) {
-            p.gw = {w.sf[F_GWR * P + i], w.sf[F_GWI * P + i]};
+            // the guiding weight is in memory once a gather has stored it: from depth 1 on, or
+            // depth 2 when the direct paths are not traced (no gather at depth 0)
+            const bool gwZero = depth == 0 || (depth == 1 && a.onlyIndirect);
+            p.gw = gwZero ? cf{0.0f, 0.0f} : cf{w.sf[F_GWR * P + i], w.sf[F_GWI * P + i]};
             p.x0 = w.sf[F_X0 * P + i];
             p.x1 = w.sf[F_X1 * P + i];
             p.pdf = w.sf[F_PDF * P + i];
-            p.depth = w.si[I_DEPTH * P + i];
+            p.depth = depth;
             p.local = w.base + w.si[I_LOCAL * P + i];
         }
         const bool rec = valid && a.recNee && p.local < a.recPaths && p.depth >= 1;
