@@ -20,6 +20,7 @@ OPT_COUNTERS, OPT_RECORD_HITS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE, OPT_PHASE_MOD
 PHASE_FP64, PHASE_FP32, PHASE_DOUBLE_FLOAT = 0, 1, 2
 OPT_TRAV_WAVES = 6
 OPT_BATCH_PATHS = 7
+OPT_CUBE_EVENTS = 8
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER

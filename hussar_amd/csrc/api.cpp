@@ -337,6 +337,11 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
                 return fail(HSR_ERR_INVALID, "paths per batch must lie in [1024, 2^24]");
             b->wfCap = value;
             return HSR_OK;
+        case HSR_OPT_CUBE_EVENTS:
+            if (value < 0 || value > (int64_t)hsr::kCubeEvents)
+                return fail(HSR_ERR_INVALID, "cube event list capacity must lie in [0, 2^22]");
+            b->optCubeEvents = value;
+            return HSR_OK;
         case HSR_OPT_TRAV_WAVES:
             if (value < 0 || value > 64) return fail(HSR_ERR_INVALID, "traversal waves per CU out of range");
             b->optTravWaves = (int32_t)value;
@@ -464,15 +469,17 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     const int64_t cubeCells = (int64_t)chirps * N * C;
     int cubeG = (int)std::min<int64_t>(std::max(1, (2 * b->cu + cubeTiles - 1) / cubeTiles),
                                        kCubePartDoubles / (2 * cubeCells));
-    if (cubeMt > 0 && cubeG >= 1 && b->optPipeline == 1) {
+    const unsigned evWant = b->optCubeEvents < 0 ? (b->evCap ? b->evCap : hsr::kCubeEvents)
+                                                 : (unsigned)b->optCubeEvents;
+    if (cubeMt > 0 && cubeG >= 1 && b->optPipeline == 1 && evWant > 0) {
         // a slot that reserved less room for partial cubes uses fewer of them
         if (b->cubePartsCap >= 2 * cubeCells)
             cubeG = (int)std::min<int64_t>(cubeG, b->cubePartsCap / (2 * cubeCells));
-        hsr_status s = hsr::backend_events(b, (int64_t)cubeG * 2 * cubeCells, b->evCap ? b->evCap : hsr::kCubeEvents);
+        hsr_status s = hsr::backend_events(b, (int64_t)cubeG * 2 * cubeCells, evWant);
         if (s != HSR_OK) return s;
         a.events = b->d_events;
         a.evCount = b->d_evCount;
-        a.evCap = b->evCap;
+        a.evCap = std::min(b->evCap, evWant);
     }
 
     // grid: as many workgroups as fit at once, fewer for small budgets (>= 8 paths per lane)

@@ -101,6 +101,7 @@ struct hsr_backend {
     int32_t optPhaseMode = 0;
     int32_t optPipeline = 1;
     int32_t optTravWaves = 0;  // resident waves per CU of the traversal kernels, 0 = all that fit
+    int64_t optCubeEvents = -1;  // cube event list capacity (HSR_OPT_CUBE_EVENTS), -1 = the default
     int32_t maxGrid = 0;
     // wavefront buffers
     void *d_wf = nullptr;

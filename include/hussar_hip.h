@@ -281,6 +281,11 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        more path state in device memory (about 0.5 KB per path with 4
  *                        receivers). */
 #define HSR_OPT_BATCH_PATHS 7
+/*  HSR_OPT_CUBE_EVENTS   wavefront pipeline, frames with several chirps (or one chirp above
+ *                        the LDS frame tile): capacity of the event list whose events are
+ *                        accumulated through LDS tiles (default 2^22 = 128 MiB; 0 = none). Events
+ *                        beyond it are added to the frame directly with device atomics. */
+#define HSR_OPT_CUBE_EVENTS 8
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]

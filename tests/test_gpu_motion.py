@@ -121,3 +121,37 @@ def test_motion_errors(clutter):
     three = H.Scene(rfConfig=sc.rfConfig, tx=sc.tx, rx=sc.rx, rxs=sc.rxs, txs=[sc.tx, sc.tx])
     with pytest.raises(HsrError, match="INVALID"):
         b.run_raw(three, kn, 0, 1024)  # 2 x 4 virtual channels > 4 frame channels
+
+
+def test_cube_event_list_overflow(clutter):
+    """Cube events beyond the event list of the LDS tile splat go straight to the frame with
+    device atomics: the same run with a list of 16 events, with none at all, and with the
+    default list (every event through LDS). Same taps, so the cubes agree to rounding."""
+    from hussar_amd._lib import OPT_COUNTERS, OPT_CUBE_EVENTS
+    mesh, vel = clutter
+    sc = scenes.radar_array(rf_motion(), channels=4, position=(0.0, 0.75, 0.0))
+    sc.velocity = (0.0, 0.0, 8.0)
+    kn = H.Knobs(maxDepth=3, doGuiding=False)
+    fc = H.FrameConfig(16, 64, 4)
+    n = 1 << 15
+    frames = []
+    for cap in (-1, 16, 0):
+        print(f"cube event list: capacity {cap}", flush=True)
+        b = H.HipBackend(mesh)
+        b._configure(fc)
+        b.set_velocities(vel)
+        if cap >= 0:
+            b.set_option(OPT_CUBE_EVENTS, cap)
+        b.set_option(OPT_COUNTERS, 1)
+        b.run_raw(sc, kn, 0, n)
+        z, tw = b.fetch("complex128")
+        assert tw == n
+        frames.append(z)
+        if cap == 16:
+            assert b.stats()["events"] > 16  # the list overflowed
+    assert np.abs(frames[0]).max() > 0
+    assert rel_l2(frames[1], frames[0]) < 1e-9
+    assert rel_l2(frames[2], frames[0]) < 1e-9
+    b = H.HipBackend(mesh)
+    with pytest.raises(HsrError, match="INVALID"):
+        b.set_option(OPT_CUBE_EVENTS, (1 << 22) + 1)
