@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     int32_t spill[kStackSpill];
     Stack st;
     st.spill = spill;
-    st.lds = lds_stack + tid;
+    st.set_column(lds_stack + tid);
     st.sp = 0;
     uint32_t cn = 0, ct = 0, crc = 0, crs = 0, cev = 0;
 

@@ -37,18 +37,25 @@ __device__ inline d3 posd(const DevAntenna &a) { return {(double)a.pos[0], (doub
 // The spill entries live in a private array of the kernel, reached through a pointer, so that
 // the stack pointer itself stays in a register (kept in one aggregate with the array, it would
 // be placed in private memory with it and every push and pop would load it from there).
+// The LDS column is typed as LDS (address space 3) so that its accesses are ds_read/ds_write
+// and are not merged with the spill's into one flat access.
+typedef __attribute__((address_space(3))) int32_t lds_int32;
 struct Stack {
-    int32_t *lds;    // column of this thread: lds[i * kTraceBlock]
+    lds_int32 *lds;  // column of this thread: lds[i * kTraceBlock]
     int32_t *spill;  // [kStackSpill] private
     int sp;
+    __device__ inline void set_column(int32_t *column) { lds = (lds_int32 *)column; }
     __device__ inline void push(int32_t v) {
-        if (sp < kStackShort) lds[sp * kTraceBlock] = v;
+        if (__builtin_expect(sp < kStackShort, 1)) lds[sp * kTraceBlock] = v;
         else spill[sp - kStackShort] = v;
         ++sp;
     }
     __device__ inline int32_t pop() {
         --sp;
-        return sp < kStackShort ? lds[sp * kTraceBlock] : spill[sp - kStackShort];
+        int32_t v;
+        if (__builtin_expect(sp < kStackShort, 1)) v = lds[sp * kTraceBlock];
+        else v = spill[sp - kStackShort];
+        return v;
     }
 };
 

@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
     int32_t spill[kStackSpill];
     Stack st;
     st.spill = spill;
-    st.lds = lds_stack + tid;
+    st.set_column(lds_stack + tid);
     st.sp = 0;
     WfCounts *cnt = &w.cnt[depth];
     const unsigned int n = cnt->n_ext.v;
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
     int32_t spill[kStackSpill];
     Stack st;
     st.spill = spill;
-    st.lds = lds_stack + tid;
+    st.set_column(lds_stack + tid);
     st.sp = 0;
     WfCounts *cnt = &w.cnt[depth];
     const unsigned int n = cnt->n_shd.v;
