@@ -362,6 +362,7 @@ hsr_status hsr_hip_last_stats(hsr_backend *b, hsr_run_stats *out) {
 }
 
 static hsr_status guide_upload(hsr_guide *g);
+static hsr_status guide_queue_stats(hsr_guide *g);
 
 static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
     const hsr_rf_config &rf = sc->rfConfig;
@@ -385,7 +386,7 @@ static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
 // schedule call instead of replacing them.
 static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
                            int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
-                           const volatile uint8_t *interruptFlag, bool accumulate) {
+                           const volatile uint8_t *interruptFlag, bool accumulate, bool queueStats = false) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured (hsr_hip_configure_frame)");
     if (budget < 0 || sampleIndexOffset < 0) return fail(HSR_ERR_INVALID, "negative budget or offset");
@@ -419,6 +420,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         hsr_status s = guide_upload(guide);
         if (s != HSR_OK) return s;
     }
+    if (guide) guide->statsPending = false;  // this run adds records: a queued download is stale
 
     const int N = b->fc.samplesPerChirp, C = b->fc.channelCount;
     const int MD = 2 * N * C;  // doubles of one chirp's frame (one LDS tile / partial)
@@ -604,6 +606,11 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, MD, b->d_stage_sum, b->d_stage_comp, S,
                                        b->d_frame_sum, b->d_frame_comp, b->stream));
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
+        // the caller steps the guide next: its statistics' download goes with the last batch
+        if (queueStats && guide && tx == T - 1 && done + n >= budget) {
+            hsr_status s = guide_queue_stats(guide);
+            if (s != HSR_OK) return s;
+        }
         HSR_HIP(hipEventSynchronize(b->ev[3]));
         float ms1 = 0.0f, ms2 = 0.0f;
         HSR_HIP(hipEventElapsedTime(&ms1, b->ev[0], b->ev[1]));
@@ -826,26 +833,58 @@ hsr_status hsr_hip_get_hit_record(hsr_backend *b, int32_t *hit_ids, int8_t *nee,
 }
 
 // ---------------------------------------------------------------- guiding
+// The trees go to the device asynchronously on the backend's stream, after every kernel
+// queued there before (which may still read the previous trees): they are packed into the
+// pinned staging area (once the previous upload has left it) and copied from there, and the
+// statistics' copies are cleared by a kernel. The host returns at once.
 static hsr_status guide_upload(hsr_guide *g) {
     const hsr::GuideTree &s = g->host.sampling, &t = g->host.training;
     if (s.size() > g->capacity || t.size() > g->capacity) return fail(HSR_ERR_INVALID, "guide tree too large");
-    HSR_HIP(hipStreamSynchronize(g->backend->stream));
-    HSR_HIP(hipMemcpy(g->d_sChild, s.child.data(), s.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HSR_HIP(hipMemcpy(g->d_sP0, s.p0.data(), s.size() * sizeof(float), hipMemcpyHostToDevice));
-    HSR_HIP(hipMemcpy(g->d_tChild, t.child.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HSR_HIP(hipMemset(g->d_tStat, 0, (size_t)hsr::kStatCopies * g->capacity * sizeof(double)));
-    HSR_HIP(hipMemcpy(g->d_tStat, t.stat.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    hipStream_t st = g->backend->stream;
+    const size_t cap = (size_t)g->capacity;
+    int32_t *sChild = (int32_t *)g->h_stage;
+    float *sP0 = (float *)(g->h_stage + 4 * cap);
+    int32_t *tChild = (int32_t *)(g->h_stage + 8 * cap);
+    double *tStat = (double *)(g->h_stage + 12 * cap);
+    HSR_HIP(hipEventSynchronize(g->evUp));
+    std::memcpy(sChild, s.child.data(), s.size() * sizeof(int32_t));
+    std::memcpy(sP0, s.p0.data(), s.size() * sizeof(float));
+    std::memcpy(tChild, t.child.data(), t.size() * sizeof(int32_t));
+    std::memcpy(tStat, t.stat.data(), t.size() * sizeof(double));
+    HSR_HIP(hsr::launch_zero(g->d_tStat, (int64_t)hsr::kStatCopies * g->capacity, st));
+    HSR_HIP(hipMemcpyAsync(g->d_sChild, sChild, s.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HSR_HIP(hipMemcpyAsync(g->d_sP0, sP0, s.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    HSR_HIP(hipMemcpyAsync(g->d_tChild, tChild, t.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HSR_HIP(hipMemcpyAsync(g->d_tStat, tStat, t.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HSR_HIP(hipEventRecord(g->evUp, st));
     g->dirty = false;
+    g->statsPending = false;
     return HSR_OK;
 }
 
-// the training statistics: the kStatCopies device copies folded into the first, in order
-static hsr_status guide_download_stats(hsr_guide *g) {
-    hsr::GuideTree &t = g->host.training;
+// Queues the training statistics' download: the kStatCopies device copies folded into the
+// first, in order, then copied into the pinned h_stats (complete at evDl).
+static hsr_status guide_queue_stats(hsr_guide *g) {
+    const hsr::GuideTree &t = g->host.training;
     hipStream_t s = g->backend->stream;
     HSR_HIP(hsr::launch_stat_fold(g->d_tStat, g->capacity, (int64_t)t.size(), s));
-    HSR_HIP(hipMemcpyAsync(t.stat.data(), g->d_tStat, t.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HSR_HIP(hipStreamSynchronize(s));
+    HSR_HIP(hipMemcpyAsync(g->h_stats, g->d_tStat, t.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HSR_HIP(hipEventRecord(g->evDl, s));
+    g->statsPending = true;
+    return HSR_OK;
+}
+
+// The training statistics into the host tree: the download a run has queued at its end, or a
+// new one.
+static hsr_status guide_download_stats(hsr_guide *g) {
+    hsr::GuideTree &t = g->host.training;
+    if (!g->statsPending) {
+        hsr_status s = guide_queue_stats(g);
+        if (s != HSR_OK) return s;
+    }
+    HSR_HIP(hipEventSynchronize(g->evDl));
+    std::memcpy(t.stat.data(), g->h_stats, t.size() * sizeof(double));
+    g->statsPending = false;
     return HSR_OK;
 }
 
@@ -866,6 +905,11 @@ hsr_status hsr_guide_create(hsr_backend *b, float uniformProb, float splitThresh
     if ((s = dalloc(&g->d_sP0, g->capacity)) != HSR_OK) return s;
     if ((s = dalloc(&g->d_tChild, g->capacity)) != HSR_OK) return s;
     if ((s = dalloc(&g->d_tStat, (size_t)hsr::kStatCopies * g->capacity)) != HSR_OK) return s;
+    const size_t cap = (size_t)g->capacity;
+    HSR_HIP(hipHostMalloc((void **)&g->h_stage, 20 * cap, hipHostMallocDefault));
+    HSR_HIP(hipHostMalloc((void **)&g->h_stats, 8 * cap, hipHostMallocDefault));
+    HSR_HIP(hipEventCreateWithFlags(&g->evUp, hipEventDisableTiming));
+    HSR_HIP(hipEventCreateWithFlags(&g->evDl, hipEventDisableTiming));
     g->host.reset();
     if ((s = guide_upload(g.get())) != HSR_OK) return s;
     *out = g.release();
@@ -875,6 +919,13 @@ hsr_status hsr_guide_create(hsr_backend *b, float uniformProb, float splitThresh
 void hsr_guide_destroy(hsr_guide *g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
+    // the last upload or download may still use the buffers
+    if (g->evUp) (void)hipEventSynchronize(g->evUp);
+    if (g->evDl) (void)hipEventSynchronize(g->evDl);
+    if (g->evUp) (void)hipEventDestroy(g->evUp);
+    if (g->evDl) (void)hipEventDestroy(g->evDl);
+    if (g->h_stage) (void)hipHostFree(g->h_stage);
+    if (g->h_stats) (void)hipHostFree(g->h_stats);
     dfree(g->d_sChild);
     dfree(g->d_sP0);
     dfree(g->d_tChild);
@@ -892,7 +943,6 @@ hsr_status hsr_guide_reset(hsr_guide *g) {
 hsr_status hsr_guide_step(hsr_guide *g) {
     if (!g) return fail(HSR_ERR_INVALID, "null guide");
     HSR_HIP(hipSetDevice(g->backend->device));
-    HSR_HIP(hipStreamSynchronize(g->backend->stream));
     hsr_status s = guide_download_stats(g);
     if (s != HSR_OK) return s;
     g->host.step();
@@ -941,7 +991,6 @@ hsr_status hsr_guide_training_size(hsr_guide *g, int32_t *n) {
 hsr_status hsr_guide_get_training(hsr_guide *g, int32_t *child, double *stat, int32_t capacity) {
     if (!g) return fail(HSR_ERR_INVALID, "null guide");
     HSR_HIP(hipSetDevice(g->backend->device));
-    HSR_HIP(hipStreamSynchronize(g->backend->stream));
     hsr_status s = guide_download_stats(g);
     if (s != HSR_OK) return s;
     const hsr::GuideTree &t = g->host.training;
@@ -967,7 +1016,8 @@ hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr
     int64_t milestone = 16384, remaining = samples;
     while (true) {
         milestone = std::min(milestone, remaining);
-        s = run_impl(b, scene, knobs, offset, milestone, weight, isFinal, guide, interruptFlag, true);
+        s = run_impl(b, scene, knobs, offset, milestone, weight, isFinal, guide, interruptFlag, true,
+                     remaining > milestone);
         if (s != HSR_OK) return s;
         offset += milestone;
         remaining -= milestone;
@@ -1001,7 +1051,8 @@ hsr_status hsr_path_tracer_run_schedule(hsr_backend *b, const hsr_scene *scene, 
         if ((s = hsr_guide_reset(guide)) != HSR_OK) return s;
         for (int p = 0; p < trainingPasses; ++p) {
             if ((s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
-            if ((s = run_impl(b, scene, knobs, 0, trainingPaths, 1.0f, 0, guide, interruptFlag, true)) != HSR_OK)
+            if ((s = run_impl(b, scene, knobs, 0, trainingPaths, 1.0f, 0, guide, interruptFlag, true, true)) !=
+                HSR_OK)
                 return s;
             if ((s = hsr_guide_step(guide)) != HSR_OK) return s;
         }

@@ -50,6 +50,13 @@ struct hsr_guide {
     double *d_tStat = nullptr;
     int32_t capacity = 1 << 16;
     bool dirty = true;  // host trees newer than the device copies
+    // pinned host staging: uploads [sampling child | sampling p0 | training child | training
+    // stat] (capacity entries each) and the downloaded training statistics; uploads are
+    // asynchronous on the backend's stream, evUp marks when the staging area is free again
+    char *h_stage = nullptr;
+    double *h_stats = nullptr;
+    hipEvent_t evUp = nullptr, evDl = nullptr;
+    bool statsPending = false;  // a run has queued the statistics' download into h_stats (evDl)
 };
 
 struct hsr_backend {
