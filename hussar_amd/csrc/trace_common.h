@@ -851,6 +851,68 @@ __device__ inline void tile_flush(const double *tile, double2 *dst, int N, int C
     }
 }
 
+// The guard-band tile of the wavefront gather (N >= 16): each channel row holds N + 32 bins,
+// bin b in [-16, N + 16) at row offset 16 + b, real parts then imaginary parts (rows of
+// channel c at c * (N + 32) and (N + 32) * C + c * (N + 32)). An event's 33 taps around a
+// centre in [0, N) then never wrap inside the splat; tile_flush_g adds the 16 guard bins on
+// each side into bins b mod N.
+constexpr int kGuard = 16;
+// 0: the compiler unrolls the 33 taps and keeps the lane's tap offsets and their float values in
+// registers across events (fewer instructions per tap, more registers)
+#ifndef HSR_TAP_UNROLL
+#define HSR_TAP_UNROLL 0
+#endif
+__device__ inline int guard_row(int N) { return N + 2 * kGuard; }
+
+// flush_events into the guard-band tile: the same taps, weights and lane rotation, with the
+// wrap of each tap replaced by the guard bins.
+__device__ inline void flush_events_g(bool has, const Event &e, int ch, double *tile, int N, int C, int lane) {
+    if (!has) return;
+    const int R = guard_row(N);
+    int c0 = e.center;  // in [0, N]: round(N * frac) reaches N
+    if (c0 >= N) c0 -= N;
+    double *re = tile + ch * R + kGuard + c0, *im = tile + R * C + ch * R + kGuard + c0;
+    if (e.delta) {
+        atomicAdd(re, (double)e.A.re);
+        atomicAdd(im, (double)e.A.im);
+        return;
+    }
+    int k = lane % 33 - 16;
+#if HSR_TAP_UNROLL > 0
+#pragma unroll HSR_TAP_UNROLL
+#endif
+    for (int it = 0; it < 33; ++it) {
+        const float coef = __builtin_amdgcn_rcpf(e.s - (float)k);
+        atomicAdd(&re[k], (double)(e.A.re * coef));
+        atomicAdd(&im[k], (double)(e.A.im * coef));
+        k = k == 16 ? -16 : k + 1;
+    }
+}
+
+// Adds the guard-band tile into the workgroup's partial frame [N][C] (complex, frame order):
+// bin b gets its own entry, the right guard entry of bin b + N (b < 16) and the left guard
+// entry of bin b - N (b >= N - 16).
+__device__ inline void tile_flush_g(const double *tile, double2 *dst, int N, int C, int tid, int threads) {
+    const int NC = N * C, R = guard_row(N), RC = R * C;
+    for (int j = tid; j < NC; j += threads) {
+        const int b = j / C, c = j % C;
+        const int t = c * R + kGuard + b;
+        double re = tile[t], im = tile[RC + t];
+        if (b < kGuard) {
+            re += tile[t + N];
+            im += tile[RC + t + N];
+        }
+        if (b >= N - kGuard) {
+            re += tile[t - N];
+            im += tile[RC + t - N];
+        }
+        double2 v = dst[j];
+        v.x += re;
+        v.y += im;
+        dst[j] = v;
+    }
+}
+
 // Wave-cooperative splat of this lane's event (if any) into the frame cube in device memory
 // [M][N][C] with FP64 atomics: up to 33 x 33 taps per event, spread over the wave's lanes
 // (tap weights with the hardware reciprocal, 1 ulp).

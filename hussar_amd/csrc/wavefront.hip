@@ -773,13 +773,17 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a
 // ------------------------------------------------------------------------ gather + splat
 __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, int depth, double2 *partials,
                                                         bool count) {
-    extern __shared__ double tile[];  // one chirp's N*C complex doubles (tile_re); unused with M > 1
+    // one chirp's frame tile in LDS: the guard-band layout (flush_events_g) for N >= 16, else
+    // N*C complex doubles (tile_re); unused with M > 1
+    extern __shared__ double tile[];
     const int NC = a.N * a.C;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const bool cube = a.cube != 0;  // uniform: splat into the frame cube in device memory
+    const bool guard = a.N >= 2 * kGuard;
     if (!cube) {
-        for (int j = tid; j < 2 * NC; j += kWfBlock) tile[j] = 0.0;
+        const int tileDoubles = 2 * (guard ? guard_row(a.N) : a.N) * a.C;
+        for (int j = tid; j < tileDoubles; j += kWfBlock) tile[j] = 0.0;
         __syncthreads();
     }
     const int64_t wave = ((int64_t)blockIdx.x * kWfBlock + tid) >> 6;
@@ -833,6 +837,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
             }
             if (cube)
                 cube_event(has, ev, a.chBase + ch, a, lane);
+            else if (guard)
+                flush_events_g(has, ev, a.chBase + ch, tile, a.N, a.C, lane);
             else
                 flush_events(has, ev, a.chBase + ch, tile, a.N, a.C, lane);
         }
@@ -846,7 +852,10 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, i
     if (count) wave_count(&a.counters[4], cev, lane);
     if (cube) return;
     __syncthreads();
-    tile_flush(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
+    if (guard)
+        tile_flush_g(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
+    else
+        tile_flush(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
 }
 
 // ------------------------------------------------------------------------ cube splat
@@ -994,7 +1003,9 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.cellStart = (int32_t *)take(sizeof(int32_t) * (kCells + 1));
 }
 
-size_t gather_lds_bytes(int N, int C) { return (size_t)N * C * 2 * sizeof(double); }
+size_t gather_lds_bytes(int N, int C) {
+    return (size_t)(N >= 2 * kGuard ? N + 2 * kGuard : N) * C * 2 * sizeof(double);
+}
 
 std::vector<int32_t> coherence_cells() {
     // cell (u, v): the first 3 base-5 digits and the first 5 base-3 digits of the primary
@@ -1095,20 +1106,25 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     for (int k = 0; k <= maxDepth; ++k) {
         // the depth's closest-hit traversal on s2, beside its shadow and gather stages: it
         // reads the extension queue and rays of depth k and writes only the hits, which the
-        // shading stage reads after the join
+        // shading stage reads after the join. A depth without those stages (depth 0 when the
+        // direct paths are not traced) keeps it on s: no fork and join to wait for.
+        const bool nee = k >= 1 || !a.onlyIndirect;
+        const hipStream_t ts = nee ? s2 : s;
         if (k < maxDepth) {
-            (void)hipEventRecord(evFork[k], s);
-            (void)hipStreamWaitEvent(s2, evFork[k], 0);
-            (void)hipEventRecord(evClosest[2 * *nClosest], s2);
+            if (nee) {
+                (void)hipEventRecord(evFork[k], s);
+                (void)hipStreamWaitEvent(s2, evFork[k], 0);
+            }
+            (void)hipEventRecord(evClosest[2 * *nClosest], ts);
             if (count)
-                hipLaunchKernelGGL((k_wf_trav<true>), dim3(gClosest), dim3(kWfBlock), 0, s2, a, w, k);
+                hipLaunchKernelGGL((k_wf_trav<true>), dim3(gClosest), dim3(kWfBlock), 0, ts, a, w, k);
             else
-                hipLaunchKernelGGL((k_wf_trav<false>), dim3(gClosest), dim3(kWfBlock), 0, s2, a, w, k);
-            (void)hipEventRecord(evClosest[2 * *nClosest + 1], s2);
+                hipLaunchKernelGGL((k_wf_trav<false>), dim3(gClosest), dim3(kWfBlock), 0, ts, a, w, k);
+            (void)hipEventRecord(evClosest[2 * *nClosest + 1], ts);
             ++*nClosest;
-            (void)hipEventRecord(evJoin[k], s2);
+            if (nee) (void)hipEventRecord(evJoin[k], s2);
         }
-        if (k >= 1 || !a.onlyIndirect) {
+        if (nee) {
             if (!a.poDiffraction) {
                 (void)hipEventRecord(evShadow[2 * *nShadow], s);
                 if (count)
@@ -1121,7 +1137,7 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
             hipLaunchKernelGGL(k_wf_gather, dim3(gridGather), dim3(kWfBlock), glds, s, a, w, k, partials, count);
         }
         if (k < maxDepth) {
-            (void)hipStreamWaitEvent(s, evJoin[k], 0);
+            if (nee) (void)hipStreamWaitEvent(s, evJoin[k], 0);
             hipLaunchKernelGGL(k_wf_shade, dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
         }
     }
