@@ -456,8 +456,10 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
 // own exact any-hit result while the node fetches are shared.
 struct Packet {
     f3 o;
-    f3 d[kPacket], invd[kPacket];
+    f3 d[kPacket];
     float tfar[kPacket];
+    f3 ivlo, ivhi;   // per axis the least and greatest reciprocal direction of the rays
+    float tmax;      // the greatest extent
     uint32_t alive;  // rays not yet occluded
     int32_t node;
     bool same;       // all rays share their direction signs, given by sgn (bit k: axis k >= 0)
@@ -504,31 +506,37 @@ __device__ inline void packet_run(const float4 *__restrict__ nodes, const float4
             }
             uint32_t m = 0;
             if (pk.same) {
-                // the direction signs pick each axis' near and far face for every ray: the
-                // same products as the min/max form, without the min/max
+                // one interval test for the whole packet: the direction signs pick each axis'
+                // near and far face, and a face's distance over the packet's reciprocal
+                // directions [ivlo, ivhi] lies between its two products. The entry is bounded
+                // below and the exit above by those of every ray (rounding is monotonic), so
+                // a child any ray enters within its extent passes; the leaves test each ray.
                 const bool sx = pk.sgn & 1u, sy = pk.sgn & 2u, sz = pk.sgn & 4u;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float nx = sx ? dl[i][0] : dl[i][1], fx = sx ? dl[i][1] : dl[i][0];
                     const float ny = sy ? dl[i][2] : dl[i][3], fy = sy ? dl[i][3] : dl[i][2];
                     const float nz = sz ? dl[i][4] : dl[i][5], fz = sz ? dl[i][5] : dl[i][4];
-#pragma unroll
-                    for (int r = 0; r < kPacket; ++r) {
-                        const f3 iv = pk.invd[r];
-                        const float t0 = fmaxf(fmaxf(nx * iv.x, ny * iv.y), fmaxf(nz * iv.z, kEps));
-                        const float t1 = fminf(fminf(fx * iv.x, fy * iv.y), fz * iv.z) * 1.0000004f;
-                        if ((pk.alive & (1u << r)) && t0 <= t1 && t0 <= pk.tfar[r]) m |= 1u << i;
-                    }
+                    const float tnx = fminf(nx * pk.ivlo.x, nx * pk.ivhi.x);
+                    const float tny = fminf(ny * pk.ivlo.y, ny * pk.ivhi.y);
+                    const float tnz = fminf(nz * pk.ivlo.z, nz * pk.ivhi.z);
+                    const float tfx = fmaxf(fx * pk.ivlo.x, fx * pk.ivhi.x);
+                    const float tfy = fmaxf(fy * pk.ivlo.y, fy * pk.ivhi.y);
+                    const float tfz = fmaxf(fz * pk.ivlo.z, fz * pk.ivhi.z);
+                    const float t0 = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, kEps));
+                    const float t1 = fminf(fminf(tfx, tfy), tfz) * 1.0000004f;
+                    if (t0 <= t1 && t0 <= pk.tmax) m |= 1u << i;
                 }
             } else {
 #pragma unroll
                 for (int r = 0; r < kPacket; ++r) {
                     if (pk.alive & (1u << r)) {
+                        const f3 iv = {safe_rcp(pk.d[r].x), safe_rcp(pk.d[r].y), safe_rcp(pk.d[r].z)};
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
                             float e;
-                            if (slab_diff(dl[i][0], dl[i][1], dl[i][2], dl[i][3], dl[i][4], dl[i][5], pk.invd[r],
-                                          pk.tfar[r], e))
+                            if (slab_diff(dl[i][0], dl[i][1], dl[i][2], dl[i][3], dl[i][4], dl[i][5], iv, pk.tfar[r],
+                                          e))
                                 m |= 1u << i;
                         }
                     }
@@ -595,21 +603,27 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
                 uint32_t sg[kPacket];
                 const float4 o = w.ray_o[i];
                 pk.o = {o.x, o.y, o.z};
+                const float inf = __builtin_inff();
+                pk.ivlo = {inf, inf, inf};
+                pk.ivhi = {-inf, -inf, -inf};
+                pk.tmax = 0.0f;
 #pragma unroll
                 for (int r = 0; r < kPacket; ++r) {
+                    f3 iv = {0.0f, 0.0f, 1.0f};
                     if (r < rays) {
                         const float4 d = w.s_d[slot0 + r];
                         pk.d[r] = {d.x, d.y, d.z};
-                        pk.invd[r] = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
+                        iv = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
                         pk.tfar[r] = d.w;
                         pk.alive |= 1u << r;
+                        pk.ivlo = {fminf(pk.ivlo.x, iv.x), fminf(pk.ivlo.y, iv.y), fminf(pk.ivlo.z, iv.z)};
+                        pk.ivhi = {fmaxf(pk.ivhi.x, iv.x), fmaxf(pk.ivhi.y, iv.y), fmaxf(pk.ivhi.z, iv.z)};
+                        pk.tmax = fmaxf(pk.tmax, d.w);
                     } else {
                         pk.d[r] = {0.0f, 0.0f, 1.0f};
-                        pk.invd[r] = {0.0f, 0.0f, 1.0f};
                         pk.tfar[r] = 0.0f;
                     }
-                    sg[r] = (pk.invd[r].x >= 0.0f ? 1u : 0u) | (pk.invd[r].y >= 0.0f ? 2u : 0u) |
-                            (pk.invd[r].z >= 0.0f ? 4u : 0u);
+                    sg[r] = (iv.x >= 0.0f ? 1u : 0u) | (iv.y >= 0.0f ? 2u : 0u) | (iv.z >= 0.0f ? 4u : 0u);
                 }
                 pk.sgn = sg[0];
                 pk.same = true;
