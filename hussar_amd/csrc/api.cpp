@@ -310,8 +310,7 @@ hsr_status hsr_hip_clear_frame(hsr_backend *b) {
     if (!b) return fail(HSR_ERR_INVALID, "null backend");
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured");
     HSR_HIP(hipSetDevice(b->device));
-    HSR_HIP(hsr::launch_zero(b->d_frame_sum, b->frameDoubles, b->stream));
-    HSR_HIP(hsr::launch_zero(b->d_frame_comp, b->frameDoubles, b->stream));
+    HSR_HIP(hsr::launch_zero2(b->d_frame_sum, b->d_frame_comp, b->frameDoubles, b->stream));
     b->totalWeight = 0.0;
     return HSR_OK;
 }
@@ -363,6 +362,15 @@ hsr_status hsr_hip_last_stats(hsr_backend *b, hsr_run_stats *out) {
 
 static hsr_status guide_upload(hsr_guide *g);
 static hsr_status guide_queue_stats(hsr_guide *g);
+
+// Waits for an event by polling it: a blocking wait sleeps and wakes tens of microseconds late,
+// and the host's next step (the guide, the next batch) waits on the answer.
+static hipError_t event_wait(hipEvent_t e) {
+    hipError_t r;
+    while ((r = hipEventQuery(e)) == hipErrorNotReady) {
+    }
+    return r;
+}
 
 static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
     const hsr_rf_config &rf = sc->rfConfig;
@@ -611,7 +619,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             hsr_status s = guide_queue_stats(guide);
             if (s != HSR_OK) return s;
         }
-        HSR_HIP(hipEventSynchronize(b->ev[3]));
+        HSR_HIP(event_wait(b->ev[3]));
         float ms1 = 0.0f, ms2 = 0.0f;
         HSR_HIP(hipEventElapsedTime(&ms1, b->ev[0], b->ev[1]));
         HSR_HIP(hipEventElapsedTime(&ms2, b->ev[1], b->ev[3]));
@@ -834,29 +842,32 @@ hsr_status hsr_hip_get_hit_record(hsr_backend *b, int32_t *hit_ids, int8_t *nee,
 
 // ---------------------------------------------------------------- guiding
 // The trees go to the device asynchronously on the backend's stream, after every kernel
-// queued there before (which may still read the previous trees): they are packed into the
-// pinned staging area (once the previous upload has left it) and copied from there, and the
-// statistics' copies are cleared by a kernel. The host returns at once.
+// queued there before (which may still read the previous trees): packed into the pinned
+// staging area (once the previous upload has left it) as child[ns] | p0[ns] | child[nt] and
+// copied in one piece, the statistics' copies cleared by a kernel (and the host's statistics
+// copied only when one is nonzero). The host returns at once.
 static hsr_status guide_upload(hsr_guide *g) {
     const hsr::GuideTree &s = g->host.sampling, &t = g->host.training;
     if (s.size() > g->capacity || t.size() > g->capacity) return fail(HSR_ERR_INVALID, "guide tree too large");
     hipStream_t st = g->backend->stream;
-    const size_t cap = (size_t)g->capacity;
-    int32_t *sChild = (int32_t *)g->h_stage;
-    float *sP0 = (float *)(g->h_stage + 4 * cap);
-    int32_t *tChild = (int32_t *)(g->h_stage + 8 * cap);
+    const size_t cap = (size_t)g->capacity, ns = (size_t)s.size(), nt = (size_t)t.size();
     double *tStat = (double *)(g->h_stage + 12 * cap);
     HSR_HIP(hipEventSynchronize(g->evUp));
-    std::memcpy(sChild, s.child.data(), s.size() * sizeof(int32_t));
-    std::memcpy(sP0, s.p0.data(), s.size() * sizeof(float));
-    std::memcpy(tChild, t.child.data(), t.size() * sizeof(int32_t));
-    std::memcpy(tStat, t.stat.data(), t.size() * sizeof(double));
+    std::memcpy(g->h_stage, s.child.data(), ns * sizeof(int32_t));
+    std::memcpy(g->h_stage + 4 * ns, s.p0.data(), ns * sizeof(float));
+    std::memcpy(g->h_stage + 8 * ns, t.child.data(), nt * sizeof(int32_t));
+    bool statsZero = true;
+    for (size_t i = 0; i < nt && statsZero; ++i) statsZero = t.stat[i] == 0.0;
     HSR_HIP(hsr::launch_zero(g->d_tStat, (int64_t)hsr::kStatCopies * g->capacity, st));
-    HSR_HIP(hipMemcpyAsync(g->d_sChild, sChild, s.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HSR_HIP(hipMemcpyAsync(g->d_sP0, sP0, s.size() * sizeof(float), hipMemcpyHostToDevice, st));
-    HSR_HIP(hipMemcpyAsync(g->d_tChild, tChild, t.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HSR_HIP(hipMemcpyAsync(g->d_tStat, tStat, t.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HSR_HIP(hipMemcpyAsync(g->d_trees, g->h_stage, 8 * ns + 4 * nt, hipMemcpyHostToDevice, st));
+    if (!statsZero) {
+        std::memcpy(tStat, t.stat.data(), nt * sizeof(double));
+        HSR_HIP(hipMemcpyAsync(g->d_tStat, tStat, nt * sizeof(double), hipMemcpyHostToDevice, st));
+    }
     HSR_HIP(hipEventRecord(g->evUp, st));
+    g->d_sChild = (int32_t *)g->d_trees;
+    g->d_sP0 = (float *)(g->d_trees + 4 * ns);
+    g->d_tChild = (int32_t *)(g->d_trees + 8 * ns);
     g->dirty = false;
     g->statsPending = false;
     return HSR_OK;
@@ -882,7 +893,7 @@ static hsr_status guide_download_stats(hsr_guide *g) {
         hsr_status s = guide_queue_stats(g);
         if (s != HSR_OK) return s;
     }
-    HSR_HIP(hipEventSynchronize(g->evDl));
+    HSR_HIP(event_wait(g->evDl));
     std::memcpy(t.stat.data(), g->h_stats, t.size() * sizeof(double));
     g->statsPending = false;
     return HSR_OK;
@@ -901,9 +912,7 @@ hsr_status hsr_guide_create(hsr_backend *b, float uniformProb, float splitThresh
     g->host.splitThreshold = splitThreshold;
     g->host.secondMoment = secondMoment != 0;
     hsr_status s;
-    if ((s = dalloc(&g->d_sChild, g->capacity)) != HSR_OK) return s;
-    if ((s = dalloc(&g->d_sP0, g->capacity)) != HSR_OK) return s;
-    if ((s = dalloc(&g->d_tChild, g->capacity)) != HSR_OK) return s;
+    if ((s = dalloc(&g->d_trees, (size_t)12 * g->capacity)) != HSR_OK) return s;
     if ((s = dalloc(&g->d_tStat, (size_t)hsr::kStatCopies * g->capacity)) != HSR_OK) return s;
     const size_t cap = (size_t)g->capacity;
     HSR_HIP(hipHostMalloc((void **)&g->h_stage, 20 * cap, hipHostMallocDefault));
@@ -926,9 +935,7 @@ void hsr_guide_destroy(hsr_guide *g) {
     if (g->evDl) (void)hipEventDestroy(g->evDl);
     if (g->h_stage) (void)hipHostFree(g->h_stage);
     if (g->h_stats) (void)hipHostFree(g->h_stats);
-    dfree(g->d_sChild);
-    dfree(g->d_sP0);
-    dfree(g->d_tChild);
+    dfree(g->d_trees);
     dfree(g->d_tStat);
     delete g;
 }

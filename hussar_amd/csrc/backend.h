@@ -44,7 +44,8 @@ struct hsr_guide {
     hsr_backend *backend = nullptr;  // used only while the backend is alive (runs, steps)
     int device = 0;                  // own copy: destroy must not touch the backend
     hsr::GuideHost host;
-    int32_t *d_sChild = nullptr;
+    char *d_trees = nullptr;      // the sampling tree and the training tree's children, packed
+    int32_t *d_sChild = nullptr;  // into d_trees at the last upload: child[ns] | p0[ns] | child[nt]
     float *d_sP0 = nullptr;
     int32_t *d_tChild = nullptr;
     double *d_tStat = nullptr;

@@ -168,6 +168,7 @@ size_t gather_lds_bytes(int N, int C);
 hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum, double *stage_comp, int S,
                          double *frame_sum, double *frame_comp, hipStream_t stream);
 hipError_t launch_zero(double *p, int64_t n, hipStream_t stream);
+hipError_t launch_zero2(double *p, double *q, int64_t n, hipStream_t stream);  // both arrays
 // Folds the kStatCopies copies of n training statistics (stride apart) into the first.
 hipError_t launch_stat_fold(double *stat, int64_t stride, int64_t n, hipStream_t stream);
 // Cube frames: chirps per LDS tile of one channel (0: a chirp of N samples does not fit).

@@ -184,6 +184,14 @@ __global__ void k_zero(double *p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+__global__ void k_zero2(double *p, double *q, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = 0.0;
+        q[i] = 0.0;
+    }
+}
+
 // ------------------------------------------------------------------------ launchers
 size_t trace_lds_bytes(int N, int C) {
     return (size_t)N * C * 2 * sizeof(double) + (size_t)kStackShort * kTraceBlock * sizeof(int32_t);
@@ -237,6 +245,14 @@ hipError_t launch_stat_fold(double *stat, int64_t stride, int64_t n, hipStream_t
     if (n <= 0) return hipSuccess;
     (void)hipGetLastError();
     hipLaunchKernelGGL(k_stat_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, stat, stride, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero2(double *p, double *q, int64_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    (void)hipGetLastError();
+    const int bs = 256;
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, stream, p, q, n);
     return hipGetLastError();
 }
 
