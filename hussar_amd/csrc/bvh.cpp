@@ -18,7 +18,7 @@ namespace hsr {
 namespace {
 
 #ifndef HSR_SAH_BINS
-#define HSR_SAH_BINS 16
+#define HSR_SAH_BINS 64
 #endif
 #ifndef HSR_SAH_TRAV
 #define HSR_SAH_TRAV 0.5f
