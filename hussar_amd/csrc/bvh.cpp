@@ -269,13 +269,155 @@ int32_t collapse_node(const std::vector<float> &n2, int32_t node, std::vector<fl
     return me;
 }
 
+// SAH-optimal collapse (HSR_COLLAPSE_DP; off: on the vehicle workload it makes 12 % fewer
+// nodes and 1 % fewer node visits than the greedy collapse, but one level more, and the frame
+// takes 1 % longer). Dynamic programming over the BVH2, as in the wide-BVH collapse of
+// Ylitie, Karras and Laine 2017, here for four children). For a BVH2 subtree n and k in 1..4,
+// D(n, k) is the least SAH cost of covering n with at most k four-wide children: one child
+// (n itself as a four-wide node, or n's leaf) or its two subtrees covered by i + j <= k. A
+// four-wide node costs its area times kCollapseNode and the cheapest cover of its subtree
+// with up to four children; a leaf its area times its triangle count. Areas are those of the
+// boxes the parent holds for its children.
+#ifndef HSR_COLLAPSE_NODE
+#define HSR_COLLAPSE_NODE 1.5f
+#endif
+constexpr float kCollapseNode = HSR_COLLAPSE_NODE;  // four-wide node visit / triangle test
+
+struct Cover {
+    float cost[5];     // D(n, k), k = 1..4 (index 0 unused)
+    int8_t left[5];    // the left subtree's share of the cover with D(n, k), 0: n as one child
+};
+
+struct Collapser {
+    const std::vector<float> &n2;
+    std::vector<Cover> dp;      // per BVH2 node
+    std::vector<float> area;    // per BVH2 node: area of its box
+    std::vector<int8_t> split;  // per BVH2 node: the left share of its four-wide node's cover
+
+    static float box_area(const float lo[3], const float hi[3]) {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+    static int32_t leaf_count(int32_t ref) { return ((ref >> 27) & 0xF) + 1; }
+
+    // D(child, k) of child slot c of BVH2 node n
+    float child_cost(int32_t n, int c, int k) const {
+        float lo[3], hi[3];
+        int32_t ref;
+        child2(n2, n, c, lo, hi, ref);
+        if (ref < 0) return box_area(lo, hi) * (float)leaf_count(ref);
+        return dp[ref].cost[k];
+    }
+
+    void solve(int32_t n) {  // children are solved first (post-order)
+        for (int c = 0; c < 2; ++c) {
+            float lo[3], hi[3];
+            int32_t ref;
+            child2(n2, n, c, lo, hi, ref);
+            if (ref >= 0) {
+                area[ref] = box_area(lo, hi);
+                solve(ref);
+            }
+        }
+        Cover &d = dp[n];
+        // n as one four-wide node: its two subtrees covered by i + j <= 4 children
+        float best = std::numeric_limits<float>::infinity();
+        int8_t bi = 1;
+        for (int i = 1; i <= 3; ++i) {
+            const float v = child_cost(n, 0, i) + child_cost(n, 1, 4 - i);
+            if (v < best) {
+                best = v;
+                bi = (int8_t)i;
+            }
+        }
+        split[n] = bi;
+        d.cost[1] = area[n] * kCollapseNode + best;
+        d.left[1] = 0;
+        for (int k = 2; k <= 4; ++k) {
+            d.cost[k] = d.cost[k - 1];
+            d.left[k] = d.left[k - 1];
+            for (int i = 1; i < k; ++i) {
+                const float v = child_cost(n, 0, i) + child_cost(n, 1, k - i);
+                if (v < d.cost[k]) {
+                    d.cost[k] = v;
+                    d.left[k] = (int8_t)i;
+                }
+            }
+        }
+    }
+
+    // The four-wide children covering BVH2 child slot c of node n with at most k children.
+    void gather(int32_t n, int c, int k, float (*lo)[3], float (*hi)[3], int32_t *refs, bool *wide, int &m) const {
+        float blo[3], bhi[3];
+        int32_t ref;
+        child2(n2, n, c, blo, bhi, ref);
+        const int share = ref >= 0 ? dp[ref].left[k] : 0;
+        if (share == 0) {  // one child: a leaf or ref as a four-wide node
+            std::memcpy(lo[m], blo, sizeof(blo));
+            std::memcpy(hi[m], bhi, sizeof(bhi));
+            refs[m] = ref;
+            wide[m] = ref >= 0;
+            ++m;
+            return;
+        }
+        gather(ref, 0, share, lo, hi, refs, wide, m);
+        gather(ref, 1, k - share, lo, hi, refs, wide, m);
+    }
+
+    int32_t emit(int32_t n, std::vector<float> &n4, int depth, int &maxd) const {
+        float lo[4][3], hi[4][3];
+        int32_t refs[4];
+        bool wide[4];
+        int m = 0;
+        gather(n, 0, split[n], lo, hi, refs, wide, m);
+        gather(n, 1, 4 - split[n], lo, hi, refs, wide, m);
+        maxd = std::max(maxd, depth);
+        const int32_t me = (int32_t)(n4.size() / 32);
+        n4.resize(n4.size() + 32, 0.0f);
+        int32_t out[4];
+        for (int i = 0; i < 4; ++i) out[i] = i < m ? (wide[i] ? emit(refs[i], n4, depth + 1, maxd) : refs[i]) : kNoChild;
+        float *q = &n4[32 * (size_t)me];
+        for (int i = 0; i < 4; ++i) {
+            const bool used = i < m;
+            q[0 + i] = used ? lo[i][0] : 0.0f;
+            q[4 + i] = used ? hi[i][0] : 0.0f;
+            q[8 + i] = used ? lo[i][1] : 0.0f;
+            q[12 + i] = used ? hi[i][1] : 0.0f;
+            q[16 + i] = used ? lo[i][2] : 0.0f;
+            q[20 + i] = used ? hi[i][2] : 0.0f;
+            std::memcpy(&q[24 + i], &out[i], 4);
+        }
+        return me;
+    }
+};
+
 }  // namespace
 
 void collapse4(const std::vector<float> &n2, std::vector<float> &n4, int32_t &depth4) {
     n4.clear();
     n4.reserve(n2.size());
     int maxd = 0;
+#ifndef HSR_COLLAPSE_DP
     collapse_node(n2, 0, n4, 0, maxd);
+#else
+    Collapser cl{n2, {}, {}, {}};
+    const size_t count = n2.size() / 16;
+    cl.dp.resize(count);
+    cl.area.assign(count, 0.0f);
+    cl.split.assign(count, 1);
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    int32_t r0, r1;
+    child2(n2, 0, 0, lo0, hi0, r0);
+    child2(n2, 0, 1, lo1, hi1, r1);
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo0[k], lo1[k]);
+        hi[k] = std::max(hi0[k], hi1[k]);
+    }
+    cl.area[0] = Collapser::box_area(lo, hi);
+    cl.solve(0);
+    cl.emit(0, n4, 0, maxd);
+#endif
     depth4 = maxd;
 }
 

@@ -41,7 +41,7 @@ constexpr int kWfBlock = 256;
 #define HSR_TRAV_STEPS 16
 #endif
 #ifndef HSR_REFILL_MIN
-#define HSR_REFILL_MIN 16
+#define HSR_REFILL_MIN 32
 #endif
 // minimum resident waves per SIMD asked of the register allocator for the shading stages
 #ifndef HSR_HIT_WAVES
