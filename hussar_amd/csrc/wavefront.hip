@@ -63,7 +63,7 @@ constexpr int kMaxGroups = (kMaxChannels + kPacket - 1) / kPacket;  // shadow pa
 #define HSR_CLAIM 64
 #endif
 #ifndef HSR_STAGE_ITERS
-#define HSR_STAGE_ITERS 8
+#define HSR_STAGE_ITERS 4
 #endif
 constexpr unsigned kClaim = HSR_CLAIM;          // queue items a traversal wave claims per atomic
 constexpr int kStageIters = HSR_STAGE_ITERS;    // workgroup rounds per producer range
