@@ -79,7 +79,7 @@ def test_dihedral_pose_parity(oracle, dihedral, yaw):
 
 
 def test_dihedral_full_pose_size_vs_fp64(oracle, dihedral):
-    """The configs[0] pose size: 2^20 paths, depth 3, vs the FP64 oracle."""
+    """The dihedral example's per-pose size: 2^20 paths, depth 3, vs the FP64 oracle."""
     mesh, b, fc = dihedral
     om = oracle.OracleMesh.from_mesh(mesh)
     kn = H.Knobs(maxDepth=3, doGuiding=False)
