@@ -198,7 +198,9 @@ void hsr_guide_destroy(hsr_guide *guide);
 /* GuidingWrapper::reset (path.h:48-51): both trees back to a single uniform leaf. */
 hsr_status hsr_guide_reset(hsr_guide *guide);
 /* GuidingWrapper::step (path.h:119-126): build the training tree from its statistics,
- * copy it to the sampling tree, refine the training tree and clear its statistics. */
+ * copy it to the sampling tree, refine the training tree and clear its statistics. Returns
+ * once the statistics are read; the new trees reach the device asynchronously, on the
+ * backend's stream ahead of every later run (as do reset and set_sampling). */
 hsr_status hsr_guide_step(hsr_guide *guide);
 
 /* Inspection of a guide (no reference counterpart; for tests and persistence).
