@@ -53,6 +53,70 @@ struct Box {
     }
 };
 
+// Early split clipping (Ernst and Greiner 2007): a triangle whose box is large next to the
+// mean triangle box is entered in the build as several references, each with the box of the
+// triangle's part inside one half of its box (split at the middle of the longest axis,
+// recursively), so that thin or tilted triangles do not inflate the boxes above them. A leaf
+// holds references, and its slots the referenced triangles (whole, so every test is the
+// exact triangle test; a triangle met through two leaves gives the same t and id).
+#ifndef HSR_PRESPLIT
+#define HSR_PRESPLIT 4.0
+#endif
+constexpr double kPresplit = HSR_PRESPLIT;  // split while box area > kPresplit x the mean (0: off)
+constexpr int kPresplitMaxRefs = 8;         // references per triangle at most
+constexpr double kPresplitBudget = 0.5;     // extra references at most, per triangle
+
+// Box of the part of triangle (a, b, c) inside box [lo, hi] (Sutherland-Hodgman clipping in
+// double, bounds rounded outwards); false when that part is empty.
+bool clipped_box(const double a[3], const double b[3], const double c[3], const double lo[3], const double hi[3],
+                 float olo[3], float ohi[3]) {
+    double poly[12][3], tmp[12][3];
+    int n = 3;
+    for (int k = 0; k < 3; ++k) {
+        poly[0][k] = a[k];
+        poly[1][k] = b[k];
+        poly[2][k] = c[k];
+    }
+    for (int axis = 0; axis < 3 && n > 0; ++axis) {
+        for (int side = 0; side < 2 && n > 0; ++side) {
+            const double plane = side ? hi[axis] : lo[axis];
+            auto inside = [&](const double *p) { return side ? p[axis] <= plane : p[axis] >= plane; };
+            int m = 0;
+            for (int i = 0; i < n; ++i) {
+                const double *P = poly[i], *Q = poly[(i + 1) % n];
+                const bool pin = inside(P), qin = inside(Q);
+                if (pin && m < 12) {
+                    for (int k = 0; k < 3; ++k) tmp[m][k] = P[k];
+                    ++m;
+                }
+                if (pin != qin && m < 12) {
+                    const double t = (plane - P[axis]) / (Q[axis] - P[axis]);
+                    for (int k = 0; k < 3; ++k) tmp[m][k] = P[k] + t * (Q[k] - P[k]);
+                    tmp[m][axis] = plane;
+                    ++m;
+                }
+            }
+            n = m;
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) poly[i][k] = tmp[i][k];
+        }
+    }
+    if (n == 0) return false;
+    for (int k = 0; k < 3; ++k) {
+        double l = poly[0][k], h = poly[0][k];
+        for (int i = 1; i < n; ++i) {
+            l = std::min(l, poly[i][k]);
+            h = std::max(h, poly[i][k]);
+        }
+        // the clipped part lies within the triangle's own box: never beyond it
+        l = std::max(l, std::min({a[k], b[k], c[k]}));
+        h = std::min(h, std::max({a[k], b[k], c[k]}));
+        olo[k] = std::nextafter((float)l, -std::numeric_limits<float>::infinity());
+        ohi[k] = std::nextafter((float)h, std::numeric_limits<float>::infinity());
+    }
+    return true;
+}
+
 struct Builder {
     const float *V;
     const int32_t *I;
@@ -493,9 +557,8 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
     Builder B;
     B.V = V;
     B.I = I;
-    B.tbox.resize(nt);
-    B.cen.resize(3 * nt);
-    B.order.resize(nt);
+    std::vector<Box> tb(nt);
+    double meanArea = 0.0;
     for (int64_t t = 0; t < nt; ++t) {
         Box bx;
         bx.reset();
@@ -506,15 +569,86 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
                 bx.hi[k] = std::max(bx.hi[k], p[k]);
             }
         }
-        B.tbox[t] = bx;
-        for (int k = 0; k < 3; ++k) B.cen[3 * t + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
-        B.order[t] = (int32_t)t;
+        tb[t] = bx;
+        meanArea += bx.area();
+    }
+    meanArea = nt ? meanArea / (double)nt : 0.0;
+    // references: (triangle, box); the triangle ids of the slots follow from them
+    std::vector<int32_t> refTri;
+    refTri.reserve((size_t)nt + (size_t)(kPresplitBudget * (double)nt));
+    B.tbox.reserve(refTri.capacity());
+    const int64_t budgetExtra = (int64_t)(kPresplitBudget * (double)nt);
+    int64_t extra = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        const Box &bx = tb[t];
+        if (kPresplit <= 0.0 || bx.area() <= kPresplit * meanArea || extra >= budgetExtra) {
+            refTri.push_back((int32_t)t);
+            B.tbox.push_back(bx);
+            continue;
+        }
+        double v[3][3];
+        for (int q = 0; q < 3; ++q)
+            for (int k = 0; k < 3; ++k) v[q][k] = V[3 * (int64_t)I[3 * t + q] + k];
+        // split the largest pending box at the middle of its longest axis until the boxes are
+        // small enough or the triangle has kPresplitMaxRefs references
+        std::vector<Box> parts{bx};
+        bool grew = true;
+        while (grew && (int)parts.size() < kPresplitMaxRefs && extra < budgetExtra) {
+            grew = false;
+            int bi = -1;
+            float ba = (float)(kPresplit * meanArea);
+            for (int i = 0; i < (int)parts.size(); ++i)
+                if (parts[i].area() > ba) {
+                    ba = parts[i].area();
+                    bi = i;
+                }
+            if (bi < 0) break;
+            const Box pb = parts[bi];
+            int axis = 0;
+            for (int k = 1; k < 3; ++k)
+                if (pb.hi[k] - pb.lo[k] > pb.hi[axis] - pb.lo[axis]) axis = k;
+            const double mid = 0.5 * ((double)pb.lo[axis] + (double)pb.hi[axis]);
+            Box out[2];
+            int got = 0;
+            for (int side = 0; side < 2; ++side) {
+                double lo[3], hi[3];
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = pb.lo[k];
+                    hi[k] = pb.hi[k];
+                }
+                if (side) lo[axis] = mid;
+                else hi[axis] = mid;
+                Box cbx;
+                if (clipped_box(v[0], v[1], v[2], lo, hi, cbx.lo, cbx.hi)) out[got++] = cbx;
+            }
+            if (got == 2) {
+                parts[bi] = out[0];
+                parts.push_back(out[1]);
+                ++extra;
+                grew = true;
+            } else if (got == 1 && out[0].area() < pb.area()) {
+                parts[bi] = out[0];
+                grew = true;
+            }
+        }
+        for (const Box &pb : parts) {
+            refTri.push_back((int32_t)t);
+            B.tbox.push_back(pb);
+        }
+    }
+    const int64_t nr = (int64_t)refTri.size();
+    if (nr >= (1 << 27)) return false;
+    B.cen.resize(3 * nr);
+    B.order.resize(nr);
+    for (int64_t r = 0; r < nr; ++r) {
+        for (int k = 0; k < 3; ++k) B.cen[3 * r + k] = 0.5f * (B.tbox[r].lo[k] + B.tbox[r].hi[k]);
+        B.order[r] = (int32_t)r;
     }
 
     std::vector<float> nodes;
-    nodes.reserve(16 * (size_t)std::max<int64_t>(2, nt));
-    int32_t slotCount = (int32_t)nt;
-    if (nt == 0) {
+    nodes.reserve(16 * (size_t)std::max<int64_t>(2, nr));
+    int32_t slotCount = (int32_t)nr;
+    if (nr == 0) {
         // a root whose two children are a far point box over one degenerate triangle slot
         slotCount = 1;
         nodes.assign(16, 1e30f);
@@ -525,20 +659,20 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
     } else {
         // the root must be an inner node: its children are the two halves of the first split,
         // or the single leaf twice
-        const int32_t mid = nt > 1 ? B.split(0, (int32_t)nt, 0) : -1;
+        const int32_t mid = nr > 1 ? B.split(0, (int32_t)nr, 0) : -1;
         nodes.resize(16, 0.0f);
         if (mid < 0) {
-            const int32_t ref = Builder::leaf_ref(0, (int32_t)nt);
-            B.fill(nodes, 0, 0, (int32_t)nt, (int32_t)nt, ref, ref, 0);
+            const int32_t ref = Builder::leaf_ref(0, (int32_t)nr);
+            B.fill(nodes, 0, 0, (int32_t)nr, (int32_t)nr, ref, ref, 0);
             float *n = &nodes[0];
             for (int k = 0; k < 4; ++k) { n[4 + k] = n[k]; }
             n[10] = n[8];
             n[11] = n[9];
-            B.maxLeaf = (int32_t)nt;
+            B.maxLeaf = (int32_t)nr;
         } else {
             const int32_t r0 = B.build(0, mid, 1, nodes, 0);
-            const int32_t r1 = B.build(mid, (int32_t)nt, 1, nodes, 0);
-            B.fill(nodes, 0, 0, mid, (int32_t)nt, r0, r1, 0);
+            const int32_t r1 = B.build(mid, (int32_t)nr, 1, nodes, 0);
+            B.fill(nodes, 0, 0, mid, (int32_t)nr, r0, r1, 0);
         }
     }
     collapse4(nodes, out.nodesF, out.depth4);
@@ -548,8 +682,8 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
     out.maxDepth = B.maxDepth.load();
     out.maxLeaf = B.maxLeaf.load();
     out.tris.assign(12 * (size_t)slotCount, 0.0f);
-    for (int32_t s = 0; s < (int32_t)nt; ++s) {
-        const int32_t t = B.order[s];
+    for (int32_t s = 0; s < (int32_t)nr; ++s) {
+        const int32_t t = refTri[B.order[s]];
         const float *a = V + 3 * (int64_t)I[3 * t + 0];
         const float *b = V + 3 * (int64_t)I[3 * t + 1];
         const float *c = V + 3 * (int64_t)I[3 * t + 2];
