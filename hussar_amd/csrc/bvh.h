@@ -41,9 +41,11 @@ bool quantize4(const std::vector<float> &n4, std::vector<float> &out);
 // origin + q * scale as the kernels evaluate it
 float dequant(float origin, uint32_t q, float scale);
 
-// Binned SAH (16 bins, all three axes), leaves of up to 8 triangles, depth <= 80 (median
-// splits below depth 48). Child boxes are widened by a relative margin of 2^-20 plus 1e-6 m
-// so that a ray accepted by the triangle test is never culled by a box test.
+// Binned SAH (64 bins, all three axes) over triangle references (a triangle with a large box
+// enters as several clipped references, bvh.cpp), leaves of up to 8 references, depth <= 80
+// (median splits below depth 48). Child boxes are widened by a relative margin of 2^-20 plus
+// 1e-6 m so that a ray accepted by the triangle test is never culled by a box test. Slots hold
+// the referenced triangles in leaf order (slotCount >= triangleCount).
 bool build_bvh(const float *vertices, int64_t vertexCount, const int32_t *indices, int64_t triangleCount,
                FlatBVH &out, int threads = 0);
 
