@@ -269,6 +269,306 @@ struct Builder {
     }
 };
 
+// Spatial-split build (HSR_SBVH; Stich, Friedrich and Dietrich 2009): besides the binned object
+// split, a node whose two object-split children overlap can be cut by a plane, with the
+// references that straddle it clipped into both sides; the cheaper of the two by SAH wins.
+#ifndef HSR_SBVH
+#define HSR_SBVH 1
+#endif
+constexpr int kSpatialBins = 32;
+#ifndef HSR_SPATIAL_ALPHA
+#define HSR_SPATIAL_ALPHA 1e-5f
+#endif
+#ifndef HSR_SPATIAL_BUDGET
+#define HSR_SPATIAL_BUDGET 0.3
+#endif
+constexpr float kSpatialAlpha = HSR_SPATIAL_ALPHA;    // overlap / root area above which spatial splits are tried
+constexpr double kSpatialBudget = HSR_SPATIAL_BUDGET;  // extra references at most, per triangle (a soft cap)
+constexpr int kSpatialParMin = 1 << 15;  // references of a subtree split over two threads
+constexpr int kSpatialParDepth = 5;      // at most 2^5 threads
+
+struct SRef {
+    int32_t tri;
+    Box box;
+};
+
+struct SpatialBuilder {
+    const float *V;
+    const int32_t *I;
+    std::vector<float> nodes;
+    std::vector<int32_t> slotTri;
+    int64_t budget = 0;
+    float rootArea = 1.0f;
+    int maxDepth = 0, maxLeaf = 0;
+
+    void tri(int32_t t, double v[3][3]) const {
+        for (int q = 0; q < 3; ++q)
+            for (int k = 0; k < 3; ++k) v[q][k] = V[3 * (int64_t)I[3 * t + q] + k];
+    }
+    // the part of reference r inside [lo, hi] on axis k (false: empty)
+    bool clip(const SRef &r, int k, float lo, float hi, Box &out) const {
+        double v[3][3];
+        tri(r.tri, v);
+        double blo[3], bhi[3];
+        for (int e = 0; e < 3; ++e) {
+            blo[e] = r.box.lo[e];
+            bhi[e] = r.box.hi[e];
+        }
+        blo[k] = std::max(blo[k], (double)lo);
+        bhi[k] = std::min(bhi[k], (double)hi);
+        if (blo[k] > bhi[k]) return false;
+        if (!clipped_box(v[0], v[1], v[2], blo, bhi, out.lo, out.hi)) return false;
+        for (int e = 0; e < 3; ++e) {  // never beyond the reference's own box
+            out.lo[e] = std::max(out.lo[e], r.box.lo[e]);
+            out.hi[e] = std::min(out.hi[e], r.box.hi[e]);
+        }
+        return true;
+    }
+
+    int32_t leaf(const std::vector<SRef> &refs, int depth) {
+        const int32_t first = (int32_t)slotTri.size();
+        for (const SRef &r : refs) slotTri.push_back(r.tri);
+        maxDepth = std::max(maxDepth, depth);
+        maxLeaf = std::max(maxLeaf, (int)refs.size());
+        return Builder::leaf_ref(first, (int32_t)refs.size());
+    }
+
+    static Box bounds(const std::vector<SRef> &refs) {
+        Box b;
+        b.reset();
+        for (const SRef &r : refs) b.grow(r.box);
+        return b;
+    }
+
+    // Splits refs into l and r; returns false for a leaf.
+    bool split(std::vector<SRef> &refs, int depth, std::vector<SRef> &L, std::vector<SRef> &R) {
+        const int n = (int)refs.size();
+        const Box bb = bounds(refs);
+        Box cb;
+        cb.reset();
+        for (const SRef &r : refs)
+            for (int k = 0; k < 3; ++k) {
+                const float c = 0.5f * (r.box.lo[k] + r.box.hi[k]);
+                cb.lo[k] = std::min(cb.lo[k], c);
+                cb.hi[k] = std::max(cb.hi[k], c);
+            }
+        // object split
+        float best = std::numeric_limits<float>::infinity();
+        int bAxis = -1, bBin = -1;
+        Box bl, br;
+        for (int k = 0; k < 3; ++k) {
+            const float ext = cb.hi[k] - cb.lo[k];
+            if (!(ext > 0.0f)) continue;
+            Box bins[kBins];
+            int cnt[kBins] = {0};
+            for (auto &x : bins) x.reset();
+            const float scale = (float)kBins / ext;
+            for (const SRef &r : refs) {
+                int bi = (int)((0.5f * (r.box.lo[k] + r.box.hi[k]) - cb.lo[k]) * scale);
+                bi = std::min(std::max(bi, 0), kBins - 1);
+                cnt[bi]++;
+                bins[bi].grow(r.box);
+            }
+            Box racc[kBins];
+            int rc[kBins];
+            Box acc;
+            acc.reset();
+            int c = 0;
+            for (int i = kBins - 1; i > 0; --i) {
+                acc.grow(bins[i]);
+                c += cnt[i];
+                racc[i] = acc;
+                rc[i] = c;
+            }
+            acc.reset();
+            c = 0;
+            for (int i = 0; i < kBins - 1; ++i) {
+                acc.grow(bins[i]);
+                c += cnt[i];
+                if (c == 0 || rc[i + 1] == 0) continue;
+                const float cost = acc.area() * (float)c + racc[i + 1].area() * (float)rc[i + 1];
+                if (cost < best) {
+                    best = cost;
+                    bAxis = k;
+                    bBin = i;
+                    bl = acc;
+                    br = racc[i + 1];
+                }
+            }
+        }
+        // spatial split, where the object split's children overlap
+        float sBest = std::numeric_limits<float>::infinity();
+        int sAxis = -1;
+        float sPlane = 0.0f;
+        if (bAxis >= 0 && budget > 0 && depth < 32) {
+            Box ov;
+            for (int k = 0; k < 3; ++k) {
+                ov.lo[k] = std::max(bl.lo[k], br.lo[k]);
+                ov.hi[k] = std::min(bl.hi[k], br.hi[k]);
+            }
+            if (ov.area() > kSpatialAlpha * rootArea) {
+                for (int k = 0; k < 3; ++k) {
+                    const float ext = bb.hi[k] - bb.lo[k];
+                    if (!(ext > 0.0f)) continue;
+                    Box bins[kSpatialBins];
+                    int enter[kSpatialBins] = {0}, leave[kSpatialBins] = {0};
+                    for (auto &x : bins) x.reset();
+                    const float w = ext / (float)kSpatialBins;
+                    auto binOf = [&](float x) {
+                        return std::min(std::max((int)((x - bb.lo[k]) / w), 0), kSpatialBins - 1);
+                    };
+                    for (const SRef &r : refs) {
+                        const int b0 = binOf(r.box.lo[k]), b1 = binOf(r.box.hi[k]);
+                        enter[b0]++;
+                        leave[b1]++;
+                        if (b0 == b1) {
+                            bins[b0].grow(r.box);
+                            continue;
+                        }
+                        for (int b = b0; b <= b1; ++b) {
+                            Box part;
+                            if (clip(r, k, bb.lo[k] + w * b, b == kSpatialBins - 1 ? bb.hi[k] : bb.lo[k] + w * (b + 1),
+                                     part))
+                                bins[b].grow(part);
+                        }
+                    }
+                    Box racc[kSpatialBins];
+                    int rc[kSpatialBins];
+                    Box acc;
+                    acc.reset();
+                    int c = 0;
+                    for (int i = kSpatialBins - 1; i > 0; --i) {
+                        acc.grow(bins[i]);
+                        c += leave[i];
+                        racc[i] = acc;
+                        rc[i] = c;
+                    }
+                    acc.reset();
+                    c = 0;
+                    for (int i = 0; i < kSpatialBins - 1; ++i) {
+                        acc.grow(bins[i]);
+                        c += enter[i];
+                        if (c == 0 || rc[i + 1] == 0) continue;
+                        const float cost = acc.area() * (float)c + racc[i + 1].area() * (float)rc[i + 1];
+                        if (cost < sBest) {
+                            sBest = cost;
+                            sAxis = k;
+                            sPlane = bb.lo[k] + w * (i + 1);
+                        }
+                    }
+                }
+            }
+        }
+        const float leafCost = bb.area() * (float)n;
+        const float bestAny = std::min(best, sBest);
+        if (n <= kMaxLeaf && (bAxis < 0 && sAxis < 0 || leafCost <= bestAny + kSahTrav * bb.area())) return false;
+        L.clear();
+        R.clear();
+        if (sAxis >= 0 && sBest < best) {
+            const int k = sAxis;
+            for (const SRef &r : refs) {
+                if (r.box.hi[k] <= sPlane) L.push_back(r);
+                else if (r.box.lo[k] >= sPlane) R.push_back(r);
+                else {
+                    SRef a{r.tri, {}}, b{r.tri, {}};
+                    const bool ha = clip(r, k, r.box.lo[k], sPlane, a.box);
+                    const bool hb = clip(r, k, sPlane, r.box.hi[k], b.box);
+                    if (ha) L.push_back(a);
+                    if (hb) R.push_back(b);
+                    if (ha && hb) --budget;
+                    if (!ha && !hb) L.push_back(r);
+                }
+            }
+            if (!L.empty() && !R.empty()) return true;
+            L.clear();
+            R.clear();
+        }
+        if (bAxis >= 0) {
+            const int k = bAxis;
+            const float scale = (float)kBins / (cb.hi[k] - cb.lo[k]);
+            for (const SRef &r : refs) {
+                int bi = (int)((0.5f * (r.box.lo[k] + r.box.hi[k]) - cb.lo[k]) * scale);
+                bi = std::min(std::max(bi, 0), kBins - 1);
+                (bi <= bBin ? L : R).push_back(r);
+            }
+            if (!L.empty() && !R.empty()) return true;
+        }
+        // coincident centroids: by count
+        L.assign(refs.begin(), refs.begin() + n / 2);
+        R.assign(refs.begin() + n / 2, refs.end());
+        return true;
+    }
+
+    // Appends a subtree built by another builder (node indices and leaf slots local to it):
+    // its inner references move by this builder's node count, its leaves by its slot count.
+    int32_t append(SpatialBuilder &sub, int32_t root) {
+        const int32_t nodeBase = (int32_t)(nodes.size() / 16), slotBase = (int32_t)slotTri.size();
+        auto move = [&](int32_t ref) {
+            if (ref >= 0) return ref + nodeBase;
+            const int32_t first = ref & 0x07FFFFFF, count = ((ref >> 27) & 0xF) + 1;
+            return Builder::leaf_ref(first + slotBase, count);
+        };
+        for (size_t q = 0; q < sub.nodes.size() / 16; ++q) {
+            float *nd = &sub.nodes[16 * q];
+            for (int c = 0; c < 2; ++c) {
+                int32_t ref;
+                std::memcpy(&ref, &nd[12 + c], 4);
+                ref = move(ref);
+                std::memcpy(&nd[12 + c], &ref, 4);
+            }
+        }
+        nodes.insert(nodes.end(), sub.nodes.begin(), sub.nodes.end());
+        slotTri.insert(slotTri.end(), sub.slotTri.begin(), sub.slotTri.end());
+        maxDepth = std::max(maxDepth, sub.maxDepth);
+        maxLeaf = std::max(maxLeaf, sub.maxLeaf);
+        return move(root);
+    }
+
+    // Large subtrees are built by their own builders, the left one on another thread. The
+    // spatial-split budget is shared out by reference count, so the tree does not depend on
+    // the threads' timing.
+    int32_t build(std::vector<SRef> &refs, int depth) {
+        std::vector<SRef> L, R;
+        const bool inner = (int)refs.size() > 1 && depth < kMaxDepth - 1 && split(refs, depth, L, R);
+        if (!inner || ((int)refs.size() <= 16 && depth >= kMaxDepth - 1)) return leaf(refs, depth);
+        std::vector<SRef>().swap(refs);
+        const int32_t me = (int32_t)(nodes.size() / 16);
+        nodes.resize(nodes.size() + 16, 0.0f);
+        const Box b0 = bounds(L), b1 = bounds(R);
+        const int64_t bl = (int64_t)((double)budget * (double)L.size() / (double)(L.size() + R.size()));
+        const int64_t br = budget - bl;
+        int32_t r0, r1;
+        if (L.size() + R.size() >= (size_t)kSpatialParMin && depth < kSpatialParDepth) {
+            SpatialBuilder sl{V, I, {}, {}, bl, rootArea, 0, 0}, sr{V, I, {}, {}, br, rootArea, 0, 0};
+            int32_t rl = 0;
+            std::thread t([&] { rl = sl.build(L, depth + 1); });
+            const int32_t rr = sr.build(R, depth + 1);
+            t.join();
+            r0 = append(sl, rl);
+            r1 = append(sr, rr);
+            budget = sl.budget + sr.budget;
+        } else {
+            budget = bl;
+            r0 = build(L, depth + 1);
+            const int64_t leftover = budget;
+            budget = br + leftover;
+            r1 = build(R, depth + 1);
+        }
+        float lo0[3], hi0[3], lo1[3], hi1[3];
+        Builder::widen(b0, lo0, hi0);
+        Builder::widen(b1, lo1, hi1);
+        float *nd = &nodes[16 * (size_t)me];
+        nd[0] = lo0[0]; nd[1] = hi0[0]; nd[2] = lo0[1]; nd[3] = hi0[1];
+        nd[4] = lo1[0]; nd[5] = hi1[0]; nd[6] = lo1[1]; nd[7] = hi1[1];
+        nd[8] = lo0[2]; nd[9] = hi0[2]; nd[10] = lo1[2]; nd[11] = hi1[2];
+        std::memcpy(&nd[12], &r0, 4);
+        std::memcpy(&nd[13], &r1, 4);
+        nd[14] = 0.0f;
+        nd[15] = 0.0f;
+        return me;
+    }
+};
+
 // BVH2 child c of node n: its widened box and reference
 void child2(const std::vector<float> &n2, int32_t n, int c, float lo[3], float hi[3], int32_t &ref) {
     const float *p = &n2[16 * (size_t)n];
@@ -648,7 +948,25 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
     std::vector<float> nodes;
     nodes.reserve(16 * (size_t)std::max<int64_t>(2, nr));
     int32_t slotCount = (int32_t)nr;
-    if (nr == 0) {
+    std::vector<int32_t> slotTri;  // spatial-split build: triangle of each slot
+    if (HSR_SBVH && nr >= 2) {
+        SpatialBuilder S;
+        S.V = V;
+        S.I = I;
+        S.budget = (int64_t)(kSpatialBudget * (double)nt);
+        std::vector<SRef> refs((size_t)nr);
+        for (int64_t r = 0; r < nr; ++r) refs[r] = SRef{refTri[r], B.tbox[r]};
+        S.rootArea = std::max(SpatialBuilder::bounds(refs).area(), 1e-30f);
+        S.nodes.reserve(16 * (size_t)nr);
+        const int32_t root = S.build(refs, 0);
+        if (root != 0) return false;  // the root is an inner node at index 0
+        nodes.swap(S.nodes);
+        slotTri.swap(S.slotTri);
+        if (slotTri.size() >= (size_t)(1 << 27)) return false;
+        slotCount = (int32_t)slotTri.size();
+        B.maxDepth = S.maxDepth;
+        B.maxLeaf = S.maxLeaf;
+    } else if (nr == 0) {
         // a root whose two children are a far point box over one degenerate triangle slot
         slotCount = 1;
         nodes.assign(16, 1e30f);
@@ -682,8 +1000,8 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
     out.maxDepth = B.maxDepth.load();
     out.maxLeaf = B.maxLeaf.load();
     out.tris.assign(12 * (size_t)slotCount, 0.0f);
-    for (int32_t s = 0; s < (int32_t)nr; ++s) {
-        const int32_t t = refTri[B.order[s]];
+    for (int32_t s = 0; s < slotCount && nr > 0; ++s) {
+        const int32_t t = slotTri.empty() ? refTri[B.order[s]] : slotTri[s];
         const float *a = V + 3 * (int64_t)I[3 * t + 0];
         const float *b = V + 3 * (int64_t)I[3 * t + 1];
         const float *c = V + 3 * (int64_t)I[3 * t + 2];

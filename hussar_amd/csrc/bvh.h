@@ -42,7 +42,8 @@ bool quantize4(const std::vector<float> &n4, std::vector<float> &out);
 float dequant(float origin, uint32_t q, float scale);
 
 // Binned SAH (64 bins, all three axes) over triangle references (a triangle with a large box
-// enters as several clipped references, bvh.cpp), leaves of up to 8 references, depth <= 80
+// enters as several clipped references) with spatial splits where object splits overlap
+// (bvh.cpp), leaves of up to 8 references, depth <= 80
 // (median splits below depth 48). Child boxes are widened by a relative margin of 2^-20 plus
 // 1e-6 m so that a ray accepted by the triangle test is never culled by a box test. Slots hold
 // the referenced triangles in leaf order (slotCount >= triangleCount).
