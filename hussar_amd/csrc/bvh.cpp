@@ -275,7 +275,10 @@ struct Builder {
 #ifndef HSR_SBVH
 #define HSR_SBVH 1
 #endif
-constexpr int kSpatialBins = 32;
+#ifndef HSR_SPATIAL_BINS
+#define HSR_SPATIAL_BINS 32
+#endif
+constexpr int kSpatialBins = HSR_SPATIAL_BINS;
 #ifndef HSR_SPATIAL_ALPHA
 #define HSR_SPATIAL_ALPHA 1e-5f
 #endif
