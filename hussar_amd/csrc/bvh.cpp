@@ -328,7 +328,10 @@ struct SpatialBuilder {
         return true;
     }
 
+    bool overflow = false;  // a leaf beyond the 16 references a leaf reference can hold
+
     int32_t leaf(const std::vector<SRef> &refs, int depth) {
+        if (refs.size() > 16) overflow = true;
         const int32_t first = (int32_t)slotTri.size();
         for (const SRef &r : refs) slotTri.push_back(r.tri);
         maxDepth = std::max(maxDepth, depth);
@@ -524,6 +527,7 @@ struct SpatialBuilder {
         slotTri.insert(slotTri.end(), sub.slotTri.begin(), sub.slotTri.end());
         maxDepth = std::max(maxDepth, sub.maxDepth);
         maxLeaf = std::max(maxLeaf, sub.maxLeaf);
+        overflow = overflow || sub.overflow;
         return move(root);
     }
 
@@ -962,7 +966,7 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
         S.rootArea = std::max(SpatialBuilder::bounds(refs).area(), 1e-30f);
         S.nodes.reserve(16 * (size_t)nr);
         const int32_t root = S.build(refs, 0);
-        if (root != 0) return false;  // the root is an inner node at index 0
+        if (root != 0 || S.overflow) return false;  // the root is an inner node at index 0
         nodes.swap(S.nodes);
         slotTri.swap(S.slotTri);
         if (slotTri.size() >= (size_t)(1 << 27)) return false;
