@@ -47,6 +47,9 @@ constexpr int kWfBlock = 256;
 #ifndef HSR_HIT_WAVES
 #define HSR_HIT_WAVES 1
 #endif
+#ifndef HSR_GATHER_WAVES
+#define HSR_GATHER_WAVES 1
+#endif
 #ifndef HSR_NEE_WAVES
 #define HSR_NEE_WAVES 4
 #endif
@@ -790,7 +793,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a
 }
 
 // ------------------------------------------------------------------------ gather + splat
-__global__ __launch_bounds__(kWfBlock) void k_wf_gather(TraceArgs a, WfArgs w, int depth, double2 *partials,
+__global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceArgs a, WfArgs w, int depth, double2 *partials,
                                                         bool count) {
     // one chirp's frame tile in LDS: the guard-band layout (flush_events_g) for N >= 16, else
     // N*C complex doubles (tile_re); unused with M > 1
