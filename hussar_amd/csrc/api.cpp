@@ -354,8 +354,27 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
     }
 }
 
+static hipError_t event_wait(hipEvent_t e);
+
+// Adds the reduction and total device time of a run that returned before its frame reduction
+// ended, once that has ended.
+static hsr_status settle_timing(hsr_backend *b) {
+    if (!b->timingPending) return HSR_OK;
+    b->timingPending = false;
+    HSR_HIP(event_wait(b->ev[3]));
+    float reduce = 0.0f, total = 0.0f;
+    HSR_HIP(hipEventElapsedTime(&reduce, b->ev[1], b->ev[3]));
+    HSR_HIP(hipEventElapsedTime(&total, b->ev[2], b->ev[3]));
+    b->stats.reduce_ms += reduce;
+    b->stats.total_ms += total;
+    return HSR_OK;
+}
+
 hsr_status hsr_hip_last_stats(hsr_backend *b, hsr_run_stats *out) {
     if (!b || !out) return fail(HSR_ERR_INVALID, "null argument");
+    HSR_HIP(hipSetDevice(b->device));
+    hsr_status s = settle_timing(b);
+    if (s != HSR_OK) return s;
     *out = b->stats;
     return HSR_OK;
 }
@@ -519,6 +538,10 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     }
     if (b->optCounters) HSR_HIP(hipMemsetAsync(b->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
 
+    {
+        hsr_status s = settle_timing(b);  // before ev[] is recorded again
+        if (s != HSR_OK) return s;
+    }
     if (!accumulate) b->stats = hsr_run_stats{};
     b->stats.bvh_build_ms = b->bvh_ms;
     double trace_ms = 0.0, reduce_ms = 0.0;
@@ -608,23 +631,31 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
                                            (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));
         HSR_HIP(hipEventRecord(b->ev[1], b->stream));
+        // the caller steps the guide next: its statistics' download goes with the last batch,
+        // ahead of the frame reduction, which then runs while the host refines the trees
+        const bool statsFirst = queueStats && guide && tx == T - 1 && done + n >= budget;
+        if (statsFirst) {
+            hsr_status s = guide_queue_stats(guide);
+            if (s != HSR_OK) return s;
+        }
         // LDS tiles: the workgroup tiles are reduced into the frame; cube: the gather kernel
         // has added into the frame directly
         if (!cube)
             HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, MD, b->d_stage_sum, b->d_stage_comp, S,
                                        b->d_frame_sum, b->d_frame_comp, b->stream));
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
-        // the caller steps the guide next: its statistics' download goes with the last batch
-        if (queueStats && guide && tx == T - 1 && done + n >= budget) {
-            hsr_status s = guide_queue_stats(guide);
-            if (s != HSR_OK) return s;
-        }
-        HSR_HIP(event_wait(b->ev[3]));
-        float ms1 = 0.0f, ms2 = 0.0f;
+        // the traversal events all precede ev[1] on the stream, and so the statistics' copy
+        HSR_HIP(event_wait(statsFirst ? guide->evDl : b->ev[3]));
+        float ms1 = 0.0f;
         HSR_HIP(hipEventElapsedTime(&ms1, b->ev[0], b->ev[1]));
-        HSR_HIP(hipEventElapsedTime(&ms2, b->ev[1], b->ev[3]));
         trace_ms += ms1;
-        reduce_ms += ms2;
+        if (statsFirst) {
+            b->timingPending = true;
+        } else {
+            float ms2 = 0.0f;
+            HSR_HIP(hipEventElapsedTime(&ms2, b->ev[1], b->ev[3]));
+            reduce_ms += ms2;
+        }
         launches += 1;
         for (int q = 0; q < nClosest; ++q) {
             float m = 0.0f;
@@ -695,7 +726,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     }
     }  // transmitters
     float total = 0.0f;
-    if (budget > 0) HSR_HIP(hipEventElapsedTime(&total, b->ev[2], b->ev[3]));
+    if (budget > 0 && !b->timingPending) HSR_HIP(hipEventElapsedTime(&total, b->ev[2], b->ev[3]));
     b->totalWeight += (double)budget * (double)sampleWeight;
     b->stats.trace_ms += trace_ms;
     b->stats.reduce_ms += reduce_ms;

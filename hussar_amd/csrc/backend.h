@@ -121,6 +121,9 @@ struct hsr_backend {
     void *d_postScratch = nullptr;
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the last run returned before its frame reduction ended (its guide statistics were
+    // ready first): reduce_ms and total_ms still to add from ev[1..3] (settle_timing)
+    bool timingPending = false;
     std::vector<hipEvent_t> evTrav;  // pairs around traversal launches (wavefront)
     // the closest-hit traversal of a depth runs on stream2, beside that depth's shadow and
     // gather stages on stream (default priority both); evFork/evJoin order them per depth

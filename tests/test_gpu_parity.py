@@ -169,6 +169,10 @@ def test_path_tracer_run_doubling_schedule(oracle, dihedral):
     pt.run(b2, sc, 1 << 20)
     z = pt.fetchFrame("complex128")
     _, tw = b2.fetch()
+    # the training iterations return once their statistics are downloaded; their frame
+    # reduction's timing is settled by the next iteration, the last one's here
+    st = b2.stats()
+    assert st["total_ms"] >= st["trace_ms"] > 0 and st["reduce_ms"] >= 0
     zo, two = oracle.path_tracer_run(om, sc, pt, fc, 1 << 20, prec=oracle.PREC_MIXED, guide=oracle.OracleGuide(),
                                      threads=THREADS)
     assert tw == two == 540672
