@@ -231,7 +231,9 @@ hsr_status hsr_hip_frame_peak(hsr_backend *backend, int32_t fourier, int32_t ind
 
 /* Statistics of the last top-level call (hsr_hip_run, or all the runs of one
  * hsr_path_tracer_run / hsr_path_tracer_run_schedule), measured with HIP events on the
- * stream the kernels run on. */
+ * stream the kernels run on. A guided training run returns once its guide statistics have
+ * arrived, before its frame reduction ends; hsr_hip_last_stats (or the next run) waits for
+ * that reduction before it adds its reduce_ms and total_ms. */
 typedef struct hsr_run_stats {
     double trace_ms;        /* device time of the path-kernel launches */
     double reduce_ms;       /* device time of the frame reduction */
