@@ -7,6 +7,7 @@ with packed-FP32 instruction selection disabled.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -25,8 +26,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
 
 def build(verbose: bool = False) -> str:
     """Compile every source into the shared library; returns its path."""
-    objs = []
-    for src in SOURCES:
+    def compile_one(src: str) -> str:
         obj = os.path.join(CSRC, "_build", src + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
@@ -38,7 +38,11 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed on {src}:\n{err}")
         if verbose and err:
             print(err, file=sys.stderr)
-        objs.append(obj)
+        return obj
+
+    # one hipcc process per source, at most four at a time
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as pool:
+        objs = list(pool.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

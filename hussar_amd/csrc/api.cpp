@@ -1,6 +1,6 @@
 // C ABI of the gfx950 backend (include/hussar_hip.h).
 //
-// hsr_hip_create     ~ Backend(const TriangleMesh &, Integrator &)  (cpu.cpp:54-69, gpu.cpp:482-493)
+// hsr_hip_create     ~ Backend(const TriangleMesh &, Integrator &)  (cpu.cpp:32-47, gpu.cpp:482-493)
 // hsr_hip_run        ~ Backend::run(scene, budget, interruptFlag)   (cpu.h:44-46, gpu.cpp:499-532)
 // hsr_path_tracer_run ~ PathTracer::run                             (path.h:171-224)
 // hsr_hip_fetch_frame ~ PathTracer::fetchFrame                      (path.h:453-455)
@@ -368,6 +368,15 @@ static hsr_status settle_timing(hsr_backend *b) {
     b->stats.reduce_ms += reduce;
     b->stats.total_ms += total;
     return HSR_OK;
+}
+
+// Starts the statistics of a new top-level call. A guided training run of an earlier call
+// that failed before its next run may still hold unsettled timings; they are settled (and
+// so dropped) here, never added to the new call's figures.
+static hsr_status reset_stats(hsr_backend *b) {
+    hsr_status s = settle_timing(b);
+    b->stats = hsr_run_stats{};
+    return s;
 }
 
 hsr_status hsr_hip_last_stats(hsr_backend *b, hsr_run_stats *out) {
@@ -1044,7 +1053,7 @@ hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     hsr_status s = hsr_hip_clear_frame(b);  // setup() + clearFrame(), path.h:173-174
     if (s != HSR_OK) return s;
-    b->stats = hsr_run_stats{};
+    if ((s = reset_stats(b)) != HSR_OK) return s;
     int64_t offset = 0;
     float weight = 1.0f;
     if (!knobs->doGuiding) return run_impl(b, scene, knobs, 0, samples, 1.0f, 0, nullptr, interruptFlag, true);
@@ -1084,7 +1093,8 @@ hsr_status hsr_path_tracer_run_schedule(hsr_backend *b, const hsr_scene *scene, 
     hsr_status s;
     const bool guiding = knobs->doGuiding && guide;
     if (knobs->doGuiding && !guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");
-    b->stats = hsr_run_stats{};
+    HSR_HIP(hipSetDevice(b->device));
+    if ((s = reset_stats(b)) != HSR_OK) return s;
     if (guiding) {
         if ((s = hsr_guide_reset(guide)) != HSR_OK) return s;
         for (int p = 0; p < trainingPasses; ++p) {

@@ -956,6 +956,7 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
     nodes.reserve(16 * (size_t)std::max<int64_t>(2, nr));
     int32_t slotCount = (int32_t)nr;
     std::vector<int32_t> slotTri;  // spatial-split build: triangle of each slot
+    bool spatial = false;
     if (HSR_SBVH && nr >= 2) {
         SpatialBuilder S;
         S.V = V;
@@ -966,14 +967,21 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
         S.rootArea = std::max(SpatialBuilder::bounds(refs).area(), 1e-30f);
         S.nodes.reserve(16 * (size_t)nr);
         const int32_t root = S.build(refs, 0);
-        if (root != 0 || S.overflow) return false;  // the root is an inner node at index 0
-        nodes.swap(S.nodes);
-        slotTri.swap(S.slotTri);
-        if (slotTri.size() >= (size_t)(1 << 27)) return false;
-        slotCount = (int32_t)slotTri.size();
-        B.maxDepth = S.maxDepth;
-        B.maxLeaf = S.maxLeaf;
-    } else if (nr == 0) {
+        // The root must be an inner node at index 0, and no leaf may hold more than 16
+        // references. A mesh of a few triangles whose split is not worth it by SAH comes back
+        // as a single leaf, and a degenerate subtree can overflow a leaf: both are built by the
+        // object builder below instead (its root over a single leaf has that leaf twice, and it
+        // splits by the median below depth kMedianDepth).
+        if (root == 0 && !S.overflow && S.slotTri.size() < (size_t)(1 << 27)) {
+            nodes.swap(S.nodes);
+            slotTri.swap(S.slotTri);
+            slotCount = (int32_t)slotTri.size();
+            B.maxDepth = S.maxDepth;
+            B.maxLeaf = S.maxLeaf;
+            spatial = true;
+        }
+    }
+    if (!spatial && nr == 0) {
         // a root whose two children are a far point box over one degenerate triangle slot
         slotCount = 1;
         nodes.assign(16, 1e30f);
@@ -981,7 +989,7 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
         std::memcpy(&nodes[12], &ref, 4);
         std::memcpy(&nodes[13], &ref, 4);
         nodes[14] = nodes[15] = 0.0f;
-    } else {
+    } else if (!spatial) {
         // the root must be an inner node: its children are the two halves of the first split,
         // or the single leaf twice
         const int32_t mid = nr > 1 ? B.split(0, (int32_t)nr, 0) : -1;

@@ -583,7 +583,7 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
 }
 
 // Hit update and specular reflection (path.h:413-442, integrator.h:164-168,
-// intersection.h:38): geometric normal (kernel.cu:86) face-forwarded (cpu.cpp:104-107), and
+// intersection.h:38): geometric normal (kernel.cu:86) face-forwarded (cpu.cpp:82-85), and
 // the segment length re-evaluated in double against the hit triangle's plane, both from the
 // float edges promoted to double. Returns false when the hit is grazing (path ends).
 __device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot) {

@@ -126,7 +126,7 @@ typedef struct hsr_backend hsr_backend;
 typedef struct hsr_guide hsr_guide;
 
 /* Replaces the backend constructor Backend(const TriangleMesh &, Integrator &)
- * (cpu.h:18-20 -> cpu.cpp:54-69 Embree scene build; gpu.h:18-20 -> gpu.cpp:482-493 OptiX
+ * (cpu.h:18-20 -> cpu.cpp:32-47 Embree scene build; gpu.h:18-20 -> gpu.cpp:482-493 OptiX
  * GAS build). vertices: float[3*vertexCount]; indices: int32[3*triangleCount] (the
  * TriangleMesh vertexBuffer / indexBuffer, mesh.h:16-28). The mesh is copied, a BVH is
  * built on the host and uploaded; the caller may free its buffers afterwards.

@@ -25,10 +25,10 @@
 // reference leaves room):
 //  * speed of light = the reference's float constant 299792458f, i.e. 299792448 m/s after
 //    float rounding (radar.h:107), in every precision;
-//  * Embree (cpu.cpp:85-109) is replaced by a Möller–Trumbore test with tnear < t < tfar
+//  * Embree (cpu.cpp:63-87) is replaced by a Möller–Trumbore test with tnear < t < tfar
 //    (tnear = Epsilon = 5e-5, hussar.h:52), closest hit = smallest t, ties broken by the
 //    smaller triangle id; geometric normal normalize(cross(v1-v0, v2-v0)) face-forwarded
-//    against the ray (kernel.cu:86-97, cpu.cpp:98-107);
+//    against the ray (kernel.cu:86-97, cpu.cpp:73-86);
 //  * get2D's unspecified argument order (halton.h:24-27) is a knob (primaryOrder);
 //  * FP32 sin/cos of the sphere warp (sampling.h:38-44) use one fixed double-precision
 //    Cody–Waite + Taylor evaluation rounded to float, so CPU and GPU agree bit for bit;
@@ -1046,7 +1046,7 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
             if (rec && depth < maxDepth) rec->hits[depth] = id;
             if (id < 0) break;
             V3<Gd> n = normal(id);
-            if (dot(n, d) > Gd(0)) n = -n;  // face-forward (cpu.cpp:104-107)
+            if (dot(n, d) > Gd(0)) n = -n;  // face-forward (cpu.cpp:82-85)
             cosTheta = std::abs(dot(n, -d));
             if (cosTheta < Gd(1e-3)) break;
             if (motion) {

@@ -1,8 +1,9 @@
 // Host check of the product's BVH: closest hits through the quantised four-wide nodes (the
 // box test of slab4q in trace_common.h, in float on the host, FMA as written) against brute
 // force over every triangle slot, for rays aimed at triangle vertices (edge-heavy), from
-// points on triangles in random directions and along the axes. Prints
-// "nodes N slots S depth4 D rays R hits H mismatches M".
+// points on triangles in random directions and along the axes, and that every triangle sits
+// in at least one slot. Prints
+// "nodes N slots S depth4 D rays R hits H mismatches M missing K".
 // Usage: bvh_brute_force mesh.bin rays   (mesh.bin: int64 nv, int64 nt, float v[3nv], int32 i[3nt])
 #include <algorithm>
 #include <cmath>
@@ -178,7 +179,14 @@ int main(int argc, char **argv) {
         if (id != bid || (id >= 0 && tb != bt)) ++bad;
         if (id >= 0) ++hits;
     }
-    std::printf("nodes %d slots %d depth4 %d rays %d hits %d mismatches %d\n", B.nodeCount, B.slotCount, B.depth4, NR,
-                hits, bad);
+    std::vector<char> seen((size_t)nt, 0);
+    for (int s = 0; s < B.slotCount && nt > 0; ++s) {
+        int ii;
+        std::memcpy(&ii, &B.tris[12 * (size_t)s + 3], 4);
+        if (ii >= 0 && ii < nt) seen[(size_t)ii] = 1;
+    }
+    const long long missing = std::count(seen.begin(), seen.end(), (char)0);
+    std::printf("nodes %d slots %d depth4 %d rays %d hits %d mismatches %d missing %lld\n", B.nodeCount, B.slotCount,
+                B.depth4, NR, hits, bad, missing);
     return 0;
 }

@@ -1,5 +1,5 @@
 """Oracle pins for ray queries: the BVH closest hit and occlusion equal brute force over all
-triangles (the Embree replacement of cpu.cpp:75-109), including rays aimed at vertices and
+triangles (the Embree replacement of cpu.cpp:53-87), including rays aimed at vertices and
 edges, in FP32 and FP64."""
 import numpy as np
 import pytest
