@@ -1170,6 +1170,7 @@ static RenderFn render_for(int prec) {
     switch (prec) {
         case 0: return &render<float, float, float, float, true>;
         case 1: return &render<float, double, float, double, false>;
+        case 3: return &render<float, float, float, double, false>;
         default: return &render<double, double, double, double, true>;
     }
 }

@@ -238,6 +238,33 @@ def test_knob_and_shape_variants(oracle, case, pipeline):
     check_parity(oracle, om, b, sc, kn, fc, n, record=min(n, 1 << 16), fp64=True)
 
 
+def few_triangle_mesh(kind):
+    """Meshes of one to four triangles (a plate 2 m ahead of a radar at the origin looking down
+    +z): whose best split by SAH is a single leaf, and, for the coplanar quads, exact t ties
+    that the smaller triangle id wins."""
+    if kind == "triangle":
+        return H.TriangleMesh(np.array([[-0.3, -0.3, 2.0], [0.3, -0.3, 2.0], [-0.3, 0.3, 2.0]], np.float32),
+                              np.array([[0, 1, 2]], np.int32))
+    m = scenes.plate_mesh(size=0.5, distance=2.0)
+    if kind == "two_coplanar_quads":
+        m.addQuad([-0.1, -0.1, 2.0], [0.5, 0.0, 0.0], [0.0, 0.5, 0.0])
+    return m
+
+
+@pytest.mark.parametrize("kind", ["triangle", "quad", "two_coplanar_quads"])
+@pytest.mark.parametrize("pipeline", PIPELINES, ids=["fused", "wavefront"])
+def test_few_triangle_meshes_parity(oracle, kind, pipeline):
+    mesh = few_triangle_mesh(kind)
+    fc = H.FrameConfig(1, 256, 1)
+    sc = scenes.radar_array(scenes.dihedral_rf(), channels=1, position=(0.0, 0.0, 0.0))
+    b = make_backend(mesh, fc, pipeline)
+    om = oracle.OracleMesh.from_mesh(mesh)
+    n = 1 << 18
+    print("few-triangle mesh", kind, "pipeline", pipeline, flush=True)
+    z = check_parity(oracle, om, b, sc, H.Knobs(maxDepth=3, doGuiding=False), fc, n, record=n)
+    assert np.abs(z).argmax() > 0
+
+
 @pytest.mark.parametrize("pipeline", PIPELINES, ids=["fused", "wavefront"])
 def test_empty_mesh_and_zero_depth(pipeline):
     fc = H.FrameConfig(1, 256, 1)
