@@ -37,6 +37,7 @@ SIGNATURES = {
     "hsr_hip_fetch_frame": (C.c_int, [_vp, _P(C.c_float), _P(C.c_double), _P(C.c_double)]),
     "hsr_hip_fetch_frame_fft": (C.c_int, [_vp, _P(C.c_float), _P(C.c_double)]),
     "hsr_hip_frame_peak": (C.c_int, [_vp, C.c_int32, _P(C.c_int32), _P(C.c_double), _P(C.c_double)]),
+    "hsr_hip_frame_peak_noalloc": (C.c_int, [_vp, C.c_int32, _P(C.c_int32), _P(C.c_double), _P(C.c_double)]),
     "hsr_path_tracer_run": (C.c_int, [_vp, _P(c_scene), _P(c_knobs), C.c_int64, _vp, _P(C.c_uint8)]),
     "hsr_path_tracer_run_schedule": (C.c_int, [_vp, _P(c_scene), _P(c_knobs), C.c_int32, C.c_int64, C.c_int64,
                                                _vp, _P(C.c_uint8)]),

@@ -782,31 +782,44 @@ hsr_status hsr_hip_fetch_frame(hsr_backend *b, float *out_c64, double *out_c128,
     return HSR_OK;
 }
 
-// The normalised frame (and, with fft, its forward 3-D DFT) in b->d_post.
-static hsr_status post_prepare(hsr_backend *b, bool fft) {
+// The normalised frame (and, with fft, its forward 3-D DFT) in post, with post2 as the DFT's
+// second buffer (both of b->frameDoubles doubles).
+static hsr_status post_fill(hsr_backend *b, bool fft, double *post, double *post2) {
+    const hsr_frame_config &f = b->fc;
+    const double inv = b->totalWeight != 0.0 ? 1.0 / b->totalWeight : 0.0;
+    HSR_HIP(hsr::launch_post_normalize(b->d_frame_sum, b->d_frame_comp, inv, b->frameDoubles, post, b->stream));
+    if (fft)
+        HSR_HIP(hsr::launch_post_fft3((double2 *)post, (double2 *)post2, f.chirpCount, f.samplesPerChirp,
+                                      f.channelCount, b->stream));
+    return HSR_OK;
+}
+
+static hsr_status post_check(hsr_backend *b, bool fft) {
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured");
     const hsr_frame_config &f = b->fc;
     if (fft && (f.chirpCount > hsr::kPostMaxAxis || f.samplesPerChirp > hsr::kPostMaxAxis ||
                 f.channelCount > hsr::kPostMaxAxis))
         return fail(HSR_ERR_UNSUPPORTED, "frame axis longer than the device DFT supports");
+    return HSR_OK;
+}
+
+// The normalised frame (and, with fft, its forward 3-D DFT) in b->d_post, the backend's own
+// post-processing buffers (allocated on first use).
+static hsr_status post_prepare(hsr_backend *b, bool fft) {
+    hsr_status s = post_check(b, fft);
+    if (s != HSR_OK) return s;
     HSR_HIP(hipSetDevice(b->device));
     if (b->frameDoubles > b->postCap) {
         HSR_HIP(hipStreamSynchronize(b->stream));
         dfree(b->d_post);
         dfree(b->d_post2);
         b->postCap = 0;
-        hsr_status s;
         if ((s = dalloc(&b->d_post, b->frameDoubles)) != HSR_OK) return s;
         if ((s = dalloc(&b->d_post2, b->frameDoubles)) != HSR_OK) return s;
         b->postCap = b->frameDoubles;
     }
     if (!b->d_postScratch) HSR_HIP(hipMalloc(&b->d_postScratch, hsr::post_scratch_bytes()));
-    const double inv = b->totalWeight != 0.0 ? 1.0 / b->totalWeight : 0.0;
-    HSR_HIP(hsr::launch_post_normalize(b->d_frame_sum, b->d_frame_comp, inv, b->frameDoubles, b->d_post, b->stream));
-    if (fft)
-        HSR_HIP(hsr::launch_post_fft3((double2 *)b->d_post, (double2 *)b->d_post2, f.chirpCount, f.samplesPerChirp,
-                                      f.channelCount, b->stream));
-    return HSR_OK;
+    return post_fill(b, fft, b->d_post, b->d_post2);
 }
 
 hsr_status hsr_hip_fetch_frame_fft(hsr_backend *b, float *out_c64, double *out_c128) {
@@ -823,15 +836,14 @@ hsr_status hsr_hip_fetch_frame_fft(hsr_backend *b, float *out_c64, double *out_c
     return HSR_OK;
 }
 
-hsr_status hsr_hip_frame_peak(hsr_backend *b, int32_t fourier, int32_t index[3], double refined[3],
-                              double *magnitude) {
-    if (!b || !index || !refined) return fail(HSR_ERR_INVALID, "null argument");
-    hsr_status s = post_prepare(b, fourier != 0);
-    if (s != HSR_OK) return s;
+// Frame::argmax and frequencyEstimation over the prepared frame `post` (complex, b->frameDoubles
+// doubles) with the argmax scratch `scratch` (post_scratch_bytes()).
+static hsr_status peak_search(hsr_backend *b, const double *post, void *scratch, int32_t index[3],
+                              double refined[3], double *magnitude) {
     const hsr_frame_config &f = b->fc;
     const int64_t n = b->frameDoubles / 2;
-    int64_t *d_out = (int64_t *)((char *)b->d_postScratch + hsr::post_scratch_bytes() - sizeof(int64_t));
-    HSR_HIP(hsr::launch_post_argmax((const double2 *)b->d_post, n, b->d_postScratch, d_out, b->stream));
+    int64_t *d_out = (int64_t *)((char *)scratch + hsr::post_scratch_bytes() - sizeof(int64_t));
+    HSR_HIP(hsr::launch_post_argmax((const double2 *)post, n, scratch, d_out, b->stream));
     int64_t best = 0;
     HSR_HIP(hipMemcpyAsync(&best, d_out, sizeof(best), hipMemcpyDeviceToHost, b->stream));
     HSR_HIP(hipStreamSynchronize(b->stream));
@@ -846,11 +858,12 @@ hsr_status hsr_hip_frame_peak(hsr_backend *b, int32_t fourier, int32_t index[3],
     }
     auto mag = [&](int64_t i, double *out) -> hsr_status {
         double v[2];
-        HSR_HIP(hipMemcpy(v, b->d_post + 2 * i, sizeof(v), hipMemcpyDeviceToHost));
+        HSR_HIP(hipMemcpy(v, post + 2 * i, sizeof(v), hipMemcpyDeviceToHost));
         *out = std::hypot(v[0], v[1]);
         return HSR_OK;
     };
     double m = 0.0;
+    hsr_status s;
     if ((s = mag(best, &m)) != HSR_OK) return s;
     if (magnitude) *magnitude = m;
     int64_t off = 1;
@@ -865,6 +878,31 @@ hsr_status hsr_hip_frame_peak(hsr_backend *b, int32_t fourier, int32_t index[3],
         off *= mx;
     }
     return HSR_OK;
+}
+
+hsr_status hsr_hip_frame_peak(hsr_backend *b, int32_t fourier, int32_t index[3], double refined[3],
+                              double *magnitude) {
+    if (!b || !index || !refined) return fail(HSR_ERR_INVALID, "null argument");
+    hsr_status s = post_prepare(b, fourier != 0);
+    if (s != HSR_OK) return s;
+    return peak_search(b, b->d_post, b->d_postScratch, index, refined, magnitude);
+}
+
+hsr_status hsr_hip_frame_peak_noalloc(hsr_backend *b, int32_t fourier, int32_t index[3], double refined[3],
+                                      double *magnitude) {
+    if (!b || !index || !refined) return fail(HSR_ERR_INVALID, "null argument");
+    hsr_status s = post_check(b, fourier != 0);
+    if (s != HSR_OK) return s;
+    if (b->frameDoubles / 2 > hsr::kNoAllocCells)
+        return fail(HSR_ERR_INVALID, "frame of " + std::to_string(b->frameDoubles / 2) +
+                                         " cells exceeds the allocation-free peak search's limit of " +
+                                         std::to_string(hsr::kNoAllocCells) + " cells");
+    HSR_HIP(hipSetDevice(b->device));
+    double2 *scr = nullptr;
+    HSR_HIP(hsr::noalloc_scratch(b->device, &scr));
+    double *post = (double *)scr, *post2 = (double *)(scr + hsr::kNoAllocCells);
+    if ((s = post_fill(b, fourier != 0, post, post2)) != HSR_OK) return s;
+    return peak_search(b, post, scr + 2 * hsr::kNoAllocCells, index, refined, magnitude);
 }
 
 hsr_status hsr_hip_get_hit_record(hsr_backend *b, int32_t *hit_ids, int8_t *nee, int64_t paths, int32_t maxDepth,

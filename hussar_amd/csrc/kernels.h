@@ -190,6 +190,13 @@ hipError_t launch_post_fft3(double2 *post, double2 *tmp, int M, int N, int C, hi
 hipError_t launch_post_argmax(const double2 *v, int64_t n, void *scratch, int64_t *out, hipStream_t stream);
 size_t post_scratch_bytes();
 constexpr int kPostMaxAxis = 4096;  // longest DFT axis (line + twiddles in LDS)
+// Allocation-free peak search: a static device array per device holding two frame buffers of
+// kNoAllocCells complex cells and the argmax scratch; noalloc_scratch returns this device's
+// copy (the device must be current). Frame buffer b starts at cell b * kNoAllocCells, the
+// argmax scratch at cell 2 * kNoAllocCells.
+constexpr int64_t kNoAllocCells = 1 << 19;  // 8 MiB per buffer
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noalloc_scratch(int device, double2 **out);
 // resident workgroups of the path kernel per CU for this frame size
 int trace_blocks_per_cu(int N, int C, bool count);
 

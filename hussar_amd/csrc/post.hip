@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <utility>
 
 #include "device_math.h"
@@ -114,6 +115,39 @@ __global__ __launch_bounds__(kPostBlock) void k_post_argmax_final(const double *
 }
 
 size_t post_scratch_bytes() { return 1024 * (sizeof(double) + sizeof(int64_t)) + sizeof(int64_t); }
+
+// Scratch of the allocation-free peak search (hsr_hip_frame_peak_noalloc): two frame buffers
+// of kNoAllocCells complex cells each (the normalised frame and the DFT's second buffer), then
+// the argmax candidates and result (post_scratch_bytes()). Every device the library runs on
+// holds its own copy (16 MiB + 16 KiB) until the process ends. Nothing in it needs to start at
+// zero: each call writes every element it reads. Calls on one device share it, so calls that
+// overlap on one device (from different threads or backends) race.
+constexpr int64_t kNoAllocArgmaxCells = 1025;  // >= post_scratch_bytes() / sizeof(double2)
+static __device__ __align__(256) double2 scratch[2 * kNoAllocCells + kNoAllocArgmaxCells];
+
+// The device address of this device's copy of `scratch`, looked up once per device with that
+// device current (the array's name is a host-side stand-in, never a device address).
+hipError_t noalloc_scratch(int device, double2 **out) {
+    static std::mutex mtx;
+    static double2 *ptrs[kNoAllocMaxDevices] = {};
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    static_assert(kNoAllocArgmaxCells * sizeof(double2) >= 1024 * (sizeof(double) + sizeof(int64_t)) + sizeof(int64_t),
+                  "argmax scratch too small");
+    std::lock_guard<std::mutex> lock(mtx);
+    if (!ptrs[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        ptrs[device] = (double2 *)p;
+    }
+    *out = ptrs[device];
+    return hipSuccess;
+}
 
 hipError_t launch_post_normalize(const double *sum, const double *comp, double inv, int64_t n, double *post,
                                  hipStream_t s) {

@@ -229,6 +229,18 @@ hsr_status hsr_hip_fetch_frame_fft(hsr_backend *backend, float *out_c64, double 
 hsr_status hsr_hip_frame_peak(hsr_backend *backend, int32_t fourier, int32_t index[3], double refined[3],
                               double *magnitude);
 
+/* hsr_hip_frame_peak without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments and results (bit for bit)
+ * as hsr_hip_frame_peak, but the normalised frame, its DFT's second buffer and the argmax
+ * candidates live in a static device array of the library (16 MiB + 16 KiB, one copy per
+ * device, held until the process ends) instead of buffers the backend allocates. Frames of
+ * more than 2^19 complex cells (chirps * samples * channels) return HSR_ERR_INVALID. Calls
+ * on one device share that array: calls that overlap on one device (different threads or
+ * backends) race, and the caller must serialise them. */
+hsr_status hsr_hip_frame_peak_noalloc(hsr_backend *backend, int32_t fourier, int32_t index[3],
+                                      double refined[3], double *magnitude);
+
 /* Statistics of the last top-level call (hsr_hip_run, or all the runs of one
  * hsr_path_tracer_run / hsr_path_tracer_run_schedule), measured with HIP events on the
  * stream the kernels run on. A guided training run returns once its guide statistics have
