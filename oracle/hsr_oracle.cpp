@@ -39,9 +39,16 @@
 // random and edge-aimed rays; splat taps == v·e^{jπd}·sinc(d) and the tap sum against a
 // numpy DFT of a tone; single-bounce range bin and phase against the closed form; the
 // reference-held dihedral fixtures (evaluation/simulation/*) for peak range bins; guiding
-// invariants (∫pdf = 1, pdf(sample(x)) == returned pdf, untrained == uniform).
-// Parity unpinned: the libguiding BTree split/merge/build rules (source absent; this file
-// states its own tree) and Eigen's ulp-level mixed-type product order.
+// invariants (∫pdf = 1, pdf(sample(x)) == returned pdf, untrained == uniform); the NEE weight
+// (nee_weight: GO density correction, sphere and plane phase errors, soft-sphere and plane
+// filters) against closed forms of a plate vertex (tests/test_oracle_pins.py); the guide
+// step against hand-worked trees and the density it must build (∝ sqrt(stat/area) with the
+// second moment, ∝ stat/area without) and the refinement it must make (a leaf of share s > t
+// becomes a complete subtree of depth ceil(log2(s/t))).
+// libguiding's source is absent, so that rule is the published SD-tree rule (Müller et al.
+// 2017) in binary form, not a reading of libguiding; Eigen's ulp-level order of mixed-type
+// products (Matrix33f·Vector3c, Vector3f.cross(Vector3c)) is parity unpinned (Eigen absent; the
+// ref32 instance sums (a0 + a1) + a2).
 
 #include <algorithm>
 #include <atomic>
@@ -869,6 +876,57 @@ template <class A> inline M33<A> mcast(const float *m) {
     return r;
 }
 
+// The weight of an NEE contribution v from a bounce vertex (depth > 0, path.h:330-394): the GO
+// density correction v·r/cosθ/4π (path.h:330-341; false = skipped when cosθ < 1e-3), the phase
+// error dphase in wavelengths (path.h:343-355: for the sphere, the distance from the receiver
+// to the reflected ray o + λd, λ >= 0; for the plane, r·tan of the angle between the half
+// vector normalize(dn - d_in) and the normal n_in) and the filter (path.h:357-387: the soft
+// sphere max(1 - 0.2·(dphase/R - 1), 0) beyond R; the plane's linear ramp from filteringMin to
+// filteringMax and zero beyond, whose cut value may feed the guiding weight), and the guiding
+// weight update gw += v (path.h:392-394). o, d: the vertex and the reflected direction; dn, rr:
+// unit direction and distance to the receiver; d_in, n_in: incoming direction and normal at the
+// vertex; r: path length up to the vertex. dphase_out (may be null) receives dphase.
+template <class Gd, class F>
+bool nee_weight(const Knobs &knobs, Gd r, Gd cosTheta, V3<Gd> o, V3<Gd> d, V3<Gd> dn, Gd rr, V3<Gd> d_in,
+                V3<Gd> n_in, Gd lambda, Cx<F> &v, Cx<F> &guidingWeight, Gd *dphase_out) {
+    const F piF = K<F>::pi();
+    v = v * F(r / cosTheta);
+    v = v / (F(4) * piF);
+    if (cosTheta < Gd(1e-3)) return false;
+    Gd dphase;
+    if (knobs.filteringSphere) {  // path.h:343-349
+        const V3<Gd> rxPos = o + rr * dn;
+        const Gd lam = std::max(Gd(0), dot(d, rxPos - o));
+        dphase = norm((o + lam * d) - rxPos);
+    } else {  // path.h:350-353
+        const Gd c = dot(normalized(dn - d_in), n_in);
+        dphase = r * std::sqrt(Gd(1) - c * c) / c;
+    }
+    dphase = dphase / lambda;  // path.h:355
+    if (dphase_out) *dphase_out = dphase;
+    if (knobs.doFiltering) {  // path.h:357-387
+        if (knobs.filteringSphere) {
+            const Gd R = Gd(knobs.filteringRadius);
+            if (dphase > R) {
+                const Gd f = Gd(1.0 - 0.20 * (double)(dphase / R - Gd(1)));
+                v = v * F(std::max(f, Gd(0)));
+            }
+        } else {
+            const Gd fmin = Gd(knobs.filteringMin), fmax = Gd(knobs.filteringMax);
+            if (dphase > fmax) {
+                if (knobs.filterGuiding) {
+                    const double q = (double)(dphase / fmax);
+                    guidingWeight = guidingWeight + v / F(q * q + 1.0);
+                }
+                v = Cx<F>{F(0), F(0)};
+            }
+            if (dphase > fmin) v = v * F((fmax - dphase) / (fmax - fmin));
+        }
+    }
+    guidingWeight = guidingWeight + v;  // path.h:392-394
+    return true;
+}
+
 template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
     static constexpr bool refine = !std::is_same<T, Gd>::value;
     const Mesh *mesh;
@@ -993,40 +1051,9 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
                     const P tn = time + P(rr) / P(kSpeedOfLight);  // addDistance in connect
                     Cx<F> v = cdot(Hn, Hrx);                       // measureH, geometry.h:138
                     if (depth > 0) {
-                        // GO density correction (path.h:330-341)
-                        v = v * F(r / cosTheta);
-                        v = v / (F(4) * piF);
-                        if (cosTheta < Gd(1e-3)) continue;
-                        Gd dphase;
-                        if (knobs->filteringSphere) {  // path.h:343-349
-                            const V3<Gd> rxPos = o + rr * dn;
-                            const Gd lam = std::max(Gd(0), dot(d, rxPos - o));
-                            dphase = norm((o + lam * d) - rxPos);
-                        } else {  // path.h:350-353
-                            const Gd c = dot(normalized(dn - d_in), n_in);
-                            dphase = r * std::sqrt(Gd(1) - c * c) / c;
-                        }
-                        dphase = dphase / lambda;  // path.h:355
-                        if (knobs->doFiltering) {  // path.h:357-387
-                            if (knobs->filteringSphere) {
-                                const Gd R = Gd(knobs->filteringRadius);
-                                if (dphase > R) {
-                                    const Gd f = Gd(1.0 - 0.20 * (double)(dphase / R - Gd(1)));
-                                    v = v * F(std::max(f, Gd(0)));
-                                }
-                            } else {
-                                const Gd fmin = Gd(knobs->filteringMin), fmax = Gd(knobs->filteringMax);
-                                if (dphase > fmax) {
-                                    if (knobs->filterGuiding) {
-                                        const double q = (double)(dphase / fmax);
-                                        guidingWeight = guidingWeight + v / F(q * q + 1.0);
-                                    }
-                                    v = Cx<F>{F(0), F(0)};
-                                }
-                                if (dphase > fmin) v = v * F((fmax - dphase) / (fmax - fmin));
-                            }
-                        }
-                        guidingWeight = guidingWeight + v;  // path.h:392-394
+                        if (!nee_weight<Gd, F>(*knobs, r, cosTheta, o, d, dn, rr, d_in, n_in, lambda, v,
+                                               guidingWeight, nullptr))
+                            continue;
                     }
                     // Integrator::splat skip rule (integrator.h:121-124), off with the debug image
                     if (!knobs->produceDebugImage && (sampleWeight == 0.0f || v.re == F(0) || v.im == F(0))) {
@@ -1372,6 +1399,37 @@ void orc_guide_get(void *gh, int which, int32_t *child, float *p0, double *stat)
         if (p0) p0[i] = t.p0[i];
         if (stat) stat[i] = t.stat[i];
     }
+}
+// replace the training tree and its statistics (depth recomputed from the structure)
+void orc_guide_set_training(void *gh, const int32_t *child, const double *stat, int32_t n) {
+    Guide *g = (Guide *)gh;
+    Tree &t = g->training;
+    t.child.assign(child, child + n);
+    t.p0.assign(n, 0.5f);
+    t.stat.assign(stat, stat + n);
+    t.mass.assign(n, 0.0);
+    t.depth.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i)
+        if (t.child[i]) { t.depth[t.child[i]] = t.depth[i] + 1; t.depth[t.child[i] + 1] = t.depth[i] + 1; }
+}
+// NEE weight of one contribution v at a bounce vertex (nee_weight), in double (prec 2) or in
+// float (prec 0): vec = o[3], d[3], dn[3], d_in[3], n_in[3]; sc = r, cosTheta, rr, lambda;
+// v [2] in/out, gw [2] in/out; returns 1 when the contribution is kept, 0 when skipped.
+int orc_nee_weight(int prec, const Knobs *k, const double *vec, const double *sc, double *v, double *gw,
+                   double *dphase) {
+    auto run = [&](auto zero) -> int {
+        using G = decltype(zero);
+        auto V = [&](int i) { return v3<G>(G(vec[3 * i]), G(vec[3 * i + 1]), G(vec[3 * i + 2])); };
+        Cx<G> vv{G(v[0]), G(v[1])}, gg{G(gw[0]), G(gw[1])};
+        G dp = G(0);
+        const bool kept = nee_weight<G, G>(*k, G(sc[0]), G(sc[1]), V(0), V(1), V(2), G(sc[2]), V(3), V(4), G(sc[3]),
+                                           vv, gg, &dp);
+        v[0] = vv.re; v[1] = vv.im;
+        gw[0] = gg.re; gw[1] = gg.im;
+        if (dphase) *dphase = dp;
+        return kept ? 1 : 0;
+    };
+    return prec == 0 ? run(0.0f) : run(0.0);
 }
 // replace the sampling tree (depth recomputed from the structure)
 void orc_guide_set_sampling(void *gh, const int32_t *child, const float *p0, int32_t n) {

@@ -66,6 +66,10 @@ def lib():
         L.orc_guide_size.argtypes = [vp, C.c_int]
         L.orc_guide_get.argtypes = [vp, C.c_int, P(C.c_int32), P(C.c_float), P(C.c_double)]
         L.orc_guide_set_sampling.argtypes = [vp, P(C.c_int32), P(C.c_float), C.c_int32]
+        L.orc_guide_set_training.argtypes = [vp, P(C.c_int32), P(C.c_double), C.c_int32]
+        L.orc_nee_weight.restype = C.c_int
+        L.orc_nee_weight.argtypes = [C.c_int, P(c_knobs), P(C.c_double), P(C.c_double), P(C.c_double),
+                                     P(C.c_double), P(C.c_double)]
         L.orc_render.restype = C.c_int
         L.orc_render.argtypes = [vp, P(c_scene), P(c_knobs), P(c_frame_config), C.c_int, C.c_int64, C.c_int64,
                                  C.c_float, C.c_int, vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int32),
@@ -198,6 +202,11 @@ class OracleGuide:
         p0 = np.ascontiguousarray(p0, np.float32)
         lib().orc_guide_set_sampling(self.h, _p(child, C.c_int32), _p(p0, C.c_float), child.size)
 
+    def set_training(self, child, stat):
+        child = np.ascontiguousarray(child, np.int32)
+        stat = np.ascontiguousarray(stat, np.float64)
+        lib().orc_guide_set_training(self.h, _p(child, C.c_int32), _p(stat, C.c_double), child.size)
+
     def __del__(self):
         try:
             lib().orc_guide_destroy(self.h)
@@ -214,6 +223,20 @@ def frame_peak(z: np.ndarray):
     ref = np.zeros(3, np.float64)
     lib().orc_frame_peak(_p(buf, C.c_double), M, N, C_, _p(idx, C.c_int32), _p(ref, C.c_double))
     return tuple(int(i) for i in idx), tuple(float(r) for r in ref)
+
+
+def nee_weight(knobs, o, d, dn, rr, d_in, n_in, r, cos_theta, wavelength, v, gw=0j, prec=PREC_FP64):
+    """The weight of one NEE contribution v at a bounce vertex (GO density correction, phase
+    error dphase and the filter, path.h:330-394). Returns (kept, v, guidingWeight, dphase)."""
+    kn = knobs.to_c() if hasattr(knobs, "to_c") else knobs
+    vec = np.ascontiguousarray(np.concatenate([o, d, dn, d_in, n_in]), np.float64)
+    sc = np.array([r, cos_theta, rr, wavelength], np.float64)
+    vv = np.array([v.real, v.imag], np.float64)
+    gg = np.array([complex(gw).real, complex(gw).imag], np.float64)
+    dp = C.c_double(0.0)
+    kept = lib().orc_nee_weight(0 if prec == PREC_REF32 else 2, C.byref(kn), _p(vec, C.c_double), _p(sc, C.c_double), _p(vv, C.c_double),
+                                _p(gg, C.c_double), C.byref(dp))
+    return bool(kept), complex(vv[0], vv[1]), complex(gg[0], gg[1]), dp.value
 
 
 def frame_size(fc) -> int:
