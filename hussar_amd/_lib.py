@@ -21,6 +21,7 @@ PHASE_FP64, PHASE_FP32, PHASE_DOUBLE_FLOAT = 0, 1, 2
 OPT_TRAV_WAVES = 6
 OPT_BATCH_PATHS = 7
 OPT_CUBE_EVENTS = 8
+OPT_CHAIN = 9
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER

@@ -350,6 +350,11 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
                 return fail(HSR_ERR_UNSUPPORTED, "phase mode must be 0 (FP64), 1 (FP32) or 2 (double-float)");
             b->optPhaseMode = (int32_t)value;
             return HSR_OK;
+        case HSR_OPT_CHAIN:
+            if (value < 0 || value > 1)
+                return fail(HSR_ERR_UNSUPPORTED, "chain must be 0 (FP64) or 1 (the reference's FP32 chain)");
+            b->optChain = (int32_t)value;
+            return HSR_OK;
         default: return fail(HSR_ERR_INVALID, "unknown option");
     }
 }
@@ -429,6 +434,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     if (knobs->correctPhase) return fail(HSR_ERR_UNSUPPORTED, "correctPhase is not supported");
     if (!knobs->useGeometricalOptics) return fail(HSR_ERR_UNSUPPORTED, "only geometrical optics is supported");
     if (knobs->maxDepth < 0) return fail(HSR_ERR_INVALID, "negative maxDepth");
+    if (b->optChain && b->optPipeline != 1)
+        return fail(HSR_ERR_UNSUPPORTED, "the reference FP32 chain needs the wavefront pipeline");
     if (scene->rxCount < 1 || scene->rxCount > HSR_MAX_CHANNELS) return fail(HSR_ERR_INVALID, "rxCount out of range");
     if (scene->rxCount > b->fc.channelCount)
         return fail(HSR_ERR_INVALID, "more receivers than frame channels");
@@ -476,6 +483,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.filteringMax = knobs->filteringMax;
     a.filteringRadius = knobs->filteringRadius;
     a.phaseMode = b->optPhaseMode;
+    a.refChain = b->optChain;
     a.N = N;
     a.C = C;
     a.sampleWeight = sampleWeight;

@@ -49,6 +49,7 @@ struct TraceArgs {
         primaryOrder, produceDebugImage;
     float filteringMin, filteringMax, filteringRadius;
     int32_t phaseMode;  // IF phase: 0 FP64 (default), 1 FP32 as the reference, 2 double-float
+    int32_t refChain;   // 1: trace with the reference's FP32 geometry chain (wavefront only)
     // frame
     int32_t N, C;
     // work: Halton indices offset + [0, count); wave w takes [w*chunk, (w+1)*chunk)

@@ -429,6 +429,18 @@ __device__ inline float guide_sample(const TraceArgs &a, const int32_t *child, c
 }
 
 // ------------------------------------------------------------------------ path state
+// The reference's FP32 geometry chain (the arithmetic of path.h:236-442 and cpu.cpp:9-18,
+// 73-86 in float), carried beside the FP64 chain when a run traces with it (kRef): it gives
+// the traversal and shadow rays and every discrete decision (hit ids, visibility, the grazing
+// and range cut-offs, the back-face cull, the filter thresholds), while path lengths, time,
+// phase and the GO weight come from the FP64 chain refined on the same triangles.
+struct Chain32 {
+    f3 o, d;        // vertex and direction
+    f3 n_in, d_in;  // face-forwarded normal and incoming direction at the last hit
+    float r;        // path length
+    float cosTheta;
+};
+
 struct PathState {
     d3 o, d;        // FP64 chain ray
     d3 n_in, d_in;  // normal and direction at the last hit (surface.incoming)
@@ -444,7 +456,22 @@ struct PathState {
     int64_t local;  // index - offset
     double rate;    // motion: d/dt of the path length up to the current vertex [m/s]
     f3 vprev;       // motion: velocity of the current vertex (the Tx at depth 0)
+    Chain32 c32;    // kRef: the reference's FP32 chain
 };
+
+__device__ inline f3 posf(const DevAntenna &a) { return {a.pos[0], a.pos[1], a.pos[2]}; }
+
+// UniformSampleSphere (sampling.h:38-44) in FP32 as the reference evaluates it, sin and cos
+// from the shared sincos_d rounded to float
+__device__ inline f3 uniform_sample_sphere_f(float u0, float u1) {
+    const float phi = (2.0f * kPiF) * u0;
+    const float y = 2.0f * u1 - 1.0f;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - y * y));
+    double sd, cd;
+    sincos_d((double)phi, &sd, &cd);
+    const float s = (float)sd, c = (float)cd;
+    return {-r * s, y, -r * c};
+}
 
 // The primary sample of a Halton index before the guiding warp: dimensions 1 and 2 in the order
 // get2D evaluates them (halton.h:24-27).
@@ -455,6 +482,7 @@ __device__ inline void primary_sample(const TraceArgs &a, uint64_t index, float 
     x1 = a.primaryOrder == 0 ? u_a : u_b;
 }
 
+template <bool kRef = false>
 __device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p, const int32_t *gsChild,
                                 const float *gsP0) {
     const float u_f = radical_inverse_2(index);
@@ -479,6 +507,12 @@ __device__ inline void generate(const TraceArgs &a, uint64_t index, PathState &p
     p.depth = 0;
     p.rate = 0.0;
     p.vprev = {a.ego[0], a.ego[1], a.ego[2]};
+    if (kRef) {
+        p.c32.o = posf(a.tx);
+        p.c32.d = mulv(rotf(a.tx), uniform_sample_sphere_f(x0, x1));
+        p.c32.r = 0.0f;
+        p.c32.cosTheta = 1.0f;
+    }
 }
 
 // NEE set-up for receiver ch at the path's current vertex: everything of path.h:296-396
@@ -496,6 +530,7 @@ struct NeeSetup {
     double dv;     // motion: radial velocity -1/2 d/dt(path length) [m/s], else 0
 };
 
+template <bool kRef = false>
 __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int ch) {
     const float fourPi = 4.0f * kPiF;
     NeeSetup ns;
@@ -503,9 +538,20 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
     d3 dn = posd(rxa) - p.o;
     const double rr = norm(dn);
     dn = dn / rr;
-    ns.o = cast3<float>(p.o);
-    ns.d = cast3<float>(dn);
-    ns.tfar = (float)rr;
+    f3 dn32{};
+    float rr32 = 0.0f;
+    if (kRef) {  // the shadow ray of the FP32 chain
+        dn32 = posf(rxa) - p.c32.o;
+        rr32 = norm(dn32);
+        dn32 = dn32 / rr32;
+        ns.o = p.c32.o;
+        ns.d = dn32;
+        ns.tfar = rr32;
+    } else {
+        ns.o = cast3<float>(p.o);
+        ns.d = cast3<float>(dn);
+        ns.tfar = (float)rr;
+    }
     ns.dv = 0.0;
     if (a.motion) {
         // the leg to the receiver: d|rx - p|/dt = dn . (v_rx - v_p)
@@ -527,7 +573,7 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
         // SurfaceEmitter::evaluate (emitter.h:64-70) and ::connect (emitter.h:72-81)
         const cf3 J = 2.0f * cross(cast3<float>(p.n_in), p.H_in);
         Hn = cross(cast3<float>(dn), J);
-        if (dot(dn, p.n_in) < 0.0) Hn = cf3{};
+        if (kRef ? dot(dn32, p.c32.n_in) < 0.0f : dot(dn, p.n_in) < 0.0) Hn = cf3{};
         const float k0 = ((2.0f * kPiF) * p.freq) / 299792448.0f;
         const cf green = cf{1.0f / fmaxf(rrF, 1e-3f), k0} / (fourPi * rrF);
         Hn = Hn * green;
@@ -538,9 +584,46 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
         // GO density correction (path.h:330-341)
         v = v * (float)(p.r / p.cosTheta);
         v = v / fourPi;
-        if (p.cosTheta < 1e-3) {
+        if (kRef ? p.c32.cosTheta < 1e-3f : p.cosTheta < 1e-3) {
             ns.v = v;
             ns.splat = false;
+            return ns;
+        }
+        if (kRef) {
+            // phase error and filter of the FP32 chain (path.h:343-387 in float)
+            float dp;
+            if (a.filteringSphere) {
+                const f3 rxPos = p.c32.o + rr32 * dn32;
+                const float lam = fmaxf(0.0f, dot(p.c32.d, rxPos - p.c32.o));
+                dp = norm((p.c32.o + lam * p.c32.d) - rxPos);
+            } else {
+                const float c = dot(normalized(dn32 - p.c32.d_in), p.c32.n_in);
+                dp = p.c32.r * sqrtf(1.0f - c * c) / c;
+            }
+            dp = dp / (299792448.0f / p.freq);
+            if (a.doFiltering) {
+                if (a.filteringSphere) {
+                    const float R = a.filteringRadius;
+                    if (dp > R) {
+                        const float f = (float)(1.0 - 0.20 * (double)(dp / R - 1.0f));
+                        v = v * fmaxf(f, 0.0f);
+                    }
+                } else {
+                    const float fmin_ = a.filteringMin, fmax_ = a.filteringMax;
+                    if (dp > fmax_) {
+                        if (a.filterGuiding) {
+                            const double q = (double)(dp / fmax_);
+                            ns.gw1 = v / (float)(q * q + 1.0);
+                        }
+                        v = cf{0.0f, 0.0f};
+                    }
+                    if (dp > fmin_) v = v * ((fmax_ - dp) / (fmax_ - fmin_));
+                }
+            }
+            ns.gwAdd = true;
+            ns.v = v;
+            ns.splat = a.produceDebugImage ? !(a.sampleWeight == 0.0f || (v.re == 0.0f && v.im == 0.0f))
+                                           : !(a.sampleWeight == 0.0f || v.re == 0.0f || v.im == 0.0f);
             return ns;
         }
         double dphase;
@@ -586,7 +669,11 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
 // intersection.h:38): geometric normal (kernel.cu:86) face-forwarded (cpu.cpp:82-85), and
 // the segment length re-evaluated in double against the hit triangle's plane, both from the
 // float edges promoted to double. Returns false when the hit is grazing (path ends).
-__device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot) {
+// kRef: the hit of the FP32 chain's ray at distance t32 (the traversal's), whose normal,
+// face-forwarding, grazing test and update are the reference's float arithmetic; the FP64
+// chain is refined on the same triangle.
+template <bool kRef = false>
+__device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot, float t32 = 0.0f) {
     const float4 P = a.tris[3 * slot + 0];
     const float4 E1 = a.tris[3 * slot + 1];
     const float4 E2 = a.tris[3 * slot + 2];
@@ -597,7 +684,16 @@ __device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot)
     d3 n = normalized(nc);
     if (dot(n, p.d) > 0.0) n = -n;
     const double cosT = fabs(dot(n, -p.d));
-    if (cosT < 1e-3) return false;
+    f3 n32{};
+    if (kRef) {
+        n32 = normalized(cross(f3{E1.x, E1.y, E1.z}, f3{E2.x, E2.y, E2.z}));
+        if (dot(n32, p.c32.d) > 0.0f) n32 = -n32;
+        const float cos32 = fabsf(dot(n32, -p.c32.d));
+        if (cos32 < 1e-3f) return false;
+        p.c32.cosTheta = cos32;
+    } else if (cosT < 1e-3) {
+        return false;
+    }
     if (a.motion) {
         // the incoming leg: d|p - o|/dt = d . (v_p - v_o), the vertex moving with its triangle
         const float4 vh = a.vel ? a.vel[slot] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -619,11 +715,25 @@ __device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot)
     p.d = (2.0 * dot(n, wi)) * n - wi;
     const f3 nF = cast3<float>(n);
     p.H = rscale(2.0f * nF, rdot(nF, p.H)) - p.H;
+    if (kRef) {
+        p.c32.r = p.c32.r + t32;
+        p.c32.n_in = n32;
+        p.c32.d_in = p.c32.d;
+        p.c32.o = p.c32.o + t32 * p.c32.d;
+        const f3 wi32 = -p.c32.d;
+        p.c32.d = (2.0f * dot(n32, wi32)) * n32 - wi32;
+    }
     return true;
 }
 
-// Termination test before the extension ray (path.h:400-404).
+// Termination test before the extension ray (path.h:400-404); kRef: the range cut-off of the
+// FP32 chain.
+template <bool kRef = false>
 __device__ inline bool path_ends(const TraceArgs &a, const PathState &p) {
+    if (kRef) {
+        const float maxDist32 = (a.adcRate / a.freqSlope) * 299792448.0f;
+        return p.depth >= a.maxDepth || p.c32.r >= maxDist32 || is_zero(p.H);
+    }
     const double maxDist = ((double)a.adcRate / (double)a.freqSlope) * kC;
     return p.depth >= a.maxDepth || p.r >= maxDist || is_zero(p.H);
 }

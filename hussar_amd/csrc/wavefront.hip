@@ -270,15 +270,16 @@ __device__ inline void finish_record(const TraceArgs &a, bool finishing, const P
 
 // A live path's vertex rounded to float (the origin of its shadow rays and of its extension
 // ray), the extension direction of a path that goes on, and the flag of a path that ends
-// after its NEE.
+// after its NEE. kRef: the vertex and direction of the FP32 chain, its length in ray_o.w.
+template <bool kRef = false>
 __device__ inline void write_extend(const WfArgs &w, bool alive, bool ends, int64_t i, const PathState &p) {
     if (alive) {
-        const f3 o = cast3<float>(p.o);
-        w.ray_o[i] = make_float4(o.x, o.y, o.z, 0.0f);
+        const f3 o = kRef ? p.c32.o : cast3<float>(p.o);
+        w.ray_o[i] = make_float4(o.x, o.y, o.z, kRef ? p.c32.r : 0.0f);
         w.si[I_FLAGS * w.P + i] = ends ? 1 : 0;
     }
     if (alive && !ends) {
-        const f3 d = cast3<float>(p.d);
+        const f3 d = kRef ? p.c32.d : cast3<float>(p.d);
         w.ray_d[i] = make_float4(d.x, d.y, d.z, 0.0f);
     }
 }
@@ -340,6 +341,7 @@ __global__ __launch_bounds__(kScanThreads) void k_cell_scan(TraceArgs a, WfArgs 
 // descends it with one dependent load per level.
 constexpr int kGuideLds = 2048;
 constexpr int kGenCellsMax = kRange;  // cells per workgroup (K)
+template <bool kRef>
 __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int K) {
     __shared__ int32_t l_gat[kRange], l_ext[kRange];
     __shared__ unsigned sn[2], sb[2], sc[2];
@@ -380,13 +382,13 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int 
                 }
                 const int64_t m = cell_first(w.cells[c0 + lo], B0) + (int64_t)(item - cstart[lo]) * kCells;
                 p.local = w.base + m;
-                generate(a, (uint64_t)(B0 + m), p, gsChild, gsP0);
+                generate<kRef>(a, (uint64_t)(B0 + m), p, gsChild, gsP0);
             }
             const bool nee = valid && !a.onlyIndirect;
             stage_append(l_gat, &sn[0], nee, (int32_t)i, lane);
-            const bool ends = valid && path_ends(a, p);
+            const bool ends = valid && path_ends<kRef>(a, p);
             stage_append(l_ext, &sn[1], valid && !ends, (int32_t)i, lane);
-            write_extend(w, valid, ends, i, p);
+            write_extend<kRef>(w, valid, ends, i, p);
             if (valid) store_state(w, i, p);
             finish_record(a, valid && ends && !nee, p, lane);
         }
@@ -686,6 +688,7 @@ __device__ inline bool write_slot(const TraceArgs &a, const WfArgs &w, int64_t j
 // the NEE set-up of the new vertex towards every receiver (its shadow slots and one shadow
 // packet per four receivers). The vertex (normal, incoming direction and field, cosθ) stays
 // in registers from the hit update to the NEE set-up.
+template <bool kRef>
 __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs a, WfArgs w, int depth) {
     __shared__ int32_t l_gat[kRange], l_ext[kRange], l_shd[kRange * kMaxGroups];
     __shared__ unsigned sn[3], sb[3], sc[3];
@@ -718,14 +721,20 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
                     load_chain(a, w, i, p, depth);
                     load_field(w, i, p);
                     if (a.motion) load_motion(a, w, i, p, depth);
-                    alive = apply_hit(a, p, h.x);
+                    if (kRef) {  // the FP32 chain is the ray that was traced
+                        const float4 o = w.ray_o[i], d = w.ray_d[i];
+                        p.c32.o = {o.x, o.y, o.z};
+                        p.c32.r = o.w;
+                        p.c32.d = {d.x, d.y, d.z};
+                    }
+                    alive = apply_hit<kRef>(a, p, h.x, __int_as_float(h.y));
                 }
                 if (!alive && recording) load_record(w, i, p, depth == 0 && a.onlyIndirect);
             }
             stage_append(l_gat, &sn[0], alive, (int32_t)i, lane);
-            const bool ends = alive && path_ends(a, p);
+            const bool ends = alive && path_ends<kRef>(a, p);
             stage_append(l_ext, &sn[1], alive && !ends, (int32_t)i, lane);
-            write_extend(w, alive, ends, i, p);
+            write_extend<kRef>(w, alive, ends, i, p);
             if (alive) store_hit(w, i, p);
             if (alive && a.motion) store_motion(w, i, p);
             finish_record(a, valid && !alive, p, lane);
@@ -735,7 +744,8 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
                 bool lead = false;
                 if (alive) {
                     const int c1 = min(nrx, (g + 1) * kPacket);
-                    for (int c = g * kPacket; c < c1; ++c) lead = write_slot(a, w, i * a.C + c, nee_setup(a, p, c));
+                    for (int c = g * kPacket; c < c1; ++c)
+                        lead = write_slot(a, w, i * a.C + c, nee_setup<kRef>(a, p, c));
                 }
                 stage_append(l_shd, &sn[2], lead, (int32_t)(i * 4 + g), lane);
             }
@@ -755,6 +765,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
 // ------------------------------------------------------------------------ NEE set-up, depth 0
 // The direct paths Tx -> Rx (onlyIndirect off): items j < n_gat * nrx of the depth-0 gather
 // list, path q_gat[j / nrx], receiver j % nrx.
+template <bool kRef>
 __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a, WfArgs w) {
     __shared__ int32_t l_shd[kRange];
     __shared__ unsigned sn, sb, sc;
@@ -777,8 +788,9 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a
                 load_chain(a, w, i, p, 0);
                 p.freq = w.sf[F_FREQ * w.P + i];
                 if (a.motion) load_motion(a, w, i, p, 0);
+                if (kRef) p.c32.o = posf(a.tx);
                 j = i * a.C + ch;
-                test = write_slot(a, w, j, nee_setup(a, p, ch));
+                test = write_slot(a, w, j, nee_setup<kRef>(a, p, ch));
             }
             // one shadow packet per four receivers of a path: entry = path * 4 + group
             const bool lead = test && (k % nrx) % kPacket == 0;
@@ -1117,14 +1129,20 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     const int64_t q = n / kCells;
     const int K = (int)std::max<int64_t>(1, std::min<int64_t>(kGenCellsMax, kRange / (q + 1)));
     hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(kScanThreads), 0, s, a, w);
-    hipLaunchKernelGGL(k_wf_gen, dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
+    if (a.refChain)
+        hipLaunchKernelGGL((k_wf_gen<true>), dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
+    else
+        hipLaunchKernelGGL((k_wf_gen<false>), dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
     if (glds > 65536) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_wf_gather,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
         if (e != hipSuccess) return e;
     }
-    if (!a.onlyIndirect) hipLaunchKernelGGL(k_wf_nee0, dim3(gNee), dim3(kWfBlock), 0, s, a, w);
+    if (!a.onlyIndirect) {
+        if (a.refChain) hipLaunchKernelGGL((k_wf_nee0<true>), dim3(gNee), dim3(kWfBlock), 0, s, a, w);
+        else hipLaunchKernelGGL((k_wf_nee0<false>), dim3(gNee), dim3(kWfBlock), 0, s, a, w);
+    }
     for (int k = 0; k <= maxDepth; ++k) {
         // the depth's closest-hit traversal on s2, beside its shadow and gather stages: it
         // reads the extension queue and rays of depth k and writes only the hits, which the
@@ -1160,7 +1178,8 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
         }
         if (k < maxDepth) {
             if (nee) (void)hipStreamWaitEvent(s, evJoin[k], 0);
-            hipLaunchKernelGGL(k_wf_shade, dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
+            if (a.refChain) hipLaunchKernelGGL((k_wf_shade<true>), dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
+            else hipLaunchKernelGGL((k_wf_shade<false>), dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
         }
     }
     return hipGetLastError();

@@ -302,6 +302,18 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        accumulated through LDS tiles (default 2^22 = 128 MiB; 0 = none). Events
  *                        beyond it are added to the frame directly with device atomics. */
 #define HSR_OPT_CUBE_EVENTS 8
+/*  HSR_OPT_CHAIN         geometry chain of the rays (wavefront pipeline): 0 = FP64 (default:
+ *                        sampled and reflected directions and hit points in double, each
+ *                        segment re-evaluated against the hit triangle's plane, the traversal
+ *                        and shadow rays rounded from it); 1 = the reference's FP32 chain
+ *                        (path.h:236-442 and cpu.cpp:9-18,73-86 in float: direction, hit point
+ *                        o + t·d, face-forwarded normal, reflection, receiver legs) gives the
+ *                        traversal and shadow rays and every discrete decision (hit ids,
+ *                        visibility, grazing and range cut-offs, back-face cull, filter
+ *                        thresholds), so hit ids follow the reference's arithmetic, while the
+ *                        FP64 chain refined on the same triangles still gives the path lengths,
+ *                        time, IF phase and GO weights. */
+#define HSR_OPT_CHAIN 9
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]

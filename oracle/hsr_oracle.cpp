@@ -18,6 +18,13 @@
 //                                         path time, phase and frame sums are FP64.
 //                                         Hit triangle ids are bit-exact against the GPU.
 //   prec 2 "fp64"   G=double, P=double  — the accuracy oracle ("truth").
+//   prec 3 "chain32" G=float, P=double  — the reference's FP32 geometry chain with FP64 time,
+//                                         phase and sums (shows that an FP32 chain alone
+//                                         misses the 1e-3 bar).
+//   prec 4 "dual"   the mixed instance plus the reference's FP32 chain, which gives the
+//                   traversal and shadow rays and every discrete decision (hit ids bit-exact
+//                   with ref32) while lengths, time, phase and weights stay FP64: the product
+//                   with HSR_OPT_CHAIN = 1.
 // All instances evaluate the same sample points: Halton values are the reference's FP32
 // values (halton.h:38-54) in every instance, and mesh/scene inputs are the same floats.
 //
@@ -886,13 +893,12 @@ template <class A> inline M33<A> mcast(const float *m) {
 // weight update gw += v (path.h:392-394). o, d: the vertex and the reflected direction; dn, rr:
 // unit direction and distance to the receiver; d_in, n_in: incoming direction and normal at the
 // vertex; r: path length up to the vertex. dphase_out (may be null) receives dphase.
-template <class Gd, class F>
-bool nee_weight(const Knobs &knobs, Gd r, Gd cosTheta, V3<Gd> o, V3<Gd> d, V3<Gd> dn, Gd rr, V3<Gd> d_in,
-                V3<Gd> n_in, Gd lambda, Cx<F> &v, Cx<F> &guidingWeight, Gd *dphase_out) {
-    const F piF = K<F>::pi();
-    v = v * F(r / cosTheta);
-    v = v / (F(4) * piF);
-    if (cosTheta < Gd(1e-3)) return false;
+// Phase error dphase in wavelengths of an NEE contribution from a bounce vertex (path.h:343-355):
+// for the sphere, the distance from the receiver to the reflected ray o + λd, λ >= 0; for the
+// plane, r·tan of the angle between the half vector normalize(dn - d_in) and the normal n_in.
+template <class Gd>
+Gd phase_error(const Knobs &knobs, Gd r, V3<Gd> o, V3<Gd> d, V3<Gd> dn, Gd rr, V3<Gd> d_in, V3<Gd> n_in,
+               Gd lambda) {
     Gd dphase;
     if (knobs.filteringSphere) {  // path.h:343-349
         const V3<Gd> rxPos = o + rr * dn;
@@ -902,32 +908,59 @@ bool nee_weight(const Knobs &knobs, Gd r, Gd cosTheta, V3<Gd> o, V3<Gd> d, V3<Gd
         const Gd c = dot(normalized(dn - d_in), n_in);
         dphase = r * std::sqrt(Gd(1) - c * c) / c;
     }
-    dphase = dphase / lambda;  // path.h:355
-    if (dphase_out) *dphase_out = dphase;
-    if (knobs.doFiltering) {  // path.h:357-387
-        if (knobs.filteringSphere) {
-            const Gd R = Gd(knobs.filteringRadius);
-            if (dphase > R) {
-                const Gd f = Gd(1.0 - 0.20 * (double)(dphase / R - Gd(1)));
-                v = v * F(std::max(f, Gd(0)));
-            }
-        } else {
-            const Gd fmin = Gd(knobs.filteringMin), fmax = Gd(knobs.filteringMax);
-            if (dphase > fmax) {
-                if (knobs.filterGuiding) {
-                    const double q = (double)(dphase / fmax);
-                    guidingWeight = guidingWeight + v / F(q * q + 1.0);
-                }
-                v = Cx<F>{F(0), F(0)};
-            }
-            if (dphase > fmin) v = v * F((fmax - dphase) / (fmax - fmin));
+    return dphase / lambda;  // path.h:355
+}
+
+// The filter of path.h:357-387: the soft sphere max(1 - 0.2·(dphase/R - 1), 0) beyond R; the
+// plane's linear ramp from filteringMin to filteringMax and zero beyond, whose cut value may
+// feed the guiding weight.
+template <class Gd, class F>
+void filter_weight(const Knobs &knobs, Gd dphase, Cx<F> &v, Cx<F> &guidingWeight) {
+    if (!knobs.doFiltering) return;
+    if (knobs.filteringSphere) {
+        const Gd R = Gd(knobs.filteringRadius);
+        if (dphase > R) {
+            const Gd f = Gd(1.0 - 0.20 * (double)(dphase / R - Gd(1)));
+            v = v * F(std::max(f, Gd(0)));
         }
+    } else {
+        const Gd fmin = Gd(knobs.filteringMin), fmax = Gd(knobs.filteringMax);
+        if (dphase > fmax) {
+            if (knobs.filterGuiding) {
+                const double q = (double)(dphase / fmax);
+                guidingWeight = guidingWeight + v / F(q * q + 1.0);
+            }
+            v = Cx<F>{F(0), F(0)};
+        }
+        if (dphase > fmin) v = v * F((fmax - dphase) / (fmax - fmin));
     }
+}
+
+// The weight of an NEE contribution v from a bounce vertex (depth > 0, path.h:330-394): the GO
+// density correction v·r/cosθ/4π (path.h:330-341; false = skipped when cosθ < 1e-3), the phase
+// error and the filter, and the guiding weight update gw += v (path.h:392-394). o, d: the
+// vertex and the reflected direction; dn, rr: unit direction and distance to the receiver;
+// d_in, n_in: incoming direction and normal at the vertex; r: path length up to the vertex.
+// dphase_out (may be null) receives dphase.
+template <class Gd, class F>
+bool nee_weight(const Knobs &knobs, Gd r, Gd cosTheta, V3<Gd> o, V3<Gd> d, V3<Gd> dn, Gd rr, V3<Gd> d_in,
+                V3<Gd> n_in, Gd lambda, Cx<F> &v, Cx<F> &guidingWeight, Gd *dphase_out) {
+    const F piF = K<F>::pi();
+    v = v * F(r / cosTheta);
+    v = v / (F(4) * piF);
+    if (cosTheta < Gd(1e-3)) return false;
+    const Gd dphase = phase_error<Gd>(knobs, r, o, d, dn, rr, d_in, n_in, lambda);
+    if (dphase_out) *dphase_out = dphase;
+    filter_weight<Gd, F>(knobs, dphase, v, guidingWeight);
     guidingWeight = guidingWeight + v;  // path.h:392-394
     return true;
 }
 
-template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
+// kDual: beside the Gd chain, the reference's FP32 chain (the ref32 instance's arithmetic) is
+// carried along; it gives the traversal and shadow rays and every discrete decision (hit ids,
+// visibility, grazing and range cut-offs, back-face cull, filter thresholds), while path
+// lengths, time, phase and the GO weight come from the Gd chain refined on the same triangles.
+template <class T, class Gd, class F, class P, bool kPromote, bool kDual = false> struct Tracer {
     static constexpr bool refine = !std::is_same<T, Gd>::value;
     const Mesh *mesh;
     const Accel<T> *acc;
@@ -1008,6 +1041,15 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
         V3<Gd> o = vcast3<Gd>(txAnt.pos);
         V3<Gd> d = txRg.mul(dl);
         CV3<F> H = txRf.mul(Hl);
+        // kDual: the FP32 chain (vertex, direction, incoming direction and normal, length, cosθ)
+        V3<float> o32{}, d32{}, d32_in{}, n32_in{};
+        float r32 = 0.0f, cos32 = 1.0f;
+        const float maxDist32 = (float(rf.adcRate) / float(rf.freqSlope)) * float(kSpeedOfLight);
+        if constexpr (kDual) {
+            const V3<float> dl32 = uniform_sample_sphere<float>(primary[0], primary[1]);
+            o32 = vcast3<float>(txAnt.pos);
+            d32 = mcast<float>(txAnt.rot).mul(dl32);
+        }
         H = (F(1) / F(primaryPdf)) * H;  // ray.weightBy(1 / primaryPdf)
         P time = P(0);
         Gd r = Gd(0), cosTheta = Gd(1);
@@ -1027,7 +1069,15 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
                     const Gd rr = norm(dn);
                     dn = dn / rr;
                     const CV3<F> Hrx = rxRf.mul(awr_evaluate<F, kPromote>(vcast<F>(rxRg.mulT(-dn))));
-                    if (!knobs->poDiffraction && !trace_visible(vcast<T>(o), vcast<T>(dn), T(rr))) {
+                    V3<float> dn32{};
+                    float rr32 = 0.0f;
+                    if constexpr (kDual) {
+                        dn32 = vcast3<float>(scene->rx[ch].pos) - o32;
+                        rr32 = norm(dn32);
+                        dn32 = dn32 / rr32;
+                    }
+                    if (!knobs->poDiffraction && !(kDual ? trace_visible(vcast<T>(o32), vcast<T>(dn32), T(rr32))
+                                                         : trace_visible(vcast<T>(o), vcast<T>(dn), T(rr)))) {
                         if (rec && depth >= 1) rec->nee[(depth - 1) * C + chBase + ch] = 1;
                         continue;
                     }
@@ -1043,7 +1093,7 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
                         // SurfaceEmitter::evaluate (emitter.h:64-70)
                         const CV3<F> J = F(2) * cross(vcast<F>(n_in), H_in);
                         Hn = cross(vcast<F>(dn), J);
-                        if (dot(dn, n_in) < Gd(0)) Hn = CV3<F>{};
+                        if (kDual ? dot(dn32, n32_in) < 0.0f : dot(dn, n_in) < Gd(0)) Hn = CV3<F>{};
                         // SurfaceEmitter::connect (emitter.h:72-81)
                         const Cx<F> green = Cx<F>{F(1) / std::max(rrF, F(1e-3)), k0} / ((F(4) * piF) * rrF);
                         Hn = Hn * green;
@@ -1051,9 +1101,21 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
                     const P tn = time + P(rr) / P(kSpeedOfLight);  // addDistance in connect
                     Cx<F> v = cdot(Hn, Hrx);                       // measureH, geometry.h:138
                     if (depth > 0) {
-                        if (!nee_weight<Gd, F>(*knobs, r, cosTheta, o, d, dn, rr, d_in, n_in, lambda, v,
-                                               guidingWeight, nullptr))
+                        if constexpr (kDual) {
+                            // the GO weight from the Gd chain, the grazing skip, phase error and
+                            // filter from the FP32 chain (as the ref32 instance)
+                            v = v * F(r / cosTheta);
+                            v = v / (F(4) * piF);
+                            if (cos32 < 1e-3f) continue;
+                            const float lambda32 = float(kSpeedOfLight) / float(freq);
+                            const float dp = phase_error<float>(*knobs, r32, o32, d32, dn32, rr32, d32_in, n32_in,
+                                                                lambda32);
+                            filter_weight<float, F>(*knobs, dp, v, guidingWeight);
+                            guidingWeight = guidingWeight + v;
+                        } else if (!nee_weight<Gd, F>(*knobs, r, cosTheta, o, d, dn, rr, d_in, n_in, lambda, v,
+                                                      guidingWeight, nullptr)) {
                             continue;
+                        }
                     }
                     // Integrator::splat skip rule (integrator.h:121-124), off with the debug image
                     if (!knobs->produceDebugImage && (sampleWeight == 0.0f || v.re == F(0) || v.im == F(0))) {
@@ -1066,16 +1128,25 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
                 }
             }
             // random walk (path.h:399-442)
-            if (depth >= maxDepth || r >= maxDist) break;
+            if (depth >= maxDepth || (kDual ? r32 >= maxDist32 : r >= maxDist)) break;
             if (is_zero(H)) break;
             T t;
-            const int32_t id = trace_closest(vcast<T>(o), vcast<T>(d), &t);
+            const int32_t id = kDual ? trace_closest(vcast<T>(o32), vcast<T>(d32), &t)
+                                     : trace_closest(vcast<T>(o), vcast<T>(d), &t);
             if (rec && depth < maxDepth) rec->hits[depth] = id;
             if (id < 0) break;
             V3<Gd> n = normal(id);
             if (dot(n, d) > Gd(0)) n = -n;  // face-forward (cpu.cpp:82-85)
             cosTheta = std::abs(dot(n, -d));
-            if (cosTheta < Gd(1e-3)) break;
+            V3<float> n32{};
+            if constexpr (kDual) {
+                n32 = mesh->f.normal(id);
+                if (dot(n32, d32) > 0.0f) n32 = -n32;
+                cos32 = std::abs(dot(n32, -d32));
+                if (cos32 < 1e-3f) break;
+            } else if (cosTheta < Gd(1e-3)) {
+                break;
+            }
             if (motion) {
                 // the incoming leg: d|p - o|/dt = d . (v_p - v_o), the vertex moving with its triangle
                 const V3<Gd> vh = mesh->vel.empty() ? V3<Gd>{Gd(0), Gd(0), Gd(0)} : vcast3<Gd>(&mesh->vel[3 * (size_t)id]);
@@ -1091,6 +1162,15 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
             d_in = d;
             o = o + tg * d;
             depth++;
+            if constexpr (kDual) {
+                const float t32 = float(t);
+                r32 = r32 + t32;
+                n32_in = n32;
+                d32_in = d32;
+                o32 = o32 + t32 * d32;
+                const V3<float> wi32 = -d32;
+                d32 = (2.0f * dot(n32, wi32)) * n32 - wi32;
+            }
             // reflectRay (integrator.h:164-168) with Intersection::R (intersection.h:38)
             const V3<Gd> wi = -d;
             d = (Gd(2) * dot(n, wi)) * n - wi;
@@ -1110,7 +1190,7 @@ template <class T, class Gd, class F, class P, bool kPromote> struct Tracer {
 // ---------------------------------------------------------------------------------------
 // Backend::run (cpu.h:21-41): a std::thread pool grabbing batches of 256 consecutive
 // indices under a mutex; one accumulation frame per worker, summed after the join.
-template <class T, class Gd, class F, class P, bool kPromote>
+template <class T, class Gd, class F, class P, bool kPromote, bool kDual = false>
 void render(const Mesh &mesh, const SceneC &scene, const Knobs &knobs, const FrameCfg &fc, int64_t offset,
             int64_t count, float sampleWeight, bool isFinal, Guide *guide, int nthreads, double *frame_out,
             double *totalWeight, int32_t *hit_ids, int8_t *nee, int64_t record_paths, Counters *counters) {
@@ -1124,7 +1204,7 @@ void render(const Mesh &mesh, const SceneC &scene, const Knobs &knobs, const Fra
     std::mutex mtx;
     int64_t next = 0;
     auto worker = [&](int tid) {
-        Tracer<T, Gd, F, P, kPromote> tr;
+        Tracer<T, Gd, F, P, kPromote, kDual> tr;
         tr.mesh = &mesh;
         tr.acc = &accel_of<T>(mesh);
         tr.scene = &scene;
@@ -1198,6 +1278,7 @@ static RenderFn render_for(int prec) {
         case 0: return &render<float, float, float, float, true>;
         case 1: return &render<float, double, float, double, false>;
         case 3: return &render<float, float, float, double, false>;
+        case 4: return &render<float, double, float, double, false, true>;
         default: return &render<double, double, double, double, true>;
     }
 }

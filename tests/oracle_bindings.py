@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
-PREC_REF32, PREC_MIXED, PREC_FP64, PREC_CHAIN32 = 0, 1, 2, 3
+PREC_REF32, PREC_MIXED, PREC_FP64, PREC_CHAIN32, PREC_DUAL = 0, 1, 2, 3, 4
 
 _lib = None
 

@@ -187,3 +187,31 @@ def test_guide_refines_where_the_mass_is(oracle):
     g = trained_guide(oracle, rounds=4)
     child, p0, _ = g.get(1)  # training tree after the last refine
     assert len(child) > len(g.get(0)[0])
+
+
+def test_dual_chain_instance_follows_ref32_decisions(oracle):
+    """The dual instance (the product's HSR_OPT_CHAIN = 1 model) takes every hit id and shadow
+    outcome from the reference's FP32 chain: identical records to ref32 on guided clutter
+    paths; where those agree with the FP64 instance's, its frame is FP64-accurate."""
+    mesh, _ = scenes.vehicle_mesh(seed=1, triangles=20_000, range_lo=8.0, range_hi=12.0)
+    m = oracle.OracleMesh.from_mesh(mesh)
+    sc = scenes.radar_array(scenes.dihedral_rf(), channels=4, position=(0.0, 0.75, 0.0))
+    kn = H.Knobs(maxDepth=4, doGuiding=True)
+    fc = H.FrameConfig(1, 256, 4)
+    g = oracle.OracleGuide()
+    for _ in range(3):
+        oracle.render(m, sc, kn, fc, prec=oracle.PREC_MIXED, count=1 << 15, guide=g)
+        g.step()
+    n = 1 << 15
+    out = {}
+    for p in (oracle.PREC_REF32, oracle.PREC_DUAL, oracle.PREC_FP64):
+        z, tw, rec, _ = oracle.render(m, sc, kn, fc, prec=p, count=n, record_paths=n, guide=g, is_final=True)
+        out[p] = (z / tw, rec)
+    np.testing.assert_array_equal(out[oracle.PREC_DUAL][1][0], out[oracle.PREC_REF32][1][0])
+    np.testing.assert_array_equal(out[oracle.PREC_DUAL][1][1], out[oracle.PREC_REF32][1][1])
+    assert (out[oracle.PREC_DUAL][1][0][:, 0] >= 0).sum() > n // 4
+    same = np.array_equal(out[oracle.PREC_DUAL][1][0], out[oracle.PREC_FP64][1][0])
+    zd, zf, z32 = out[oracle.PREC_DUAL][0], out[oracle.PREC_FP64][0], out[oracle.PREC_REF32][0]
+    if same:
+        assert rel_l2(zd, zf) < 1e-5
+    assert rel_l2(zd, zf) < rel_l2(z32, zf)
