@@ -308,51 +308,74 @@ def serving_trace(count, rate, seed=3):
     return out
 
 
+def serving_run(srv, ids, trace, rf, knobs, rate, guide_mode):
+    """The trace replayed open-loop at `rate` requests/s (each request submitted at its arrival
+    time); returns the server's statistics over it."""
+    reqs = []
+    for q in trace:
+        sc = scenes.radar_array(rf, channels=q["channels"], position=q["pos"])
+        reqs.append((ids[q["scene"]], sc, H.FrameConfig(1, q["bins"], q["channels"]), q["paths"]))
+    srv.reset_stats()
+    t0 = time.perf_counter()
+    tickets = []
+    for q, (sid, sc, fc, n) in zip(trace, reqs):
+        delay = t0 + q["t"] * 100.0 / rate - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(srv.submit(sid, sc, knobs, fc, n, guide_mode=guide_mode))
+    for t in tickets:
+        srv.wait(t, fetch=False)
+    st = srv.stats()
+    return {"frames_per_s": st["frames_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "mean_ms": st["mean_ms"], "completed": st["completed"], "rejected": st["rejected"],
+            "memory_waits": st["memory_waits"]}
+
+
 def serving_bench(args):
-    """Requests of the serving mix replayed open-loop (each submitted at its arrival time)
-    against a server with 1 slot (one frame at a time, the reference's blocking run) and
-    with `--slots` slots; frames/s and p50/p99 latency of each."""
-    from hussar_amd.serving import Server
+    """The serving mix replayed open-loop at 50, 100, 200 and 500 requests/s in three forms:
+     * plain: one frame at a time (1 slot), each request the reference's PathTracer::run with
+       its doubling guiding schedule (training iterations, then the final iteration);
+     * concurrent: `--slots` slots, still PathTracer::run per request;
+     * served: `--slots` slots, each scene's resident guiding tree (trained once per scene,
+       before the trace, by one request at the scene's nominal pose) and every request's whole
+       budget as one final pass with it.
+    Frames/s, p50/p99 latency, rejections and memory waits of each."""
+    from hussar_amd.serving import GUIDE_PER_REQUEST, GUIDE_SCENE, GUIDE_TRAIN, Server
     rf = scenes.dihedral_rf()
     meshes = [scenes.vehicle_mesh(seed=10 + k, triangles=n)[0] for k, n in enumerate((10_000, 300_000, 2_000_000))]
-    trace = serving_trace(args.requests, args.rate)
+    trace = serving_trace(args.requests, 100.0)  # arrival times at 100/s, rescaled per rate
     knobs = H.Knobs(maxDepth=4, doGuiding=True)
-    results = {}
-    for slots in (1, args.slots):
+    rates = [args.rate] if args.rate > 0 else [50.0, 100.0, 200.0, 500.0]
+    forms = {"plain": (1, GUIDE_PER_REQUEST), "concurrent": (args.slots, GUIDE_PER_REQUEST),
+             "served": (args.slots, GUIDE_SCENE)}
+    results = {name: {} for name in forms}
+    for name, (slots, mode) in forms.items():
         srv = Server(slots=slots)
         ids = [srv.open_scene(m) for m in meshes]
-        scs = []
-        for q in trace:
-            sc = scenes.radar_array(rf, channels=q["channels"], position=q["pos"])
-            scs.append((ids[q["scene"]], sc, H.FrameConfig(1, q["bins"], q["channels"]), q["paths"]))
-        for sid in ids:  # warm-up: one small request per scene
-            srv.wait(srv.submit(sid, scs[0][1], knobs, scs[0][2], 1 << 16), fetch=False)
-        srv.reset_stats()
-        t0 = time.perf_counter()
-        tickets = []
-        for q, (sid, sc, fc, n) in zip(trace, scs):
-            delay = t0 + q["t"] - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            tickets.append(srv.submit(sid, sc, knobs, fc, n))
-        for t in tickets:
-            srv.wait(t, fetch=False)
-        st = srv.stats()
-        results[slots] = {"frames_per_s": st["frames_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-                          "mean_ms": st["mean_ms"], "completed": st["completed"]}
+        nominal = scenes.radar_array(rf, channels=4, position=(0.0, 0.75, 0.0))
+        for sid in ids:  # warm-up, and the scenes' resident trees
+            srv.wait(srv.submit(sid, nominal, knobs, H.FrameConfig(1, 256, 4), 1 << 20,
+                                guide_mode=GUIDE_TRAIN if mode == GUIDE_SCENE else GUIDE_PER_REQUEST), fetch=False)
+        for rate in rates:
+            results[name][str(int(rate))] = serving_run(srv, ids, trace, rf, knobs, rate, mode)
+            print(f"serving: {name} at {rate:.0f}/s: {results[name][str(int(rate))]['frames_per_s']:.1f} frames/s",
+                  file=sys.stderr, flush=True)
         srv.close()
     paths = sum(q["paths"] for q in trace)
-    base, best = results[1], results[args.slots]
+    key = str(int(rates[-1]))
+    served, plain = results["served"][key], results["plain"][key]
     return {
-        "metric": "serving frames/s on the mixed-request trace (one GPU)", "value": best["frames_per_s"],
+        "metric": "serving frames/s on the mixed-request trace (one GPU)", "value": served["frames_per_s"],
         "unit": "frames/s", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "data": "synthetic",
         "dtype": "f32 traversal+fields / f64 geometry chain+phase+sums",
-        "config": {"workload": f"serving mix: scenes of 10k / 300k / 2M triangles, Poisson arrivals at {args.rate}/s, "
-                               "pose within 20 m, bins U{128..512}, receivers U{1..12}, paths 2^U{18..24}, depth 4, "
-                               "guiding on (PathTracer::run schedule per request)",
-                   "requests": args.requests, "paths_total": paths, "slots": args.slots},
-        "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"], "mean": best["mean_ms"]},
-        "one_frame_at_a_time": base, "speedup_vs_one_frame_at_a_time": best["frames_per_s"] / base["frames_per_s"],
+        "config": {"workload": "serving mix: scenes of 10k / 300k / 2M triangles, Poisson arrivals (seed 3) "
+                               "at each rate, pose within 20 m, bins U{128..512}, receivers U{1..12}, paths "
+                               "2^U{18..24}, depth 4, guiding on",
+                   "requests": args.requests, "paths_total": paths, "slots": args.slots, "rates": rates},
+        "latency_ms": {"p50": served["p50_ms"], "p99": served["p99_ms"], "mean": served["mean_ms"]},
+        "speedup_vs_plain": {r: results["served"][r]["frames_per_s"] / results["plain"][r]["frames_per_s"]
+                             for r in results["served"]},
+        "forms": results,
     }
 
 
@@ -495,7 +518,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--pipeline", type=int, default=1, help="0 fused path kernel, 1 wavefront stages")
-    ap.add_argument("--rate", type=float, default=100.0, help="serving: request arrivals per second")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serving: request arrivals per second (0: 50, 100, 200 and 500 in turn)")
     ap.add_argument("--paths", type=int, default=1 << 26, help="urban: paths per frame")
     ap.add_argument("--requests", type=int, default=96, help="serving: requests in the trace")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent slots")

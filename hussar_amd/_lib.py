@@ -63,6 +63,7 @@ SIGNATURES = {
     "hsr_server_set_velocities": (C.c_int, [_vp, C.c_int32, _P(C.c_float), C.c_int64]),
     "hsr_server_close_scene": (C.c_int, [_vp, C.c_int32]),
     "hsr_server_submit": (C.c_int, [_vp, _P(c_request), _P(C.c_int64)]),
+    "hsr_server_request_bytes": (C.c_int, [_vp, _P(c_request), _P(C.c_int64)]),
     "hsr_server_poll": (C.c_int, [_vp, C.c_int64, _P(C.c_int32)]),
     "hsr_server_wait": (C.c_int, [_vp, C.c_int64, _P(C.c_float), C.c_int64, _P(c_request_info)]),
     "hsr_server_get_stats": (C.c_int, [_vp, _P(c_server_stats)]),
@@ -78,7 +79,7 @@ class HsrError(RuntimeError):
         self.status = status
 
 
-ABI_VERSION = 2  # HSR_ABI_VERSION of include/hussar_hip.h
+ABI_VERSION = 3  # HSR_ABI_VERSION of include/hussar_hip.h
 
 
 def lib():

@@ -160,6 +160,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_events);
     dfree(b->d_evCount);
     dfree(b->d_cubeParts);
+    dfree(b->d_statRows);
     dfree(b->d_recHits);
     dfree(b->d_recNee);
     dfree(b->d_vel);
@@ -244,6 +245,58 @@ hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64
     // events per path of a batch (more go straight to the frame with device atomics)
     return backend_events(b, kCubeReserveParts * frameDoubles,
                           (unsigned)std::min<int64_t>(kCubeEvents, 4 * P));
+}
+
+// Device bytes of a backend's work buffers: wavefront state, frame, tiles and stages, the cube
+// event list and partial cubes, statistics rows, post-processing and hit records.
+int64_t backend_device_bytes(const hsr_backend *b) {
+    return (int64_t)b->wfBytes + (int64_t)b->partialCap * 8 + (int64_t)b->stageCap * 16 + b->frameCap * 16 +
+           (int64_t)b->evCap * (int64_t)sizeof(CubeEvent) + b->cubePartsCap * 8 + (b->d_statRows ? kStatRows * 8 : 0) +
+           b->postCap * 16 + b->recCap * 5;
+}
+
+// The bytes backend_reserve(b, P, C, maxDepth, frameDoubles) leaves the backend holding (its
+// buffers only grow), plus the statistics rows of a guided training run.
+int64_t backend_reserve_bytes(const hsr_backend *b, int64_t P, int C, int maxDepth, int64_t frameDoubles,
+                              bool training) {
+    const int64_t tileDoubles = b->ldsPerBlock / (int64_t)sizeof(double);
+    const int64_t wf = std::max<int64_t>((int64_t)b->wfBytes, (int64_t)wavefront_bytes(P, C, maxDepth));
+    const int64_t part = std::max<int64_t>((int64_t)b->partialCap, (int64_t)b->cu * 2 * tileDoubles) * 8;
+    const int64_t stage = std::max<int64_t>((int64_t)b->stageCap, 64 * tileDoubles) * 16;
+    const int64_t frame = std::max<int64_t>(b->frameCap, frameDoubles) * 16;
+    const int64_t ev = std::max<int64_t>(b->evCap, std::min<int64_t>(kCubeEvents, 4 * P)) * (int64_t)sizeof(CubeEvent);
+    const int64_t parts = std::max<int64_t>(b->cubePartsCap, kCubeReserveParts * frameDoubles) * 8;
+    const int64_t rows = (b->d_statRows || training) ? kStatRows * 8 : 0;
+    return wf + part + stage + frame + ev + parts + rows + b->postCap * 16 + b->recCap * 5;
+}
+
+// Frees a backend's work buffers (not its mesh, events or guide); the next run allocates again.
+void backend_release(hsr_backend *b) {
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    if (b->d_wf) (void)hipFree(b->d_wf);
+    b->d_wf = nullptr;
+    b->wfBytes = 0;
+    dfree(b->d_partials);
+    b->partialCap = 0;
+    dfree(b->d_stage_sum);
+    dfree(b->d_stage_comp);
+    b->stageCap = 0;
+    dfree(b->d_frame_sum);
+    dfree(b->d_frame_comp);
+    b->frameCap = 0;
+    b->configured = false;
+    dfree(b->d_events);
+    b->evCap = 0;
+    dfree(b->d_cubeParts);
+    b->cubePartsCap = 0;
+    dfree(b->d_statRows);
+    dfree(b->d_post);
+    dfree(b->d_post2);
+    b->postCap = 0;
+    dfree(b->d_recHits);
+    dfree(b->d_recNee);
+    b->recCap = 0;
 }
 
 int64_t backend_mesh_bytes(const hsr_backend *b) {
@@ -623,6 +676,16 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             b->stageCap = (size_t)S * MD;
         }
         if (!cube) HSR_HIP(hsr::launch_zero(b->d_partials, (int64_t)gridGather * MD, b->stream));
+        // a training batch records each path's statistic at its slot (cleared here), summed per
+        // leaf in a fixed order after the batch
+        const bool training = guide && knobs->doGuiding && !isFinalIteration;
+        if (training) {
+            if (!b->d_statRows) {
+                hsr_status s = dalloc(&b->d_statRows, (size_t)hsr::kStatRows);
+                if (s != HSR_OK) return s;
+            }
+            HSR_HIP(hsr::launch_zero(w.statV, n, b->stream));
+        }
         HSR_HIP(hipEventRecord(b->ev[0], b->stream));
         if (first) HSR_HIP(hipEventRecord(b->ev[2], b->stream));
         first = false;
@@ -647,6 +710,9 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         if (a.events)
             HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
                                            (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));
+        if (training)
+            HSR_HIP(hsr::launch_stat_reduce(w.statV, w.statL, n, guide->host.training.size(), b->d_statRows,
+                                            hsr::kStatRows, guide->d_tStat, b->stream));
         HSR_HIP(hipEventRecord(b->ev[1], b->stream));
         // the caller steps the guide next: its statistics' download goes with the last batch,
         // ahead of the frame reduction, which then runs while the host refines the trees

@@ -97,6 +97,7 @@ struct hsr_backend {
     unsigned evCap = 0;
     unsigned *d_evCount = nullptr;
     double *d_cubeParts = nullptr;  // partial cubes of the tile splat (grow-only)
+    double *d_statRows = nullptr;   // partial rows of the training statistics (kStatRows doubles)
     int64_t cubePartsCap = 0;       // doubles
     int32_t *d_recHits = nullptr;
     int8_t *d_recNee = nullptr;
@@ -146,5 +147,11 @@ hsr_status backend_reserve(hsr_backend *b, int64_t P, int C, int maxDepth, int64
 hsr_status backend_events(hsr_backend *b, int64_t partDoubles, unsigned events);
 // Device bytes of a backend's mesh (BVH nodes, triangle slots, velocities).
 int64_t backend_mesh_bytes(const hsr_backend *b);
+// Device bytes of its work buffers, what they would be after backend_reserve (grow-only), and
+// their release.
+int64_t backend_device_bytes(const hsr_backend *b);
+int64_t backend_reserve_bytes(const hsr_backend *b, int64_t P, int C, int maxDepth, int64_t frameDoubles,
+                              bool training);
+void backend_release(hsr_backend *b);
 }  // namespace hsr
 

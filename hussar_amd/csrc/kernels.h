@@ -144,6 +144,9 @@ struct WfArgs {
     // primary sample square, cells in Morton order (coherence_cells)
     const int32_t *cells;
     int32_t *cellStart;  // [kCells + 1] first slot of each cell in this batch
+    // guiding record of a training batch: each path's statistic and leaf at its slot
+    double *statV;
+    int32_t *statL;
 };
 constexpr int kCells = 125 * 243;  // cells of the coherence order: 5^3 x 3^5
 // The kCells residues of the coherence order, cells in Morton order (host).
@@ -164,6 +167,13 @@ hipError_t launch_wavefront(const TraceArgs &args, const WfArgs &w, int gridTrav
                             hipEvent_t *evFork, hipEvent_t *evJoin, hipEvent_t *evClosest, int *nClosest,
                             hipEvent_t *evShadow, int *nShadow);
 int wavefront_trav_blocks_per_cu(bool count);
+// Training statistics of a batch: stat[leaf] += the records (statV, statL) of slots [0, n) of
+// each leaf, summed in a fixed order through up to rowsCap / nodes partial rows (rows: rowsCap
+// doubles) of a training tree of `nodes` nodes.
+constexpr int kStatLdsNodes = 5120;  // trees up to this size are summed in LDS accumulators
+constexpr int64_t kStatRows = int64_t(1) << 21;  // doubles of partial rows a backend holds
+hipError_t launch_stat_reduce(const double *statV, const int32_t *statL, int64_t n, int nodes, double *rows,
+                              int64_t rowsCap, double *stat, hipStream_t stream);
 size_t gather_lds_bytes(int N, int C);
 // partials [W][M] doubles -> stage [S][M] (sum, comp) -> frame (sum, comp) += Kahan
 hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum, double *stage_comp, int S,

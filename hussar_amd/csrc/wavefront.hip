@@ -251,21 +251,20 @@ __device__ inline void store_hit(const WfArgs &w, int64_t i, const PathState &p)
     f[(F_H + 4) * P + i] = p.H.z.re; f[(F_H + 5) * P + i] = p.H.z.im;
 }
 
-// Guiding record of a finished path (path.h:448-450), wave-aggregated.
-__device__ inline void finish_record(const TraceArgs &a, bool finishing, const PathState &p, int lane) {
+// Guiding record of a finished path (path.h:448-450): its training statistic and leaf at the
+// path's own slot (statV / statL; statV is cleared when a training batch starts). After the
+// batch, k_stat_reduce sums them per leaf in a fixed order, so the statistics do not depend on
+// the order in which the paths of a batch finish.
+__device__ inline void finish_record(const TraceArgs &a, const WfArgs &w, bool finishing, int64_t i,
+                                     const PathState &p) {
     if (!(a.doGuiding && a.guideOn && !a.isFinal)) return;  // uniform
-    bool pend = false;
-    int32_t leaf = 0;
-    double val = 0.0;
     if (finishing && p.pdf > 0.0f) {
         const float density = cabsf_(p.gw) * p.pdf;
         const float weight = 1.0f / p.pdf;
-        leaf = tree_leaf(a.gtChild, p.x0, p.x1);
         const double dd = (double)density;
-        val = (a.secondMoment ? dd * dd : dd) * (double)weight;
-        pend = true;
+        w.statV[i] = (a.secondMoment ? dd * dd : dd) * (double)weight;
+        w.statL[i] = tree_leaf(a.gtChild, p.x0, p.x1);
     }
-    record_stats(pend, leaf, val, a.gtStat + (blockIdx.x % kStatCopies) * a.gtStride, lane);
 }
 
 // A live path's vertex rounded to float (the origin of its shadow rays and of its extension
@@ -390,7 +389,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int 
             stage_append(l_ext, &sn[1], valid && !ends, (int32_t)i, lane);
             write_extend<kRef>(w, valid, ends, i, p);
             if (valid) store_state(w, i, p);
-            finish_record(a, valid && ends && !nee, p, lane);
+            finish_record(a, w, valid && ends && !nee, i, p);
         }
         __syncthreads();
         stage_reserve(&sn[0], &sb[0], &sc[0], &cnt->n_gat.v, 0);
@@ -737,7 +736,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
             write_extend<kRef>(w, alive, ends, i, p);
             if (alive) store_hit(w, i, p);
             if (alive && a.motion) store_motion(w, i, p);
-            finish_record(a, valid && !alive, p, lane);
+            finish_record(a, w, valid && !alive, i, p);
             // NEE set-up at the new vertex (path.h:290-397 of the next bounce)
             if (alive) p.freq = w.sf[F_FREQ * w.P + i];
             for (int g = 0; g < groups; ++g) {
@@ -881,7 +880,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceA
             w.sf[F_GWI * P + i] = p.gw.im;
         }
         const bool finishing = valid && (w.si[I_FLAGS * P + i] & 1);
-        finish_record(a, finishing, p, lane);
+        finish_record(a, w, finishing, i, p);
     }
     if (count) wave_count(&a.counters[4], cev, lane);
     if (cube) return;
@@ -973,6 +972,94 @@ __global__ void k_cube_reduce(double2 *__restrict__ frame, const double2 *__rest
     frame[i] = f;
 }
 
+// ------------------------------------------------------------------------ training statistics
+// The per-slot records of a batch (statV, statL over slots [0, n)) summed per leaf into
+// rows[blockIdx.x][nodes]: workgroup g takes the slots [g·n/G, (g+1)·n/G), each of its waves a
+// contiguous quarter, 64 slots at a time in slot order; the records of one leaf among the 64
+// are summed by a butterfly (zeros for the other lanes) and added by lane 0 to the wave's LDS
+// accumulator. The four accumulators are then added in wave order into the row. Every sum has
+// a fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(kWfBlock) void k_stat_reduce(const double *__restrict__ statV,
+                                                          const int32_t *__restrict__ statL, int64_t n, int nodes,
+                                                          double *__restrict__ rows) {
+    extern __shared__ double acc[];  // [4][nodes]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = threadIdx.x; j < 4 * nodes; j += kWfBlock) acc[j] = 0.0;
+    __syncthreads();
+    const int64_t lo = n * blockIdx.x / gridDim.x, hi = n * (blockIdx.x + 1) / gridDim.x;
+    const int64_t wlo = lo + (hi - lo) * wave / 4, whi = lo + (hi - lo) * (wave + 1) / 4;
+    double *my = acc + (size_t)wave * nodes;
+    for (int64_t b = wlo; b < whi; b += 64) {
+        const int64_t i = b + lane;
+        double val = 0.0;
+        int32_t leaf = 0;
+        if (i < whi) {
+            val = statV[i];
+            leaf = statL[i];
+        }
+        bool pend = val != 0.0;
+        uint64_t m = __ballot(pend);
+        while (m) {
+            const int src = __builtin_ctzll(m);
+            const int lw = __builtin_amdgcn_readlane(leaf, src);
+            const bool mine = pend && leaf == lw;
+            double sum = mine ? val : 0.0;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+            if (lane == 0) my[lw] += sum;
+            m &= ~__ballot(mine);
+            pend = pend && !mine;
+        }
+    }
+    __syncthreads();
+    double *row = rows + (int64_t)blockIdx.x * nodes;
+    for (int j = threadIdx.x; j < nodes; j += kWfBlock)
+        row[j] = ((acc[j] + acc[nodes + j]) + acc[2 * nodes + j]) + acc[3 * nodes + j];
+}
+
+// The same for trees too large for LDS accumulators: one wave per workgroup, accumulating into
+// its own row in device memory (lane 0 is the only writer, in slot order).
+__global__ __launch_bounds__(64) void k_stat_reduce_mem(const double *__restrict__ statV,
+                                                        const int32_t *__restrict__ statL, int64_t n, int nodes,
+                                                        double *__restrict__ rows) {
+    const int lane = threadIdx.x;
+    double *row = rows + (int64_t)blockIdx.x * nodes;
+    for (int j = lane; j < nodes; j += 64) row[j] = 0.0;
+    __syncthreads();
+    const int64_t lo = n * blockIdx.x / gridDim.x, hi = n * (blockIdx.x + 1) / gridDim.x;
+    for (int64_t b = lo; b < hi; b += 64) {
+        const int64_t i = b + lane;
+        double val = 0.0;
+        int32_t leaf = 0;
+        if (i < hi) {
+            val = statV[i];
+            leaf = statL[i];
+        }
+        bool pend = val != 0.0;
+        uint64_t m = __ballot(pend);
+        while (m) {
+            const int src = __builtin_ctzll(m);
+            const int lw = __builtin_amdgcn_readlane(leaf, src);
+            const bool mine = pend && leaf == lw;
+            double sum = mine ? val : 0.0;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+            if (lane == 0) row[lw] += sum;
+            m &= ~__ballot(mine);
+            pend = pend && !mine;
+        }
+    }
+}
+
+// stat[j] += rows[0][j] + rows[1][j] + ... (in row order)
+__global__ void k_stat_rows_fold(const double *__restrict__ rows, int G, int nodes, double *__restrict__ stat) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nodes) return;
+    double sum = 0.0;
+    for (int g = 0; g < G; ++g) sum += rows[(int64_t)g * nodes + j];
+    stat[j] += sum;
+}
+
 __global__ void k_wf_zero_counts(WfCounts *c, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -1004,6 +1091,7 @@ size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
     b += 3 * align256(sizeof(int32_t) * P) + align256(sizeof(int32_t) * S);
     b += align256(sizeof(WfCounts) * (maxDepth + 2));
     b += align256(sizeof(int32_t) * (kCells + 1));
+    b += align256(sizeof(double) * P) + align256(sizeof(int32_t) * P);
     return b;
 }
 
@@ -1035,6 +1123,8 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.q_shd = (int32_t *)take(sizeof(int32_t) * S);
     w.cnt = (WfCounts *)take(sizeof(WfCounts) * (maxDepth + 2));
     w.cellStart = (int32_t *)take(sizeof(int32_t) * (kCells + 1));
+    w.statV = (double *)take(sizeof(double) * P);
+    w.statL = (int32_t *)take(sizeof(int32_t) * P);
 }
 
 size_t gather_lds_bytes(int N, int C) {
@@ -1095,6 +1185,28 @@ hipError_t launch_cube_splat(const CubeEvent *events, unsigned *count, unsigned 
     const int64_t cells = (int64_t)M * N * C;
     hipLaunchKernelGGL(k_cube_reduce, dim3((unsigned)((cells + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s,
                        (double2 *)frame, (const double2 *)parts, G, cells, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_stat_reduce(const double *statV, const int32_t *statL, int64_t n, int nodes, double *rows,
+                              int64_t rowsCap, double *stat, hipStream_t s) {
+    (void)hipGetLastError();
+    if (n <= 0 || nodes <= 0) return hipSuccess;
+    const size_t lds = (size_t)4 * nodes * sizeof(double);
+    const bool inLds = nodes <= kStatLdsNodes;
+    const int64_t perBlock = inLds ? 4096 : 1024;
+    const int G = (int)std::max<int64_t>(1, std::min<int64_t>({(n + perBlock - 1) / perBlock, rowsCap / nodes, 1024}));
+    if (inLds) {
+        if (lds > 65536) {
+            const hipError_t e = hipFuncSetAttribute((const void *)k_stat_reduce,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_stat_reduce, dim3(G), dim3(kWfBlock), lds, s, statV, statL, n, nodes, rows);
+    } else {
+        hipLaunchKernelGGL(k_stat_reduce_mem, dim3(G), dim3(64), 0, s, statV, statL, n, nodes, rows);
+    }
+    hipLaunchKernelGGL(k_stat_rows_fold, dim3((nodes + 255) / 256), dim3(256), 0, s, rows, G, nodes, stat);
     return hipGetLastError();
 }
 

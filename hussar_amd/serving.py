@@ -5,8 +5,9 @@
     t = srv.submit(sid, scene, knobs, FrameConfig(1, 256, 4), paths=1 << 20)
     frame, info = srv.wait(t)      # complex64 [chirp, sample, channel], per-request timings
 
-Scenes stay resident on the device; each request is a full PathTracer::run on a cleared frame
-in one of the server's slots (own stream, buffers and guide). There is no CPU fallback.
+Scenes stay resident on the device, with a guiding tree once trained; each request runs on a
+cleared frame in one of the server's slots (own stream, buffers and guide), admitted when its
+own device footprint fits. There is no CPU fallback.
 """
 from __future__ import annotations
 
@@ -19,6 +20,9 @@ import numpy as np
 from ._lib import check, lib
 from .types import (FrameConfig, Knobs, Scene, TriangleMesh, c_request, c_request_info, c_server_config,
                     c_server_stats)
+
+
+GUIDE_PER_REQUEST, GUIDE_SCENE, GUIDE_TRAIN = 0, 1, 2  # hsr_request.guideMode
 
 
 @dataclass
@@ -58,13 +62,31 @@ class Server:
     def close_scene(self, scene_id: int) -> None:
         check(lib().hsr_server_close_scene(self._h, scene_id))
 
-    def submit(self, scene_id: int, scene: Scene, knobs: Knobs, frame: FrameConfig, paths: int) -> int:
+    @staticmethod
+    def _request(scene_id, scene, knobs, frame, paths, guide_mode) -> c_request:
         r = c_request()
         r.sceneId = scene_id
+        r.guideMode = int(guide_mode)
         r.paths = int(paths)
         r.scene = scene.to_c()
         r.knobs = knobs.to_c()
         r.frame = frame.to_c()
+        return r
+
+    def request_bytes(self, scene_id: int, scene: Scene, knobs: Knobs, frame: FrameConfig, paths: int,
+                      guide_mode: int = 0) -> int:
+        """The request's admission footprint: device bytes it needs on an empty slot."""
+        n = C.c_int64()
+        r = self._request(scene_id, scene, knobs, frame, paths, guide_mode)
+        check(lib().hsr_server_request_bytes(self._h, C.byref(r), C.byref(n)))
+        return n.value
+
+    def submit(self, scene_id: int, scene: Scene, knobs: Knobs, frame: FrameConfig, paths: int,
+               guide_mode: int = 0) -> int:
+        """guide_mode: GUIDE_PER_REQUEST (the reference's schedule per request), GUIDE_SCENE (one
+        final pass with the scene's resident tree; trains it first if the scene has none) or
+        GUIDE_TRAIN (the reference's schedule, whose tree becomes the scene's)."""
+        r = self._request(scene_id, scene, knobs, frame, paths, guide_mode)
         t = C.c_int64()
         check(lib().hsr_server_submit(self._h, C.byref(r), C.byref(t)))
         self._shapes[t.value] = (max(1, frame.chirpCount), frame.samplesPerChirp, frame.channelCount)

@@ -85,7 +85,7 @@ class c_server_config(C.Structure):
 
 
 class c_request(C.Structure):
-    _fields_ = [("sceneId", C.c_int32), ("reserved0", C.c_int32), ("paths", C.c_int64), ("scene", c_scene),
+    _fields_ = [("sceneId", C.c_int32), ("guideMode", C.c_int32), ("paths", C.c_int64), ("scene", c_scene),
                 ("knobs", c_knobs), ("frame", c_frame_config), ("reserved1", C.c_int32)]
 
 
@@ -98,7 +98,8 @@ class c_server_stats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
                 ("queued", C.c_int64), ("running", C.c_int64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_ms", C.c_double), ("max_ms", C.c_double), ("frames_per_s", C.c_double),
-                ("hbm_used", C.c_int64), ("hbm_budget", C.c_int64), ("scenes", C.c_int32), ("slots", C.c_int32)]
+                ("hbm_used", C.c_int64), ("hbm_budget", C.c_int64), ("scenes", C.c_int32), ("slots", C.c_int32),
+                ("rejected", C.c_int64), ("memory_waits", C.c_int64)]
 
 
 @dataclass

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define HSR_ABI_VERSION 2
+#define HSR_ABI_VERSION 3
 #define HSR_MAX_CHANNELS 16
 #define HSR_MAX_TX 4
 
@@ -340,20 +340,38 @@ hsr_status hsr_hip_device_info(int32_t device, hsr_device_info *out);
  * Serving runtime (no reference counterpart; the reference's only analogue is the
  * visualizer's interruptible re-render loop, visualizer/src/Simulator.h:128-183).
  *
- * A server keeps scenes resident on one device (BVH built once per scene) and runs frame
- * requests on `slots` workers, each with its own HIP stream, wavefront buffers, frame and
- * guide, all allocated when the server is created: nothing is allocated while requests
- * run. Requests wait in one FIFO queue; a free slot takes the next one, so requests of
- * different scenes and frame shapes run concurrently and a short request does not wait for a
- * long one to finish. Each request is a complete PathTracer::run (path.h:171-224) with the
- * request's knobs (the guiding schedule included) on a cleared frame.
+ * A server keeps scenes resident on one device (BVH built once per scene, and a guiding tree
+ * once trained) and runs frame requests on `slots` workers, each with its own HIP stream,
+ * wavefront buffers, frame and guide. Requests wait in one FIFO queue; a free slot takes the
+ * next one, so requests of different scenes and frame shapes run concurrently and a short
+ * request does not wait for a long one to finish.
  *
- * Admission: the device bytes of the slots (reserved at create) plus those of the resident
- * scenes must stay within hbmBudget (hsr_server_open_scene returns HSR_ERR_OOM otherwise);
- * a request whose frame exceeds a slot's frame reservation (frameReserveBytes) or whose
- * receivers, chirps or depth exceed the slot's reservation is refused at submit with
- * HSR_ERR_OOM / HSR_ERR_UNSUPPORTED; a full queue returns HSR_ERR_BUSY.
- * All functions are thread-safe. */
+ * Admission by footprint: a request needs the device bytes of its own wavefront state (its
+ * batch of min(paths, pathsPerBatch) paths with its channels and depth), frame, tiles and
+ * event list. It starts when a slot is free and those bytes fit in hbmBudget beside the
+ * resident scenes and the other slots' buffers (a slot's buffers grow to the largest request
+ * it ran; idle slots give theirs back when the head of the queue needs the room); otherwise it
+ * waits, and the requests behind it wait too (FIFO), until memory frees up. A request that
+ * would not fit beside the resident scenes even alone is refused at submit with HSR_ERR_OOM
+ * (counted as rejected), as is one whose frame exceeds frameReserveBytes when that is set;
+ * hsr_server_open_scene returns HSR_ERR_OOM for a scene that does not fit; a full queue
+ * returns HSR_ERR_BUSY. All functions are thread-safe.
+ *
+ * Guiding (knobs.doGuiding = 1), by hsr_request.guideMode:
+ *  HSR_GUIDE_PER_REQUEST  the reference's PathTracer::run (path.h:171-224) on a cleared frame
+ *                         with the slot's guide reset: the doubling schedule's training
+ *                         iterations, then the final iteration;
+ *  HSR_GUIDE_TRAIN        the same, and the trained sampling tree becomes the scene's resident
+ *                         tree (replacing any);
+ *  HSR_GUIDE_SCENE        with a resident tree: the whole budget is one final iteration
+ *                         (sampleWeight 1, no training) with that tree, so the result depends
+ *                         only on the request and the tree, not on the mix it is served in;
+ *                         while the scene has none: as HSR_GUIDE_TRAIN.
+ * Training statistics are summed in a fixed order (wavefront pipeline), so the same training
+ * request gives the same tree wherever it runs. */
+#define HSR_GUIDE_PER_REQUEST 0
+#define HSR_GUIDE_SCENE 1
+#define HSR_GUIDE_TRAIN 2
 typedef struct hsr_server hsr_server;
 
 typedef struct hsr_server_config {
@@ -363,12 +381,13 @@ typedef struct hsr_server_config {
     int32_t maxDepth;          /* largest knobs.maxDepth a request may use; 0 = 16 */
     int64_t hbmBudget;         /* device bytes for slots and scenes; 0 = 90% of the free memory */
     int64_t pathsPerBatch;     /* wavefront batch per slot; 0 = 2^22 */
-    int64_t frameReserveBytes; /* per slot frame (sum + compensation); 0 = 64 MiB */
+    int64_t frameReserveBytes; /* largest frame a request may have (sum + compensation bytes); 0 = no
+                                  limit beyond the HBM budget */
 } hsr_server_config;
 
 typedef struct hsr_request {
     int32_t sceneId;
-    int32_t reserved0;
+    int32_t guideMode;         /* HSR_GUIDE_PER_REQUEST (0), HSR_GUIDE_SCENE, HSR_GUIDE_TRAIN */
     int64_t paths;             /* the samples argument of PathTracer::run */
     hsr_scene scene;
     hsr_knobs knobs;
@@ -389,8 +408,10 @@ typedef struct hsr_server_stats {
     int64_t submitted, completed, failed, queued, running;
     double p50_ms, p99_ms, mean_ms, max_ms; /* latency (submit -> done) of completed requests */
     double frames_per_s;                    /* completed / (last completion - first submission) */
-    int64_t hbm_used, hbm_budget;
+    int64_t hbm_used, hbm_budget;           /* scenes + slots' buffers, and the budget */
     int32_t scenes, slots;
+    int64_t rejected;                       /* submits refused by admission (HSR_ERR_OOM) */
+    int64_t memory_waits;                   /* requests that waited at the head of the queue for memory */
 } hsr_server_stats;
 
 hsr_status hsr_server_create(const hsr_server_config *config, hsr_server **out);
@@ -406,6 +427,9 @@ hsr_status hsr_server_set_velocities(hsr_server *server, int32_t sceneId, const 
 /* Releases a scene once its queued and running requests are done; later submits fail. */
 hsr_status hsr_server_close_scene(hsr_server *server, int32_t sceneId);
 hsr_status hsr_server_submit(hsr_server *server, const hsr_request *request, int64_t *ticket);
+/* The device bytes a request needs on a slot that holds no buffers yet: its admission
+ * footprint (wavefront state of its batch, frame, tiles, event list, training rows). */
+hsr_status hsr_server_request_bytes(hsr_server *server, const hsr_request *request, int64_t *bytes);
 /* done = 1 once the request has finished (successfully or not). */
 hsr_status hsr_server_poll(hsr_server *server, int64_t ticket, int32_t *done);
 /* Blocks until the request is done, copies its frame (complex64, frame.sampleCount()
