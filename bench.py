@@ -292,22 +292,6 @@ def accuracy_check(backend, pt, mesh, scene, knobs, fc, n, threads):
                 "paths_with_other_nee": int((nee != rec32[1]).reshape(rec, -1).any(1).sum())}}
 
 
-def serving_trace(count, rate, seed=3):
-    """The serving mix: Poisson arrivals at `rate`/s; per request a scene (one of three), a
-    radar pose within 20 m of the origin looking down +z, bins U{128..512}, receivers
-    U{1..12} at λ/2, paths 2^U{18..24}; all draws from SplitMix64 streams of `seed`."""
-    u = scenes.uniforms(seed, 8 * count).reshape(count, 8)
-    t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
-    out = []
-    for i in range(count):
-        r = 20.0 * np.sqrt(u[i, 1])
-        phi = 2 * np.pi * u[i, 2]
-        out.append({"t": float(t[i]), "scene": int(u[i, 3] * 3), "pos": (r * np.cos(phi), 0.75, r * np.sin(phi)),
-                    "bins": 128 + int(u[i, 4] * 385), "channels": 1 + int(u[i, 5] * 12),
-                    "paths": 1 << (18 + int(u[i, 6] * 7))})
-    return out
-
-
 def serving_run(srv, ids, trace, rf, knobs, rate, guide_mode):
     """The trace replayed open-loop at `rate` requests/s (each request submitted at its arrival
     time); returns the server's statistics over it."""
@@ -340,10 +324,10 @@ def serving_bench(args):
        before the trace, by one request at the scene's nominal pose) and every request's whole
        budget as one final pass with it.
     Frames/s, p50/p99 latency, rejections and memory waits of each."""
-    from hussar_amd.serving import GUIDE_PER_REQUEST, GUIDE_SCENE, GUIDE_TRAIN, Server
+    from hussar_amd.serving import GUIDE_PER_REQUEST, GUIDE_SCENE, GUIDE_TRAIN, Server, request_mix
     rf = scenes.dihedral_rf()
     meshes = [scenes.vehicle_mesh(seed=10 + k, triangles=n)[0] for k, n in enumerate((10_000, 300_000, 2_000_000))]
-    trace = serving_trace(args.requests, 100.0)  # arrival times at 100/s, rescaled per rate
+    trace = request_mix(args.requests, 100.0)  # arrival times at 100/s, rescaled per rate
     knobs = H.Knobs(maxDepth=4, doGuiding=True)
     rates = [args.rate] if args.rate > 0 else [50.0, 100.0, 200.0, 500.0]
     forms = {"plain": (1, GUIDE_PER_REQUEST), "concurrent": (args.slots, GUIDE_PER_REQUEST),

@@ -25,6 +25,24 @@ from .types import (FrameConfig, Knobs, Scene, TriangleMesh, c_request, c_reques
 GUIDE_PER_REQUEST, GUIDE_SCENE, GUIDE_TRAIN = 0, 1, 2  # hsr_request.guideMode
 
 
+def request_mix(count: int, rate: float, seed: int = 3) -> list:
+    """The serving mix: Poisson arrivals at `rate`/s; per request one of three scenes, a radar
+    pose within 20 m of the origin looking down +z (height 0.75 m), bins U{128..512},
+    receivers U{1..12} at λ/2, paths 2^U{18..24}; every draw from SplitMix64 streams of
+    `seed`. Returns dicts with t (arrival [s]), scene, pos, bins, channels, paths."""
+    from .scenes import uniforms
+    u = uniforms(seed, 8 * count).reshape(count, 8)
+    t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
+    out = []
+    for i in range(count):
+        r = 20.0 * np.sqrt(u[i, 1])
+        phi = 2 * np.pi * u[i, 2]
+        out.append({"t": float(t[i]), "scene": int(u[i, 3] * 3), "pos": (r * np.cos(phi), 0.75, r * np.sin(phi)),
+                    "bins": 128 + int(u[i, 4] * 385), "channels": 1 + int(u[i, 5] * 12),
+                    "paths": 1 << (18 + int(u[i, 6] * 7))})
+    return out
+
+
 @dataclass
 class RequestInfo:
     status: int

@@ -1,4 +1,4 @@
-"""GPU parity on the stated scenes of the precision study and of the urban MIMO frame, through
+"""GPU parity on the stated scenes of the precision study, the urban MIMO frame and the serving mix, through
 the C ABI, against the oracle on the same Halton indices (bounded path samples; the frame
 itself is what the benchmark times at full path counts).
 
@@ -14,7 +14,9 @@ Urban MIMO frame: a 2M-triangle street canyon with four moving vehicle blocks, 3
 64 chirps x 256 samples x 12 channels, depth 5, guiding on (a tree trained on the device with
 four passes of 2^18 paths, then shared with the oracle), 2^14 final-pass paths: hits and shadow
 outcomes bit-exact against the mixed oracle, the cube within 1e-5 of it and within 1e-3 of the
-FP64 oracle, the peak cell identical."""
+FP64 oracle, the peak cell identical.
+Serving mix: the 10k / 300k / 2M-triangle scenes with resident guiding trees, eight requests of
+the mix served together, each equal to itself served alone and within 1e-5 of the oracle."""
 import os
 
 import numpy as np
@@ -135,3 +137,56 @@ def test_urban_mimo_guided_cube(oracle):
     assert rel_l2(z, zm) < 1e-5
     assert rel_l2(z, zf) < 1e-3
     assert int(np.abs(z).argmax()) == int(np.abs(zf).argmax())
+
+
+def test_serving_mix_stated_scenes(oracle):
+    """The serving mix on its stated scenes (vehicle clutter of 10k, 300k and 2M triangles,
+    seeds 10-12) with resident guiding trees: each scene's tree trained by one request (the same
+    tree a direct PathTracer::run of it trains), then eight requests of the mix (poses, bins and
+    receivers as drawn, 2^15 paths each) served together on four slots. Every served frame equals
+    the same request served alone and is within 1e-5 of the oracle's final pass with the
+    scene's tree."""
+    from hussar_amd.serving import GUIDE_SCENE, GUIDE_TRAIN, Server, request_mix
+    print("serving mix: building the 10k / 300k / 2M scenes", flush=True)
+    meshes = [scenes.vehicle_mesh(seed=10 + k, triangles=n)[0] for k, n in enumerate((10_000, 300_000, 2_000_000))]
+    rf = scenes.dihedral_rf()
+    kn = H.Knobs(maxDepth=4, doGuiding=True)
+    srv = Server(slots=4, pathsPerBatch=1 << 20)
+    ids = [srv.open_scene(m) for m in meshes]
+    nominal = scenes.radar_array(rf, channels=4, position=(0.0, 0.75, 0.0))
+    trees = []
+    for sid, m in zip(ids, meshes):
+        _, info = srv.wait(srv.submit(sid, nominal, kn, H.FrameConfig(1, 256, 4), 1 << 18, guide_mode=GUIDE_TRAIN))
+        assert info.status == 0
+        pt = H.PathTracer(**{k: getattr(kn, k) for k in kn.__dataclass_fields__})
+        pt.configureFrame(H.FrameConfig(1, 256, 4))
+        pt.run(H.HipBackend(m, pt), nominal, 1 << 18)
+        trees.append(pt.guiding.sampling())
+    mix = request_mix(8, 100.0)
+    assert {q["scene"] for q in mix} == {0, 1, 2}
+    n = 1 << 15
+    reqs = [(ids[q["scene"]], scenes.radar_array(rf, channels=q["channels"], position=q["pos"]),
+             H.FrameConfig(1, q["bins"], q["channels"])) for q in mix]
+    print("serving mix: eight requests together", flush=True)
+    tickets = [srv.submit(sid, sc, kn, fc, n, guide_mode=GUIDE_SCENE) for sid, sc, fc in reqs]
+    served = [srv.wait(t) for t in tickets]
+    oms = [oracle.OracleMesh.from_mesh(m) for m in meshes]
+    nonzero = 0
+    for (sid, sc, fc), (z, info), q in zip(reqs, served, mix):
+        assert info.status == 0 and info.totalWeight == n
+        za, _ = srv.wait(srv.submit(sid, sc, kn, fc, n, guide_mode=GUIDE_SCENE))
+        og = oracle.OracleGuide()
+        og.set_sampling(*trees[q["scene"]])
+        zo, two, _, _ = oracle.render(oms[q["scene"]], sc, kn, fc, prec=oracle.PREC_MIXED, count=n, is_final=True,
+                                      guide=og, threads=THREADS)
+        zo = (zo / two).reshape(z.shape)
+        print(f"serving mix: scene {q['scene']}, {q['bins']} bins x {q['channels']} channels, "
+              f"|frame| {np.abs(zo).max():.3g}", flush=True)
+        if np.abs(zo).max() > 0:
+            nonzero += 1
+            assert rel_l2(za, z) < 1e-6
+            assert rel_l2(z, zo) < 1e-5
+        else:
+            assert np.abs(z).max() == 0 and np.abs(za).max() == 0
+    assert nonzero >= 4
+    srv.close()
