@@ -477,10 +477,13 @@ static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
 }
 
 // One Backend::run; with `accumulate` the statistics add to those of the enclosing
-// schedule call instead of replacing them.
+// schedule call instead of replacing them. discardFrame: the caller clears the frame before
+// anything can read it (a training iteration of a guiding schedule that clears before each
+// iteration), so the wavefront pipeline splats nothing and reduces no frame.
 static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
                            int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
-                           const volatile uint8_t *interruptFlag, bool accumulate, bool queueStats = false) {
+                           const volatile uint8_t *interruptFlag, bool accumulate, bool queueStats = false,
+                           bool discardFrame = false) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured (hsr_hip_configure_frame)");
     if (budget < 0 || sampleIndexOffset < 0) return fail(HSR_ERR_INVALID, "negative budget or offset");
@@ -541,6 +544,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.C = C;
     a.sampleWeight = sampleWeight;
     a.isFinal = isFinalIteration;
+    const bool noSplat = discardFrame && b->optPipeline == 1;
+    a.noSplat = noSplat;
     a.nodes = mesh->d_nodes;
     a.nodesF = mesh->d_nodesF;
     a.tris = mesh->d_tris;
@@ -642,6 +647,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         w.cells = b->d_cells;
         w.base = done;
         w.n = (int32_t)n;
+        w.b0mod = (int32_t)((sampleIndexOffset + done) % hsr::kCells);
         a.offset = sampleIndexOffset;
         if (recPaths > 0) {
             a.recHits = b->d_recHits;
@@ -675,16 +681,13 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             if ((s = dalloc(&b->d_stage_comp, (size_t)S * MD)) != HSR_OK) return s;
             b->stageCap = (size_t)S * MD;
         }
-        if (!cube) HSR_HIP(hsr::launch_zero(b->d_partials, (int64_t)gridGather * MD, b->stream));
-        // a training batch records each path's statistic at its slot (cleared here), summed per
-        // leaf in a fixed order after the batch
+        // the batch's set-up launch clears the partial frames (unless nothing is splatted) and
+        // the per-slot statistics of a training batch, summed per leaf in a fixed order after it
+        const int64_t partialDoubles = cube || noSplat ? 0 : (int64_t)gridGather * MD;
         const bool training = guide && knobs->doGuiding && !isFinalIteration;
-        if (training) {
-            if (!b->d_statRows) {
-                hsr_status s = dalloc(&b->d_statRows, (size_t)hsr::kStatRows);
-                if (s != HSR_OK) return s;
-            }
-            HSR_HIP(hsr::launch_zero(w.statV, n, b->stream));
+        if (training && !b->d_statRows) {
+            hsr_status s = dalloc(&b->d_statRows, (size_t)hsr::kStatRows);
+            if (s != HSR_OK) return s;
         }
         HSR_HIP(hipEventRecord(b->ev[0], b->stream));
         if (first) HSR_HIP(hipEventRecord(b->ev[2], b->stream));
@@ -704,10 +707,10 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             b->evFork.push_back(f);
             b->evJoin.push_back(j);
         }
-        HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, b->optCounters,
-                                      b->stream, b->useStream2 ? b->stream2 : b->stream, b->evFork.data(),
-                                      b->evJoin.data(), evC, &nClosest, evS, &nShadow));
-        if (a.events)
+        HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, partialDoubles,
+                                      training, b->optCounters, b->stream, b->useStream2 ? b->stream2 : b->stream,
+                                      b->evFork.data(), b->evJoin.data(), evC, &nClosest, evS, &nShadow));
+        if (a.events && !noSplat)
             HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
                                            (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));
         if (training)
@@ -723,7 +726,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         }
         // LDS tiles: the workgroup tiles are reduced into the frame; cube: the gather kernel
         // has added into the frame directly
-        if (!cube)
+        if (!cube && !noSplat)
             HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, MD, b->d_stage_sum, b->d_stage_comp, S,
                                        b->d_frame_sum, b->d_frame_comp, b->stream));
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
@@ -1175,8 +1178,11 @@ hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr
     int64_t milestone = 16384, remaining = samples;
     while (true) {
         milestone = std::min(milestone, remaining);
-        s = run_impl(b, scene, knobs, offset, milestone, weight, isFinal, guide, interruptFlag, true,
-                     remaining > milestone);
+        // an iteration followed by clearFrame() leaves nothing readable in the frame unless an
+        // interrupt ends the run after it
+        const bool more = remaining > milestone;
+        s = run_impl(b, scene, knobs, offset, milestone, weight, isFinal, guide, interruptFlag, true, more,
+                     more && knobs->clearBeforeIteration && !interruptFlag);
         if (s != HSR_OK) return s;
         offset += milestone;
         remaining -= milestone;
@@ -1210,9 +1216,10 @@ hsr_status hsr_path_tracer_run_schedule(hsr_backend *b, const hsr_scene *scene, 
     if (guiding) {
         if ((s = hsr_guide_reset(guide)) != HSR_OK) return s;
         for (int p = 0; p < trainingPasses; ++p) {
-            if ((s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
-            if ((s = run_impl(b, scene, knobs, 0, trainingPaths, 1.0f, 0, guide, interruptFlag, true, true)) !=
-                HSR_OK)
+            // a training pass splats nothing unless an interrupt may end the schedule after it
+            if (interruptFlag && (s = hsr_hip_clear_frame(b)) != HSR_OK) return s;
+            if ((s = run_impl(b, scene, knobs, 0, trainingPaths, 1.0f, 0, guide, interruptFlag, true, true,
+                              !interruptFlag)) != HSR_OK)
                 return s;
             if ((s = hsr_guide_step(guide)) != HSR_OK) return s;
         }

@@ -56,6 +56,9 @@ struct TraceArgs {
     int64_t offset, count, chunk;
     float sampleWeight;
     int32_t isFinal;
+    // 1: nothing reads this run's frame before it is cleared (a training iteration of the guiding
+    // schedule): the wavefront gather keeps the guiding weights and records and splats nothing
+    int32_t noSplat;
     // mesh: four-wide BVH nodes, quantised (4 float4 each) and with float boxes (8 float4
     // each, same node indices), and triangles (3 float4 each)
     const float4 *nodes;
@@ -144,6 +147,7 @@ struct WfArgs {
     // primary sample square, cells in Morton order (coherence_cells)
     const int32_t *cells;
     int32_t *cellStart;  // [kCells + 1] first slot of each cell in this batch
+    int32_t b0mod;       // Halton index of the batch's first path (offset + base) mod kCells
     // guiding record of a training batch: each path's statistic and leaf at its slot
     double *statV;
     int32_t *statL;
@@ -162,8 +166,12 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth);
 // gather stages on stream, ordered by evFork/evJoin (maxDepth + 1 each); the batch ends
 // on stream. evClosest / evShadow: room for 2 * (maxDepth + 1) events each; the launcher
 // records a pair around every closest-hit / shadow traversal launch and returns the counts.
+// The batch opens with one set-up launch: the coherence order's slot ranges, the queue
+// counters, and the first partialDoubles doubles of partials and (zeroStats) the training
+// records cleared.
 hipError_t launch_wavefront(const TraceArgs &args, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
-                            double2 *partials, bool count, hipStream_t stream, hipStream_t stream2,
+                            double2 *partials, int64_t partialDoubles, bool zeroStats, bool count,
+                            hipStream_t stream, hipStream_t stream2,
                             hipEvent_t *evFork, hipEvent_t *evJoin, hipEvent_t *evClosest, int *nClosest,
                             hipEvent_t *evShadow, int *nShadow);
 int wavefront_trav_blocks_per_cu(bool count);

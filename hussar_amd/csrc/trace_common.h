@@ -974,8 +974,13 @@ constexpr int kGuard = 16;
 #endif
 __device__ inline int guard_row(int N) { return N + 2 * kGuard; }
 
-// flush_events into the guard-band tile: the same taps, weights and lane rotation, with the
-// wrap of each tap replaced by the guard bins.
+// flush_events into the guard-band tile: the same taps and weights, with the wrap of each tap
+// replaced by the guard bins, in a bank-aligned order. The 32 taps k in [-16, 15] of an event
+// cover 32 consecutive bins, one of each residue mod 32; lane L takes them in the order whose
+// step `it` lands on the bin of residue L + it (mod 32). Every LDS atomic of the wave then puts
+// the same number of lanes on each bank (two per 4-byte bank word for a wave of 64 doubles,
+// the least it can), wherever the lanes' events are centred; a rotation by lane alone aligns
+// only lanes whose events share a centre bin. Tap 16 follows on its own.
 __device__ inline void flush_events_g(bool has, const Event &e, int ch, double *tile, int N, int C, int lane) {
     if (!has) return;
     const int R = guard_row(N);
@@ -987,16 +992,19 @@ __device__ inline void flush_events_g(bool has, const Event &e, int ch, double *
         atomicAdd(im, (double)e.A.im);
         return;
     }
-    int k = lane % 33 - 16;
+    int k = ((lane - c0 + 16) & 31) - 16;  // c0 + k = lane (mod 32)
 #if HSR_TAP_UNROLL > 0
 #pragma unroll HSR_TAP_UNROLL
 #endif
-    for (int it = 0; it < 33; ++it) {
+    for (int it = 0; it < 32; ++it) {
         const float coef = __builtin_amdgcn_rcpf(e.s - (float)k);
         atomicAdd(&re[k], (double)(e.A.re * coef));
         atomicAdd(&im[k], (double)(e.A.im * coef));
-        k = k == 16 ? -16 : k + 1;
+        k = k == 15 ? -16 : k + 1;
     }
+    const float coef = __builtin_amdgcn_rcpf(e.s - 16.0f);
+    atomicAdd(&re[16], (double)(e.A.re * coef));
+    atomicAdd(&im[16], (double)(e.A.im * coef));
 }
 
 // Adds the guard-band tile into the workgroup's partial frame [N][C] (complex, frame order):

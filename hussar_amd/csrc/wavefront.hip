@@ -296,23 +296,44 @@ __device__ inline void write_extend(const WfArgs &w, bool alive, bool ends, int6
 // of its indices (kCells residues, made once per backend by coherence_cells); cell c holds
 // the indices h = B0 + t_c + j * kCells of the batch [B0, B0 + n), t_c = (r_c - B0) mod
 // kCells, and its slots follow those of the cells before it. The order is deterministic.
-__device__ inline int64_t cell_first(int32_t r, int64_t B0) {
-    int64_t t = ((int64_t)r - B0) % kCells;
+// With n = q kCells + rem, cell c holds q + (t_c < rem) paths.
+__device__ inline int32_t cell_first(int32_t r, int32_t b0mod) {
+    const int32_t t = r - b0mod;
     return t < 0 ? t + kCells : t;
 }
-__device__ inline int64_t cell_count(int64_t t, int64_t n) { return t < n ? (n - 1 - t) / kCells + 1 : 0; }
 
-// cellStart[c] = the first slot of cell c (exclusive prefix of the cells' path counts), and
-// cellStart[kCells] = n: one workgroup of 1024; the counts first go to LDS with coalesced
-// loads of the residues, then each thread sums and writes a contiguous run of cells.
+// The batch's set-up in one launch: workgroup 0 writes cellStart[c] = the first slot of cell c
+// (exclusive prefix of the cells' path counts) and cellStart[kCells] = n, the counts first to
+// LDS with coalesced loads of the residues, then each thread summing and writing a contiguous
+// run of cells; the other workgroups clear the batch's queue counters (nCounts depths), the
+// first nPartials doubles of the partial frames and the first nStat training records.
 constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_cell_scan(TraceArgs a, WfArgs w) {
-    __shared__ uint16_t cnt[kCells];  // n / kCells + 1 paths per cell at most: 2210 for a batch of 2^26
+__global__ __launch_bounds__(kScanThreads) void k_wf_setup(WfArgs w, int nCounts, double *partials,
+                                                           int64_t nPartials, int64_t nStat) {
+    __shared__ uint16_t cnt[kCells];  // n / kCells + 1 paths per cell at most: 553 for a batch of 2^24
     __shared__ int32_t part[kScanThreads];
+    if (blockIdx.x > 0) {
+        const int64_t t = (int64_t)(blockIdx.x - 1) * kScanThreads + threadIdx.x;
+        const int64_t T = (int64_t)(gridDim.x - 1) * kScanThreads;
+        if (t < nCounts) {
+            WfCounts &c = w.cnt[t];
+            c.n_ext.v = 0;
+            c.n_shd.v = 0;
+            c.n_gat.v = 0;
+            for (int k = 0; k < kQueueSegments; ++k) {
+                c.head_ext[k].v = 0;
+                c.head_shd[k].v = 0;
+            }
+        }
+        double2 *p2 = (double2 *)partials;  // nPartials is even (whole complex cells)
+        for (int64_t i = t; i < nPartials / 2; i += T) p2[i] = make_double2(0.0, 0.0);
+        for (int64_t i = t; i < nStat; i += T) w.statV[i] = 0.0;
+        return;
+    }
     constexpr int per = (kCells + kScanThreads - 1) / kScanThreads;
-    const int64_t B0 = a.offset + w.base, n = w.n;
+    const int32_t q = w.n / kCells, rem = w.n % kCells;
     for (int c = threadIdx.x; c < kCells; c += kScanThreads)
-        cnt[c] = (uint16_t)cell_count(cell_first(w.cells[c], B0), n);
+        cnt[c] = (uint16_t)(q + (cell_first(w.cells[c], w.b0mod) < rem ? 1 : 0));
     __syncthreads();
     const int c0 = threadIdx.x * per, c1 = min(kCells, c0 + per);
     int32_t sum = 0;
@@ -379,7 +400,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int 
                     if (cstart[mid] <= item) lo = mid;
                     else hi = mid - 1;
                 }
-                const int64_t m = cell_first(w.cells[c0 + lo], B0) + (int64_t)(item - cstart[lo]) * kCells;
+                const int64_t m = cell_first(w.cells[c0 + lo], w.b0mod) + (int64_t)(item - cstart[lo]) * kCells;
                 p.local = w.base + m;
                 generate<kRef>(a, (uint64_t)(B0 + m), p, gsChild, gsP0);
             }
@@ -814,7 +835,8 @@ __global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceA
     const int lane = tid & 63;
     const bool cube = a.cube != 0;  // uniform: splat into the frame cube in device memory
     const bool guard = a.N >= 2 * kGuard;
-    if (!cube) {
+    const bool splat = !a.noSplat;  // uniform
+    if (!cube && splat) {
         const int tileDoubles = 2 * (guard ? guard_row(a.N) : a.N) * a.C;
         for (int j = tid; j < tileDoubles; j += kWfBlock) tile[j] = 0.0;
         __syncthreads();
@@ -860,14 +882,18 @@ __global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceA
                     }
                     if (stt & S_GW) p.gw = p.gw + v;
                     if (stt & S_SPLAT) {
-                        ev = make_event(a, w.s_tn[j], v, a.sampleWeight, a.motion ? w.s_dv[j] : 0.0);
-                        has = true;
                         flag = 3;
-                        if (count) ++cev;
+                        if (splat) {
+                            ev = make_event(a, w.s_tn[j], v, a.sampleWeight, a.motion ? w.s_dv[j] : 0.0);
+                            has = true;
+                            if (count) ++cev;
+                        }
                     }
                 }
                 if (rec) a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + a.chBase + ch] = flag;
             }
+            if (!splat)
+                continue;
             if (cube)
                 cube_event(has, ev, a.chBase + ch, a, lane);
             else if (guard)
@@ -883,7 +909,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceA
         finish_record(a, w, finishing, i, p);
     }
     if (count) wave_count(&a.counters[4], cev, lane);
-    if (cube) return;
+    if (cube || !splat) return;
     __syncthreads();
     if (guard)
         tile_flush_g(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
@@ -1051,25 +1077,24 @@ __global__ __launch_bounds__(64) void k_stat_reduce_mem(const double *__restrict
     }
 }
 
-// stat[j] += rows[0][j] + rows[1][j] + ... (in row order)
-__global__ void k_stat_rows_fold(const double *__restrict__ rows, int G, int nodes, double *__restrict__ stat) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nodes) return;
+// stat[j] += the sum of rows[g][j] over g < G: workgroups of 32 columns, each column's rows in
+// eight ranges summed in row order by eight threads, then the eight sums in range order (a
+// fixed order; the loads of 32 columns are coalesced).
+__global__ __launch_bounds__(256) void k_stat_rows_fold(const double *__restrict__ rows, int G, int nodes,
+                                                        double *__restrict__ stat) {
+    __shared__ double part[8][32];
+    const int col = threadIdx.x & 31, seg = threadIdx.x >> 5;
+    const int j = blockIdx.x * 32 + col;
+    const int g0 = G * seg / 8, g1 = G * (seg + 1) / 8;
     double sum = 0.0;
-    for (int g = 0; g < G; ++g) sum += rows[(int64_t)g * nodes + j];
-    stat[j] += sum;
-}
-
-__global__ void k_wf_zero_counts(WfCounts *c, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        c[i].n_ext.v = 0;
-        c[i].n_shd.v = 0;
-        c[i].n_gat.v = 0;
-        for (int k = 0; k < kQueueSegments; ++k) {
-            c[i].head_ext[k].v = 0;
-            c[i].head_shd[k].v = 0;
-        }
+    if (j < nodes)
+        for (int g = g0; g < g1; ++g) sum += rows[(int64_t)g * nodes + j];
+    part[seg][col] = sum;
+    __syncthreads();
+    if (seg == 0 && j < nodes) {
+        double t = part[0][col];
+        for (int k = 1; k < 8; ++k) t += part[k][col];
+        stat[j] += t;
     }
 }
 
@@ -1206,7 +1231,7 @@ hipError_t launch_stat_reduce(const double *statV, const int32_t *statL, int64_t
     } else {
         hipLaunchKernelGGL(k_stat_reduce_mem, dim3(G), dim3(64), 0, s, statV, statL, n, nodes, rows);
     }
-    hipLaunchKernelGGL(k_stat_rows_fold, dim3((nodes + 255) / 256), dim3(256), 0, s, rows, G, nodes, stat);
+    hipLaunchKernelGGL(k_stat_rows_fold, dim3((nodes + 31) / 32), dim3(256), 0, s, rows, G, nodes, stat);
     return hipGetLastError();
 }
 
@@ -1218,9 +1243,9 @@ int wavefront_trav_blocks_per_cu(bool count) {
 }
 
 hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
-                            double2 *partials, bool count, hipStream_t s, hipStream_t s2, hipEvent_t *evFork,
-                            hipEvent_t *evJoin, hipEvent_t *evClosest, int *nClosest, hipEvent_t *evShadow,
-                            int *nShadow) {
+                            double2 *partials, int64_t partialDoubles, bool zeroStats, bool count, hipStream_t s,
+                            hipStream_t s2, hipEvent_t *evFork, hipEvent_t *evJoin, hipEvent_t *evClosest,
+                            int *nClosest, hipEvent_t *evShadow, int *nShadow) {
     *nClosest = 0;
     *nShadow = 0;
     (void)hipGetLastError();
@@ -1236,11 +1261,15 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket), kWfBlock);
     const int gShade = fit(gridWork, n, kRange);
     const int gNee = fit(gridWork, n * nrx, kRange);
-    hipLaunchKernelGGL(k_wf_zero_counts, dim3((maxDepth + 2 + 255) / 256), dim3(256), 0, s, w.cnt, maxDepth + 2);
+    // set-up: one workgroup for the scan, the clears spread over up to 256 more
+    const int64_t nStat = zeroStats ? n : 0;
+    const int64_t clears = std::max<int64_t>(maxDepth + 2, partialDoubles / 2 + nStat);
+    const int gSetup = 1 + (int)std::min<int64_t>(256, (clears + 4 * kScanThreads - 1) / (4 * kScanThreads));
+    hipLaunchKernelGGL(k_wf_setup, dim3(gSetup), dim3(kScanThreads), 0, s, w, maxDepth + 2, (double *)partials,
+                       partialDoubles, nStat);
     // generation: K cells per workgroup, about kRange paths each
     const int64_t q = n / kCells;
     const int K = (int)std::max<int64_t>(1, std::min<int64_t>(kGenCellsMax, kRange / (q + 1)));
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(kScanThreads), 0, s, a, w);
     if (a.refChain)
         hipLaunchKernelGGL((k_wf_gen<true>), dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
     else
