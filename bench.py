@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 import hussar_amd as H  # noqa: E402
 from hussar_amd import scenes  # noqa: E402
-from hussar_amd._lib import OPT_COUNTERS, OPT_PIPELINE, OPT_RECORD_HITS  # noqa: E402
+from hussar_amd._lib import OPT_COUNTERS, OPT_GRAPH, OPT_PIPELINE, OPT_RECORD_HITS  # noqa: E402
 
 METRIC = "paths/s & frames/s at ≤1e-3 IF rel-L2 vs FP64; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -322,7 +322,9 @@ def serving_bench(args):
      * concurrent: `--slots` slots, still PathTracer::run per request;
      * served: `--slots` slots, each scene's resident guiding tree (trained once per scene,
        before the trace, by one request at the scene's nominal pose) and every request's whole
-       budget as one final pass with it.
+       budget as one final pass with it;
+     * served_graph: the same with every batch replayed from its launch shape's graph
+       (HSR_OPT_GRAPH) instead of launched stage by stage as in the other forms.
     Frames/s, p50/p99 latency, rejections and memory waits of each."""
     from hussar_amd.serving import GUIDE_PER_REQUEST, GUIDE_SCENE, GUIDE_TRAIN, Server, request_mix
     rf = scenes.dihedral_rf()
@@ -330,11 +332,11 @@ def serving_bench(args):
     trace = request_mix(args.requests, 100.0)  # arrival times at 100/s, rescaled per rate
     knobs = H.Knobs(maxDepth=4, doGuiding=True)
     rates = [args.rate] if args.rate > 0 else [50.0, 100.0, 200.0, 500.0]
-    forms = {"plain": (1, GUIDE_PER_REQUEST), "concurrent": (args.slots, GUIDE_PER_REQUEST),
-             "served": (args.slots, GUIDE_SCENE)}
+    forms = {"plain": (1, GUIDE_PER_REQUEST, False), "concurrent": (args.slots, GUIDE_PER_REQUEST, False),
+             "served": (args.slots, GUIDE_SCENE, False), "served_graph": (args.slots, GUIDE_SCENE, True)}
     results = {name: {} for name in forms}
-    for name, (slots, mode) in forms.items():
-        srv = Server(slots=slots)
+    for name, (slots, mode, graphs) in forms.items():
+        srv = Server(slots=slots, graphs=graphs)
         ids = [srv.open_scene(m) for m in meshes]
         nominal = scenes.radar_array(rf, channels=4, position=(0.0, 0.75, 0.0))
         for sid in ids:  # warm-up, and the scenes' resident trees
@@ -508,6 +510,8 @@ def main():
     ap.add_argument("--requests", type=int, default=96, help="serving: requests in the trace")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent slots")
     ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
+                    help="1 replay each batch shape's graph (HSR_OPT_GRAPH), 0 stage-by-stage launches")
     ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.counter_child:
@@ -538,6 +542,7 @@ def main():
     pt.configureFrame(fc)
     backend = H.HipBackend(mesh, pt)
     backend.set_option(OPT_PIPELINE, args.pipeline)
+    backend.set_option(OPT_GRAPH, args.graph)
     bvh_ms = backend.stats()["bvh_build_ms"]
     setup_s = time.perf_counter() - t0
 
@@ -545,7 +550,7 @@ def main():
         run_step(backend, pt, scene, sched)
     lat = []
     acc = {"trace_ms": 0.0, "launches": 0, "closest_ms": 0.0, "closest_launches": 0, "shadow_ms": 0.0,
-           "shadow_launches": 0}
+           "shadow_launches": 0, "graph_launches": 0}
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t = time.perf_counter()
@@ -580,6 +585,8 @@ def main():
         "frames_per_s": args.steps / elapsed,
         "frame_latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "pipeline": ["fused path kernel", "wavefront"][args.pipeline],
+        "batch_launch": f"graph replay ({acc['graph_launches']} of {acc['launches']} timed batches)" if args.graph
+        else "stage by stage",
         "kernel_paths_per_s": paths_per_step * args.steps / (acc["trace_ms"] * 1e-3),
         "host": host,
         "roofline": roof,

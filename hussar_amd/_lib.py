@@ -22,6 +22,7 @@ OPT_TRAV_WAVES = 6
 OPT_BATCH_PATHS = 7
 OPT_CUBE_EVENTS = 8
 OPT_CHAIN = 9
+OPT_GRAPH = 10
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER
@@ -79,7 +80,7 @@ class HsrError(RuntimeError):
         self.status = status
 
 
-ABI_VERSION = 3  # HSR_ABI_VERSION of include/hussar_hip.h
+ABI_VERSION = 4  # HSR_ABI_VERSION of include/hussar_hip.h
 
 
 def lib():

@@ -168,6 +168,8 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_post2);
     if (b->d_postScratch) (void)hipFree(b->d_postScratch);
     if (b->d_wf) (void)hipFree(b->d_wf);
+    for (auto &g : b->graphs) hsr::destroy_graph(g.g);
+    b->graphs.clear();
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : b->evTrav) (void)hipEventDestroy(e);
@@ -403,6 +405,10 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
                 return fail(HSR_ERR_UNSUPPORTED, "phase mode must be 0 (FP64), 1 (FP32) or 2 (double-float)");
             b->optPhaseMode = (int32_t)value;
             return HSR_OK;
+        case HSR_OPT_GRAPH:
+            if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "graph must be 0 (stage launches) or 1 (replay)");
+            b->optGraph = value != 0;
+            return HSR_OK;
         case HSR_OPT_CHAIN:
             if (value < 0 || value > 1)
                 return fail(HSR_ERR_UNSUPPORTED, "chain must be 0 (FP64) or 1 (the reference's FP32 chain)");
@@ -456,6 +462,48 @@ static hipError_t event_wait(hipEvent_t e) {
     while ((r = hipEventQuery(e)) == hipErrorNotReady) {
     }
     return r;
+}
+
+// The graph entry of a batch's launch shape `key`: a cached one, or, when the shape was seen
+// before among the last kSeen shapes, one built now from `plan` (with the batch's arguments h
+// as the nodes' first parameters). A shape met for the first time returns null (its batch
+// launches stage by stage), so shapes that never recur cost no graph. Returns null too when
+// building the graph fails; graphs are then off for this backend.
+extern "C++" template <class Plan>
+static hsr_backend::GraphEntry *batch_graph(hsr_backend *b, const std::vector<int64_t> &key, Plan &&plan,
+                                            const hsr::BatchArgs &h) {
+    for (auto &g : b->graphs)
+        if (g.key == key) {
+            g.used = ++b->graphClock;
+            return &g;
+        }
+    bool seen = false;
+    for (const auto &k : b->graphSeen) seen = seen || k == key;
+    if (!seen) {
+        if ((int)b->graphSeen.size() >= hsr_backend::kSeen) b->graphSeen.erase(b->graphSeen.begin());
+        b->graphSeen.push_back(key);
+        return nullptr;
+    }
+    hsr_backend::GraphEntry e;
+    e.key = key;
+    e.plan = plan();
+    if (hsr::graph_plan(&e.g, e.plan, h) != hipSuccess) {
+        (void)hipGetLastError();
+        b->optGraph = false;
+        return nullptr;
+    }
+    if ((int)b->graphs.size() >= hsr_backend::kGraphs) {
+        auto lru = std::min_element(b->graphs.begin(), b->graphs.end(),
+                                    [](const hsr_backend::GraphEntry &x, const hsr_backend::GraphEntry &y) {
+                                        return x.used < y.used;
+                                    });
+        hsr::destroy_graph(lru->g);
+        b->graphs.erase(lru);
+    }
+    e.used = ++b->graphClock;
+    b->graphs.push_back(std::move(e));
+    ++b->stats.graph_captures;
+    return &b->graphs.back();
 }
 
 static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
@@ -707,9 +755,36 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             b->evFork.push_back(f);
             b->evJoin.push_back(j);
         }
-        HSR_HIP(hsr::launch_wavefront(a, w, gridTrav, gridWork, gridGather, (double2 *)b->d_partials, partialDoubles,
-                                      training, b->optCounters, b->stream, b->useStream2 ? b->stream2 : b->stream,
-                                      b->evFork.data(), b->evJoin.data(), evC, &nClosest, evS, &nShadow));
+        // the batch's launch sequence: replayed from the graph of its launch shape with the
+        // batch's arguments, or stage by stage
+        HSR_HIP(hsr::prepare_wavefront(N, C, cube));
+        hsr::BatchArgs host;
+        host.a = a;
+        host.w = w;
+        host.partials = (double2 *)b->d_partials;
+        auto plan = [&] {
+            return hsr::plan_wavefront(a, w, gridTrav, gridWork, gridGather, partialDoubles, training,
+                                       b->optCounters, evC, evS);
+        };
+        hsr_backend::GraphEntry *graph = nullptr;
+        if (b->optGraph) {
+            const std::vector<int64_t> key = {n, gridTrav, gridWork, gridGather, partialDoubles, training,
+                                              b->optCounters, a.maxDepth, a.onlyIndirect, a.poDiffraction,
+                                              a.refChain, a.cube, a.rxCount, a.C, a.N, (int64_t)(intptr_t)evC};
+            graph = batch_graph(b, key, plan, host);
+        }
+        if (graph) {
+            HSR_HIP(hsr::replay_plan(graph->g, graph->plan, host, b->stream));
+            nClosest = graph->plan.nClosest;
+            nShadow = graph->plan.nShadow;
+            ++b->stats.graph_launches;
+        } else {
+            const hsr::BatchPlan p = plan();
+            HSR_HIP(hsr::launch_plan(p, host, b->stream, b->useStream2 ? b->stream2 : b->stream, b->evFork.data(),
+                                     b->evJoin.data()));
+            nClosest = p.nClosest;
+            nShadow = p.nShadow;
+        }
         if (a.events && !noSplat)
             HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
                                            (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));

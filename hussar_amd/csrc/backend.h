@@ -134,6 +134,24 @@ struct hsr_backend {
     // off for the slots of a server with more than two: concurrent slots already share the
     // device, and a second stream per slot would make more streams than hardware queues
     bool useStream2 = true;
+    // HSR_OPT_GRAPH: a batch's launch sequence (set-up to the last gather) built once per
+    // launch shape into a graph and replayed, from the second batch of that shape among the
+    // last kSeen shapes; at most kGraphs graphs are kept (least recently used out). A graph
+    // that fails to build turns the option off for this backend. Off by default: replays
+    // measured no faster than stage launches (their node updates and graph launches cost the
+    // host what the launches they replace did).
+    bool optGraph = false;
+    static constexpr int kSeen = 32;
+    std::vector<std::vector<int64_t>> graphSeen;
+    struct GraphEntry {
+        std::vector<int64_t> key;
+        hsr::BatchPlan plan;
+        hsr::BatchGraph g;
+        uint64_t used = 0;
+    };
+    static constexpr int kGraphs = 8;
+    std::vector<GraphEntry> graphs;
+    uint64_t graphClock = 0;
     hsr_run_stats stats{};
 };
 

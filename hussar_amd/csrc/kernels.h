@@ -160,20 +160,65 @@ std::vector<int32_t> coherence_cells();
 size_t wavefront_bytes(int64_t P, int C, int maxDepth);
 // Carves the buffers out of one allocation.
 void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth);
-// One batch: generation, then per depth shadow / gather / extend / shade, with all queue
-// sizes on the device (no host synchronisation). partials: [gatherGrid][N*C], accumulated.
-// The closest-hit traversal of each depth runs on stream2 beside the depth's shadow and
-// gather stages on stream, ordered by evFork/evJoin (maxDepth + 1 each); the batch ends
-// on stream. evClosest / evShadow: room for 2 * (maxDepth + 1) events each; the launcher
-// records a pair around every closest-hit / shadow traversal launch and returns the counts.
-// The batch opens with one set-up launch: the coherence order's slot ranges, the queue
-// counters, and the first partialDoubles doubles of partials and (zeroStats) the training
-// records cleared.
-hipError_t launch_wavefront(const TraceArgs &args, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
-                            double2 *partials, int64_t partialDoubles, bool zeroStats, bool count,
-                            hipStream_t stream, hipStream_t stream2,
-                            hipEvent_t *evFork, hipEvent_t *evJoin, hipEvent_t *evClosest, int *nClosest,
-                            hipEvent_t *evShadow, int *nShadow);
+// One wavefront batch's arguments: the stages receive TraceArgs and WfArgs by value (their
+// pointers then stay in the global address space), the gather and set-up stages the partials.
+struct BatchArgs {
+    TraceArgs a;
+    WfArgs w;
+    double2 *partials;  // [gatherGrid][N*C] partial frames, accumulated by the gather stages
+};
+// Allows the gather stage the LDS of an N x C tile (beyond 64 KiB); before the first batch of a
+// frame shape.
+hipError_t prepare_wavefront(int N, int C, bool cube);
+// One step of a batch's launch sequence: a kernel on stream 0 or 1 (its by-value parameters
+// built from the batch's arguments and the step's own scalars x, after the layout sig), a
+// fork (stream 1 waits for stream 0's work so far), a join (the reverse) or a timing event.
+struct BatchStep {
+    enum : int8_t { kKernel, kFork, kJoin, kRecord };
+    enum : int8_t { kSetup, kAW, kAWi, kGather };
+    int8_t kind = kKernel, stream = 0, sig = kAW;
+    hipEvent_t event = nullptr;
+    const void *func = nullptr;
+    dim3 grid, block;
+    unsigned lds = 0;
+    int64_t x[3] = {};
+};
+// A batch's launch sequence: set-up, generation, then per depth shadow / gather / extend /
+// shade, with all queue sizes on the device (no host synchronisation). It depends on the
+// launch shape only (paths, grids, depth, receivers, frame tile, options, timing events), not
+// on the argument values. The closest-hit traversal of each depth runs on stream 1 beside the
+// depth's shadow and gather stages on stream 0; the batch ends on stream 0. evClosest /
+// evShadow (optional): room for 2 * (maxDepth + 1) events each, a pair recorded around every
+// closest-hit / shadow traversal launch. The set-up computes the coherence order's slot
+// ranges and clears the queue counters, the first partialDoubles doubles of the partials and
+// (zeroStats) the training records.
+struct BatchPlan {
+    std::vector<BatchStep> steps;
+    int nClosest = 0, nShadow = 0;  // traversal launches (timed pairs)
+};
+BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
+                         int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
+                         hipEvent_t *evShadow);
+// The plan launched with the arguments h on stream (0) and stream2 (1), ordered by the events
+// evFork / evJoin (maxDepth + 1 each).
+hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t stream, hipStream_t stream2,
+                       hipEvent_t *evFork, hipEvent_t *evJoin);
+// The plan as a graph: its kernels as kernel nodes, its timing events as event-record nodes,
+// each depending on the last node of its stream (built node by node: no stream is captured, so
+// other threads' HIP calls are unaffected), instantiated twice. A replay sets the kernel
+// nodes' parameters of one instance to the batch's arguments and launches it; the two
+// instances alternate, each updated only once its previous run has ended (done[i]).
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    int next = 0;
+    std::vector<hipGraphNode_t> nodes;  // the kernel nodes, in plan order
+};
+hipError_t graph_plan(BatchGraph *out, const BatchPlan &plan, const BatchArgs &h);
+hipError_t replay_plan(BatchGraph &g, const BatchPlan &plan, const BatchArgs &h, hipStream_t stream);
+void destroy_graph(BatchGraph &g);
 int wavefront_trav_blocks_per_cu(bool count);
 // Training statistics of a batch: stat[leaf] += the records (statV, statL) of slots [0, n) of
 // each leaf, summed in a fixed order through up to rowsCap / nodes partial rows (rows: rowsCap

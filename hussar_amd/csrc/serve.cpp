@@ -358,7 +358,7 @@ hsr_status hsr_server_create(const hsr_server_config *config, hsr_server **out) 
     if (c.pathsPerBatch == 0) c.pathsPerBatch = int64_t(1) << 22;
     if (c.slots < 1 || c.slots > 64 || c.queueCapacity < 1 || c.maxDepth < 1 || c.maxDepth > 64 ||
         c.pathsPerBatch < 1024 || c.pathsPerBatch > (int64_t(1) << 26) || c.frameReserveBytes < 0 ||
-        c.hbmBudget < 0)
+        c.hbmBudget < 0 || c.graphs < 0 || c.graphs > 1)
         return hsr::fail(HSR_ERR_INVALID, "server configuration out of range");
     int ndev = 0;
     HSR_HIP(hipGetDeviceCount(&ndev));
@@ -383,6 +383,7 @@ hsr_status hsr_server_create(const hsr_server_config *config, hsr_server **out) 
         // a slot's second stream only while every stream can have a hardware queue of its own
         // (HIP gives a process 4 by default); beyond that the slots' streams would share queues
         sl.b->useStream2 = 2 * c.slots <= 4;
+        sl.b->optGraph = c.graphs != 0;
         // a slot's work buffers are reserved by the requests it runs (admission by footprint)
         if ((st = hsr_guide_create(sl.b, 0.1f, 0.005f, 1, &sl.g)) != HSR_OK) {
             const std::string e = hsr_last_error();

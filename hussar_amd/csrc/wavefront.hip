@@ -825,8 +825,8 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a
 }
 
 // ------------------------------------------------------------------------ gather + splat
-__global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceArgs a, WfArgs w, int depth, double2 *partials,
-                                                        bool count) {
+__global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceArgs a, WfArgs w, int depth,
+                                                                          double2 *partials, bool count) {
     // one chirp's frame tile in LDS: the guard-band layout (flush_events_g) for N >= 16, else
     // N*C complex doubles (tile_re); unused with M > 1
     extern __shared__ double tile[];
@@ -1242,13 +1242,64 @@ int wavefront_trav_blocks_per_cu(bool count) {
     return e == hipSuccess && n > 0 ? n : 1;
 }
 
-hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
-                            double2 *partials, int64_t partialDoubles, bool zeroStats, bool count, hipStream_t s,
-                            hipStream_t s2, hipEvent_t *evFork, hipEvent_t *evJoin, hipEvent_t *evClosest,
-                            int *nClosest, hipEvent_t *evShadow, int *nShadow) {
-    *nClosest = 0;
-    *nShadow = 0;
-    (void)hipGetLastError();
+hipError_t prepare_wavefront(int N, int C, bool cube) {
+    const size_t glds = cube ? 0 : gather_lds_bytes(N, C);
+    if (glds <= 65536) return hipSuccess;
+    return hipFuncSetAttribute((const void *)k_wf_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
+}
+
+// ------------------------------------------------------------------------ batch plans
+// The by-value parameters of one kernel step, for the batch arguments h.
+struct StepParams {
+    int i0 = 0;
+    int64_t l1 = 0, l2 = 0;
+    bool b = false;
+    double *dp = nullptr;
+    double2 *d2p = nullptr;
+    void *p[5] = {};
+};
+static void step_params(const BatchStep &st, const BatchArgs &h, StepParams &sp) {
+    void *a = (void *)&h.a, *w = (void *)&h.w;
+    switch (st.sig) {
+        case BatchStep::kSetup:  // (WfArgs, int nCounts, double *partials, int64 nPartials, int64 nStat)
+            sp.i0 = (int)st.x[0];
+            sp.dp = (double *)h.partials;
+            sp.l1 = st.x[1];
+            sp.l2 = st.x[2];
+            sp.p[0] = w;
+            sp.p[1] = &sp.i0;
+            sp.p[2] = &sp.dp;
+            sp.p[3] = &sp.l1;
+            sp.p[4] = &sp.l2;
+            break;
+        case BatchStep::kAW:  // (TraceArgs, WfArgs)
+            sp.p[0] = a;
+            sp.p[1] = w;
+            break;
+        case BatchStep::kAWi:  // (TraceArgs, WfArgs, int)
+            sp.i0 = (int)st.x[0];
+            sp.p[0] = a;
+            sp.p[1] = w;
+            sp.p[2] = &sp.i0;
+            break;
+        default:  // kGather: (TraceArgs, WfArgs, int depth, double2 *partials, bool count)
+            sp.i0 = (int)st.x[0];
+            sp.d2p = h.partials;
+            sp.b = st.x[1] != 0;
+            sp.p[0] = a;
+            sp.p[1] = w;
+            sp.p[2] = &sp.i0;
+            sp.p[3] = &sp.d2p;
+            sp.p[4] = &sp.b;
+            break;
+    }
+}
+
+BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
+                         int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
+                         hipEvent_t *evShadow) {
+    BatchPlan plan;
+    auto &S = plan.steps;
     const int maxDepth = a.maxDepth;
     // grids no larger than the batch's items can use (queue sizes are only known on the
     // device; the batch's path count bounds them), so that small batches launch small grids
@@ -1261,69 +1312,185 @@ hipError_t launch_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, i
     const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket), kWfBlock);
     const int gShade = fit(gridWork, n, kRange);
     const int gNee = fit(gridWork, n * nrx, kRange);
+    auto kernel = [&](int stream, const void *f, int grid, unsigned block, size_t lds, int8_t sig, int64_t x0 = 0,
+                      int64_t x1 = 0, int64_t x2 = 0) {
+        BatchStep st;
+        st.kind = BatchStep::kKernel;
+        st.stream = (int8_t)stream;
+        st.func = f;
+        st.grid = dim3(grid);
+        st.block = dim3(block);
+        st.lds = (unsigned)lds;
+        st.sig = sig;
+        st.x[0] = x0;
+        st.x[1] = x1;
+        st.x[2] = x2;
+        S.push_back(st);
+    };
+    auto marker = [&](int8_t kind, int stream, hipEvent_t e) {
+        BatchStep st;
+        st.kind = kind;
+        st.stream = (int8_t)stream;
+        st.event = e;
+        S.push_back(st);
+    };
     // set-up: one workgroup for the scan, the clears spread over up to 256 more
-    const int64_t nStat = zeroStats ? n : 0;
-    const int64_t clears = std::max<int64_t>(maxDepth + 2, partialDoubles / 2 + nStat);
-    const int gSetup = 1 + (int)std::min<int64_t>(256, (clears + 4 * kScanThreads - 1) / (4 * kScanThreads));
-    hipLaunchKernelGGL(k_wf_setup, dim3(gSetup), dim3(kScanThreads), 0, s, w, maxDepth + 2, (double *)partials,
-                       partialDoubles, nStat);
+    {
+        const int64_t nStat = zeroStats ? n : 0;
+        const int64_t clears = std::max<int64_t>(maxDepth + 2, partialDoubles / 2 + nStat);
+        const int gSetup = 1 + (int)std::min<int64_t>(256, (clears + 4 * kScanThreads - 1) / (4 * kScanThreads));
+        kernel(0, (const void *)k_wf_setup, gSetup, kScanThreads, 0, BatchStep::kSetup, maxDepth + 2,
+               partialDoubles, nStat);
+    }
     // generation: K cells per workgroup, about kRange paths each
     const int64_t q = n / kCells;
     const int K = (int)std::max<int64_t>(1, std::min<int64_t>(kGenCellsMax, kRange / (q + 1)));
-    if (a.refChain)
-        hipLaunchKernelGGL((k_wf_gen<true>), dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
-    else
-        hipLaunchKernelGGL((k_wf_gen<false>), dim3((kCells + K - 1) / K), dim3(kWfBlock), 0, s, a, w, K);
-    const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);
-    if (glds > 65536) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_wf_gather,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
-        if (e != hipSuccess) return e;
-    }
-    if (!a.onlyIndirect) {
-        if (a.refChain) hipLaunchKernelGGL((k_wf_nee0<true>), dim3(gNee), dim3(kWfBlock), 0, s, a, w);
-        else hipLaunchKernelGGL((k_wf_nee0<false>), dim3(gNee), dim3(kWfBlock), 0, s, a, w);
-    }
+    kernel(0, a.refChain ? (const void *)k_wf_gen<true> : (const void *)k_wf_gen<false>, (kCells + K - 1) / K,
+           kWfBlock, 0, BatchStep::kAWi, K);
+    const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);  // prepare_wavefront allowed it
+    if (!a.onlyIndirect)
+        kernel(0, a.refChain ? (const void *)k_wf_nee0<true> : (const void *)k_wf_nee0<false>, gNee, kWfBlock, 0,
+               BatchStep::kAW);
     for (int k = 0; k <= maxDepth; ++k) {
-        // the depth's closest-hit traversal on s2, beside its shadow and gather stages: it
-        // reads the extension queue and rays of depth k and writes only the hits, which the
+        // the depth's closest-hit traversal on stream 1, beside its shadow and gather stages:
+        // it reads the extension queue and rays of depth k and writes only the hits, which the
         // shading stage reads after the join. A depth without those stages (depth 0 when the
-        // direct paths are not traced) keeps it on s: no fork and join to wait for.
+        // direct paths are not traced) keeps it on stream 0: no fork and join to wait for.
         const bool nee = k >= 1 || !a.onlyIndirect;
-        const hipStream_t ts = nee ? s2 : s;
+        const int ts = nee ? 1 : 0;
         if (k < maxDepth) {
-            if (nee) {
-                (void)hipEventRecord(evFork[k], s);
-                (void)hipStreamWaitEvent(s2, evFork[k], 0);
-            }
-            (void)hipEventRecord(evClosest[2 * *nClosest], ts);
-            if (count)
-                hipLaunchKernelGGL((k_wf_trav<true>), dim3(gClosest), dim3(kWfBlock), 0, ts, a, w, k);
-            else
-                hipLaunchKernelGGL((k_wf_trav<false>), dim3(gClosest), dim3(kWfBlock), 0, ts, a, w, k);
-            (void)hipEventRecord(evClosest[2 * *nClosest + 1], ts);
-            ++*nClosest;
-            if (nee) (void)hipEventRecord(evJoin[k], s2);
+            if (nee) marker(BatchStep::kFork, 0, nullptr);
+            if (evClosest) marker(BatchStep::kRecord, ts, evClosest[2 * plan.nClosest]);
+            kernel(ts, count ? (const void *)k_wf_trav<true> : (const void *)k_wf_trav<false>, gClosest, kWfBlock,
+                   0, BatchStep::kAWi, k);
+            if (evClosest) marker(BatchStep::kRecord, ts, evClosest[2 * plan.nClosest + 1]);
+            ++plan.nClosest;
         }
         if (nee) {
             if (!a.poDiffraction) {
-                (void)hipEventRecord(evShadow[2 * *nShadow], s);
-                if (count)
-                    hipLaunchKernelGGL((k_wf_shadow<true>), dim3(gShadow), dim3(kWfBlock), 0, s, a, w, k);
-                else
-                    hipLaunchKernelGGL((k_wf_shadow<false>), dim3(gShadow), dim3(kWfBlock), 0, s, a, w, k);
-                (void)hipEventRecord(evShadow[2 * *nShadow + 1], s);
-                ++*nShadow;
+                if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
+                kernel(0, count ? (const void *)k_wf_shadow<true> : (const void *)k_wf_shadow<false>, gShadow,
+                       kWfBlock, 0, BatchStep::kAWi, k);
+                if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow + 1]);
+                ++plan.nShadow;
             }
-            hipLaunchKernelGGL(k_wf_gather, dim3(gridGather), dim3(kWfBlock), glds, s, a, w, k, partials, count);
+            kernel(0, (const void *)k_wf_gather, gridGather, kWfBlock, glds, BatchStep::kGather, k, count ? 1 : 0);
         }
         if (k < maxDepth) {
-            if (nee) (void)hipStreamWaitEvent(s, evJoin[k], 0);
-            if (a.refChain) hipLaunchKernelGGL((k_wf_shade<true>), dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
-            else hipLaunchKernelGGL((k_wf_shade<false>), dim3(gShade), dim3(kWfBlock), 0, s, a, w, k);
+            if (nee) marker(BatchStep::kJoin, 0, nullptr);
+            kernel(0, a.refChain ? (const void *)k_wf_shade<true> : (const void *)k_wf_shade<false>, gShade,
+                   kWfBlock, 0, BatchStep::kAWi, k);
         }
     }
+    return plan;
+}
+
+hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t s, hipStream_t s2, hipEvent_t *evFork,
+                       hipEvent_t *evJoin) {
+    (void)hipGetLastError();
+    const hipStream_t st[2] = {s, s2};
+    int forks = 0, joins = 0;
+    hipError_t r = hipSuccess;
+    for (const BatchStep &x : plan.steps) {
+        switch (x.kind) {
+            case BatchStep::kKernel: {
+                StepParams sp;
+                step_params(x, h, sp);
+                r = hipLaunchKernel(x.func, x.grid, x.block, sp.p, x.lds, st[x.stream]);
+                break;
+            }
+            case BatchStep::kFork:
+                r = hipEventRecord(evFork[forks], s);
+                if (r == hipSuccess) r = hipStreamWaitEvent(s2, evFork[forks], 0);
+                ++forks;
+                break;
+            case BatchStep::kJoin:
+                r = hipEventRecord(evJoin[joins], s2);
+                if (r == hipSuccess) r = hipStreamWaitEvent(s, evJoin[joins], 0);
+                ++joins;
+                break;
+            default: r = hipEventRecord(x.event, st[x.stream]); break;
+        }
+        if (r != hipSuccess) return r;
+    }
     return hipGetLastError();
+}
+
+hipError_t graph_plan(BatchGraph *out, const BatchPlan &plan, const BatchArgs &h) {
+    *out = BatchGraph{};
+    hipGraph_t g = nullptr;
+    hipError_t r = hipGraphCreate(&g, 0);
+    if (r != hipSuccess) return r;
+    // each node depends on the last node of its stream; a fork hands stream 0's to stream 1,
+    // a join adds stream 1's to stream 0's
+    std::vector<hipGraphNode_t> deps[2];
+    std::vector<hipGraphNode_t> nodes;
+    for (const BatchStep &x : plan.steps) {
+        hipGraphNode_t nd = nullptr;
+        if (x.kind == BatchStep::kKernel) {
+            StepParams sp;
+            step_params(x, h, sp);
+            hipKernelNodeParams kp{};
+            kp.blockDim = x.block;
+            kp.func = const_cast<void *>(x.func);
+            kp.gridDim = x.grid;
+            kp.kernelParams = sp.p;
+            kp.sharedMemBytes = x.lds;
+            r = hipGraphAddKernelNode(&nd, g, deps[x.stream].data(), deps[x.stream].size(), &kp);
+            nodes.push_back(nd);
+        } else if (x.kind == BatchStep::kFork) {
+            deps[1] = deps[0];
+        } else if (x.kind == BatchStep::kJoin) {
+            deps[0].insert(deps[0].end(), deps[1].begin(), deps[1].end());
+        } else {
+            r = hipGraphAddEventRecordNode(&nd, g, deps[x.stream].data(), deps[x.stream].size(), x.event);
+        }
+        if (r != hipSuccess) break;
+        if (nd) deps[x.stream].assign(1, nd);
+    }
+    for (int i = 0; i < 2 && r == hipSuccess; ++i) r = hipGraphInstantiate(&out->exec[i], g, nullptr, nullptr, 0);
+    for (int i = 0; i < 2 && r == hipSuccess; ++i) r = hipEventCreateWithFlags(&out->done[i], hipEventDisableTiming);
+    out->graph = g;
+    out->nodes = std::move(nodes);
+    if (r != hipSuccess) destroy_graph(*out);
+    return r;
+}
+
+hipError_t replay_plan(BatchGraph &g, const BatchPlan &plan, const BatchArgs &h, hipStream_t s) {
+    // the instance that ran two replays ago: its parameters change only once that run is over
+    const int i = g.next;
+    g.next ^= 1;
+    hipError_t r;
+    if (g.used[i]) {
+        while ((r = hipEventQuery(g.done[i])) == hipErrorNotReady) {
+        }
+        if (r != hipSuccess) return r;
+    }
+    size_t j = 0;
+    for (const BatchStep &x : plan.steps) {
+        if (x.kind != BatchStep::kKernel) continue;
+        StepParams sp;
+        step_params(x, h, sp);
+        hipKernelNodeParams kp{};
+        kp.blockDim = x.block;
+        kp.func = const_cast<void *>(x.func);
+        kp.gridDim = x.grid;
+        kp.kernelParams = sp.p;
+        kp.sharedMemBytes = x.lds;
+        if ((r = hipGraphExecKernelNodeSetParams(g.exec[i], g.nodes[j++], &kp)) != hipSuccess) return r;
+    }
+    if ((r = hipGraphLaunch(g.exec[i], s)) != hipSuccess) return r;
+    g.used[i] = true;
+    return hipEventRecord(g.done[i], s);
+}
+
+void destroy_graph(BatchGraph &g) {
+    for (int i = 0; i < 2; ++i) {
+        if (g.exec[i]) (void)hipGraphExecDestroy(g.exec[i]);
+        if (g.done[i]) (void)hipEventDestroy(g.done[i]);
+    }
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = BatchGraph{};
 }
 
 }  // namespace hsr

@@ -55,8 +55,9 @@ class RequestInfo:
 
 class Server:
     def __init__(self, device: int = 0, slots: int = 4, queueCapacity: int = 0, maxDepth: int = 0,
-                 hbmBudget: int = 0, pathsPerBatch: int = 0, frameReserveBytes: int = 0):
-        cfg = c_server_config(device, slots, queueCapacity, maxDepth, hbmBudget, pathsPerBatch, frameReserveBytes)
+                 hbmBudget: int = 0, pathsPerBatch: int = 0, frameReserveBytes: int = 0, graphs: bool = False):
+        cfg = c_server_config(device, slots, queueCapacity, maxDepth, hbmBudget, pathsPerBatch, frameReserveBytes,
+                              1 if graphs else 0, 0)
         h = C.c_void_p()
         check(lib().hsr_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

@@ -69,7 +69,8 @@ class c_run_stats(C.Structure):
                 ("tris_tested", C.c_int64), ("rays_closest", C.c_int64),
                 ("rays_shadow", C.c_int64), ("events", C.c_int64), ("bvh_build_ms", C.c_double),
                 ("closest_ms", C.c_double), ("shadow_ms", C.c_double), ("closest_launches", C.c_int64),
-                ("shadow_launches", C.c_int64), ("nodes_shadow", C.c_int64), ("tris_shadow", C.c_int64)]
+                ("shadow_launches", C.c_int64), ("nodes_shadow", C.c_int64), ("tris_shadow", C.c_int64),
+                ("graph_launches", C.c_int64), ("graph_captures", C.c_int64)]
 
 
 class c_device_info(C.Structure):
@@ -81,7 +82,7 @@ class c_device_info(C.Structure):
 class c_server_config(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("queueCapacity", C.c_int32),
                 ("maxDepth", C.c_int32), ("hbmBudget", C.c_int64), ("pathsPerBatch", C.c_int64),
-                ("frameReserveBytes", C.c_int64)]
+                ("frameReserveBytes", C.c_int64), ("graphs", C.c_int32), ("reserved0", C.c_int32)]
 
 
 class c_request(C.Structure):

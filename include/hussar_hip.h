@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define HSR_ABI_VERSION 3
+#define HSR_ABI_VERSION 4
 #define HSR_MAX_CHANNELS 16
 #define HSR_MAX_TX 4
 
@@ -266,6 +266,10 @@ typedef struct hsr_run_stats {
     int64_t shadow_launches;
     int64_t nodes_shadow;
     int64_t tris_shadow;
+    /* wavefront batches launched by replaying a captured graph (HSR_OPT_GRAPH), and graphs
+     * captured, since the statistics were last reset */
+    int64_t graph_launches;
+    int64_t graph_captures;
 } hsr_run_stats;
 
 hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
@@ -314,6 +318,16 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        FP64 chain refined on the same triangles still gives the path lengths,
  *                        time, IF phase and GO weights. */
 #define HSR_OPT_CHAIN 9
+/*  HSR_OPT_GRAPH         wavefront pipeline: 1 = each batch's launch sequence (set-up,
+ *                        generation, the per-depth traversal, shading and gather stages, with
+ *                        their stream fork/join and timing events) is built into a HIP graph
+ *                        (node by node, no stream capture) at the second batch of its launch
+ *                        shape (paths, depth, receivers, frame tile, options) and replayed from
+ *                        then on with each batch's arguments set into its kernel nodes (up to 8
+ *                        shapes per backend); 0 = launched stage by stage (default). A graph that
+ *                        fails to build turns the option off for the backend; hsr_run_stats
+ *                        counts the replays. */
+#define HSR_OPT_GRAPH 10
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]
@@ -383,6 +397,9 @@ typedef struct hsr_server_config {
     int64_t pathsPerBatch;     /* wavefront batch per slot; 0 = 2^22 */
     int64_t frameReserveBytes; /* largest frame a request may have (sum + compensation bytes); 0 = no
                                   limit beyond the HBM budget */
+    int32_t graphs;            /* 1 = the slots replay their batches' graphs (HSR_OPT_GRAPH 1);
+                                  0 = they launch them stage by stage (default) */
+    int32_t reserved0;
 } hsr_server_config;
 
 typedef struct hsr_request {
