@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 import hussar_amd as H  # noqa: E402
 from hussar_amd import scenes  # noqa: E402
-from hussar_amd._lib import OPT_COUNTERS, OPT_GRAPH, OPT_PIPELINE, OPT_RECORD_HITS  # noqa: E402
+from hussar_amd._lib import OPT_COUNTERS, OPT_GRAPH, OPT_PIPELINE, OPT_RECORD_HITS, OPT_TAIL  # noqa: E402
 
 METRIC = "paths/s & frames/s at ≤1e-3 IF rel-L2 vs FP64; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -140,8 +140,10 @@ def roofline(pipeline, acc, cnt, steps, pmc=None, pmc_error=None, clock_mhz=2400
        issue rate (SIMDS x clock / 4).
     `bound` names the largest measured fraction; the top-level achieved / peak / frac are its."""
     if pipeline == 1:
-        n = cnt["nodes_visited"] - cnt["nodes_shadow"]
-        t = cnt["tris_tested"] - cnt["tris_shadow"]
+        # the closest-hit traversal launches' own visits: not the shadow traversal's nor those of
+        # the fused tail (the last bounces, one launch per batch)
+        n = cnt["nodes_visited"] - cnt["nodes_shadow"] - cnt["nodes_tail"]
+        t = cnt["tris_tested"] - cnt["tris_shadow"] - cnt["tris_tail"]
         launches = cnt["closest_launches"]
         kms = acc["closest_ms"] / acc["closest_launches"]
         kernel, sym = "k_wf_trav<false> (closest-hit traversal, one launch per bounce)", "k_wf_trav_closest"
@@ -510,6 +512,8 @@ def main():
     ap.add_argument("--requests", type=int, default=96, help="serving: requests in the trace")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent slots")
     ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--tail", type=int, default=-1,
+                    help="first depth of the fused tail (HSR_OPT_TAIL), 0 none, -1 the library's default")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="1 replay each batch shape's graph (HSR_OPT_GRAPH), 0 stage-by-stage launches")
     ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
@@ -543,6 +547,8 @@ def main():
     backend = H.HipBackend(mesh, pt)
     backend.set_option(OPT_PIPELINE, args.pipeline)
     backend.set_option(OPT_GRAPH, args.graph)
+    if args.tail >= 0:
+        backend.set_option(OPT_TAIL, args.tail)
     bvh_ms = backend.stats()["bvh_build_ms"]
     setup_s = time.perf_counter() - t0
 

@@ -23,6 +23,7 @@ OPT_BATCH_PATHS = 7
 OPT_CUBE_EVENTS = 8
 OPT_CHAIN = 9
 OPT_GRAPH = 10
+OPT_TAIL = 11
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER

@@ -131,7 +131,7 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     HSR_HIP(hipMemcpy(b->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(float), hipMemcpyHostToDevice));
     b->slotIds.resize(bvh.slotCount);
     for (int32_t s2 = 0; s2 < bvh.slotCount; ++s2) std::memcpy(&b->slotIds[s2], &bvh.tris[12 * (size_t)s2 + 3], 4);
-    if ((s = dalloc(&b->d_counters, 8)) != HSR_OK) return s;
+    if ((s = dalloc(&b->d_counters, 16)) != HSR_OK) return s;
     {
         static const std::vector<int32_t> cells = hsr::coherence_cells();
         if ((s = dalloc(&b->d_cells, cells.size())) != HSR_OK) return s;
@@ -405,6 +405,10 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
                 return fail(HSR_ERR_UNSUPPORTED, "phase mode must be 0 (FP64), 1 (FP32) or 2 (double-float)");
             b->optPhaseMode = (int32_t)value;
             return HSR_OK;
+        case HSR_OPT_TAIL:
+            if (value < 0 || value > 64) return fail(HSR_ERR_INVALID, "tail depth must lie in [0, 64]");
+            b->optTail = (int32_t)value;
+            return HSR_OK;
         case HSR_OPT_GRAPH:
             if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "graph must be 0 (stage launches) or 1 (replay)");
             b->optGraph = value != 0;
@@ -659,7 +663,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         b->recMaxDepth = md;
         b->recC = C;
     }
-    if (b->optCounters) HSR_HIP(hipMemsetAsync(b->d_counters, 0, 8 * sizeof(unsigned long long), b->stream));
+    if (b->optCounters) HSR_HIP(hipMemsetAsync(b->d_counters, 0, 16 * sizeof(unsigned long long), b->stream));
 
     {
         hsr_status s = settle_timing(b);  // before ev[] is recorded again
@@ -757,20 +761,26 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         }
         // the batch's launch sequence: replayed from the graph of its launch shape with the
         // batch's arguments, or stage by stage
-        HSR_HIP(hsr::prepare_wavefront(N, C, cube));
+        // the fused tail from depth optTail on, when the batch has bounces past it and its LDS fits
+        const int tailDepth = b->optTail >= 1 && b->optTail < knobs->maxDepth &&
+                                      hsr::tail_lds_bytes(N, C, cube || noSplat) <= (size_t)b->ldsPerBlock
+                                  ? b->optTail
+                                  : 0;
+        HSR_HIP(hsr::prepare_wavefront(N, C, cube, cube || noSplat));
         hsr::BatchArgs host;
         host.a = a;
         host.w = w;
         host.partials = (double2 *)b->d_partials;
         auto plan = [&] {
             return hsr::plan_wavefront(a, w, gridTrav, gridWork, gridGather, partialDoubles, training,
-                                       b->optCounters, evC, evS);
+                                       b->optCounters, evC, evS, tailDepth);
         };
         hsr_backend::GraphEntry *graph = nullptr;
         if (b->optGraph) {
             const std::vector<int64_t> key = {n, gridTrav, gridWork, gridGather, partialDoubles, training,
                                               b->optCounters, a.maxDepth, a.onlyIndirect, a.poDiffraction,
-                                              a.refChain, a.cube, a.rxCount, a.C, a.N, (int64_t)(intptr_t)evC};
+                                              a.refChain, a.cube, a.rxCount, a.C, a.N, (int64_t)(intptr_t)evC,
+                                              tailDepth, noSplat};
             graph = batch_graph(b, key, plan, host);
         }
         if (graph) {
@@ -895,10 +905,13 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     b->stats.paths += budget;
     b->stats.launches += launches;
     if (b->optCounters) {
-        unsigned long long c[8];
-        HSR_HIP(hipMemcpy(c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost));
-        b->stats.nodes_visited += (int64_t)(c[0] + c[5]);
-        b->stats.tris_tested += (int64_t)(c[1] + c[6]);
+        unsigned long long c[16];
+        HSR_HIP(hipMemcpyAsync(c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost, b->stream));
+        HSR_HIP(hipStreamSynchronize(b->stream));
+        b->stats.nodes_visited += (int64_t)(c[0] + c[5] + c[7]);
+        b->stats.tris_tested += (int64_t)(c[1] + c[6] + c[8]);
+        b->stats.nodes_tail += (int64_t)c[7];
+        b->stats.tris_tail += (int64_t)c[8];
         b->stats.nodes_shadow += (int64_t)c[5];
         b->stats.tris_shadow += (int64_t)c[6];
         b->stats.rays_closest += (int64_t)c[2];

@@ -109,6 +109,7 @@ struct hsr_backend {
     int64_t optPathsPerLaunch = int64_t(1) << 24;
     int32_t optPhaseMode = 0;
     int32_t optChain = 0;  // HSR_OPT_CHAIN
+    int32_t optTail = 2;   // HSR_OPT_TAIL: first depth of the fused tail, 0 = none
     int32_t optPipeline = 1;
     int32_t optTravWaves = 0;  // resident waves per CU of the traversal kernels, 0 = all that fit
     int64_t optCubeEvents = -1;  // cube event list capacity (HSR_OPT_CUBE_EVENTS), -1 = the default

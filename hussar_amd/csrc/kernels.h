@@ -167,15 +167,18 @@ struct BatchArgs {
     WfArgs w;
     double2 *partials;  // [gatherGrid][N*C] partial frames, accumulated by the gather stages
 };
-// Allows the gather stage the LDS of an N x C tile (beyond 64 KiB); before the first batch of a
-// frame shape.
-hipError_t prepare_wavefront(int N, int C, bool cube);
+// Allows the gather stage and the fused tail the LDS of an N x C tile (beyond 64 KiB); before
+// the first batch of a frame shape. noTile: the batch splats into no LDS tile (cube frames, or
+// nothing splatted).
+hipError_t prepare_wavefront(int N, int C, bool cube, bool noTile);
+// LDS of the fused tail: the frame tile (unless noTile) and the traversal stack.
+size_t tail_lds_bytes(int N, int C, bool noTile);
 // One step of a batch's launch sequence: a kernel on stream 0 or 1 (its by-value parameters
 // built from the batch's arguments and the step's own scalars x, after the layout sig), a
 // fork (stream 1 waits for stream 0's work so far), a join (the reverse) or a timing event.
 struct BatchStep {
     enum : int8_t { kKernel, kFork, kJoin, kRecord };
-    enum : int8_t { kSetup, kAW, kAWi, kGather };
+    enum : int8_t { kSetup, kAW, kAWi, kGather, kTail };
     int8_t kind = kKernel, stream = 0, sig = kAW;
     hipEvent_t event = nullptr;
     const void *func = nullptr;
@@ -191,14 +194,17 @@ struct BatchStep {
 // evShadow (optional): room for 2 * (maxDepth + 1) events each, a pair recorded around every
 // closest-hit / shadow traversal launch. The set-up computes the coherence order's slot
 // ranges and clears the queue counters, the first partialDoubles doubles of the partials and
-// (zeroStats) the training records.
+// (zeroStats) the training records. tailDepth K in [1, maxDepth): the fused tail (k_wf_tail)
+// takes the paths from the depth-K hits on, in place of the stages from the depth-K shading on
+// (its node visits and triangle tests are counted in counters[7] and [8]); otherwise (0) every
+// depth has its stages.
 struct BatchPlan {
     std::vector<BatchStep> steps;
     int nClosest = 0, nShadow = 0;  // traversal launches (timed pairs)
 };
 BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
                          int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
-                         hipEvent_t *evShadow);
+                         hipEvent_t *evShadow, int tailDepth);
 // The plan launched with the arguments h on stream (0) and stream2 (1), ordered by the events
 // evFork / evJoin (maxDepth + 1 each).
 hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t stream, hipStream_t stream2,

@@ -917,6 +917,235 @@ __global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceA
         tile_flush(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
 }
 
+// ------------------------------------------------------------------------ fused tail
+// The last bounces of a batch in one launch. Each lane takes a path of the depth-K extension
+// queue (its ray traced, its hit in w.hit) and carries it to its end: the hit update, the NEE at
+// the new vertex towards every receiver (set-up, any-hit shadow rays, guiding weight, splat,
+// records), the next extension ray (closest hit), bounce after bounce, and the guiding record
+// of the finished path. This replaces the stages from the depth-K shading on, few paths each
+// and every one waiting on the longest ray of the one before, by one launch whose length is
+// that of its longest path. Every value comes from the device functions the stages use (hit
+// update, termination, NEE set-up, closest-hit traversal, the shadow stage's packets of four
+// receivers, event), in the stages' order, so hits, visibility, records, guiding weights and
+// events equal the stages'. The loop is wave-uniform per bounce; idle lanes take further paths
+// from the queue through the consumer heads of depth K + 1, whose stages do not run. Events go
+// to a list per wave in LDS and are splatted 64 at a time (an LDS atomic costs the same for one
+// lane as for 64), the rest at the end; the workgroup's frame tile is then added into its
+// partial frame (blockIdx.x < the gather grid) like a gather stage's.
+constexpr int kTailEvents = 128;  // events listed per wave (64 pending at most, plus one append)
+struct TailEvent {
+    int32_t center, ch;
+    float s, are, aim;
+    int32_t delta;
+};
+
+// Any-hit visibility of receivers [c0, c0 + rays) of the lane's vertex p: one packet of the
+// shadow stage (the same rays, the same traversal), bit r of the result = receiver c0 + r is
+// visible.
+template <bool kRef, bool kCount>
+__device__ inline uint32_t tail_visibility(const TraceArgs &a, const PathState &p, int c0, int rays, Stack &st,
+                                           uint32_t &cn, uint32_t &ct) {
+    Packet pk;
+    pk.alive = 0;
+    uint32_t sg[kPacket];
+    const float inf = __builtin_inff();
+    pk.ivlo = {inf, inf, inf};
+    pk.ivhi = {-inf, -inf, -inf};
+    pk.tmax = 0.0f;
+#pragma unroll
+    for (int r = 0; r < kPacket; ++r) {
+        f3 iv = {0.0f, 0.0f, 1.0f};
+        if (r < rays) {
+            const NeeSetup ns = nee_setup<kRef>(a, p, c0 + r);
+            pk.o = ns.o;  // the same vertex for every receiver
+            pk.d[r] = ns.d;
+            iv = {safe_rcp(ns.d.x), safe_rcp(ns.d.y), safe_rcp(ns.d.z)};
+            pk.tfar[r] = ns.tfar;
+            pk.alive |= 1u << r;
+            pk.ivlo = {fminf(pk.ivlo.x, iv.x), fminf(pk.ivlo.y, iv.y), fminf(pk.ivlo.z, iv.z)};
+            pk.ivhi = {fmaxf(pk.ivhi.x, iv.x), fmaxf(pk.ivhi.y, iv.y), fmaxf(pk.ivhi.z, iv.z)};
+            pk.tmax = fmaxf(pk.tmax, ns.tfar);
+        } else {
+            pk.d[r] = {0.0f, 0.0f, 1.0f};
+            pk.tfar[r] = 0.0f;
+        }
+        sg[r] = (iv.x >= 0.0f ? 1u : 0u) | (iv.y >= 0.0f ? 2u : 0u) | (iv.z >= 0.0f ? 4u : 0u);
+    }
+    pk.sgn = sg[0];
+    pk.same = true;
+#pragma unroll
+    for (int r = 1; r < kPacket; ++r)
+        if (r < rays && sg[r] != sg[0]) pk.same = false;
+    pk.node = 0;
+    st.sp = 0;
+    while (pk.node != kDone) packet_run<kCount>(a.nodesF, a.tris, pk, st, kTravSteps, cn, ct);
+    return pk.alive;
+}
+
+template <bool kRef, bool kCount>
+__global__ __launch_bounds__(kWfBlock, 2) void k_wf_tail(TraceArgs a, WfArgs w, int K, double2 *partials) {
+    // [frame tile (splat, no cube)] [kStackShort][kWfBlock] stack [4][kTailEvents] event lists
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const bool cube = a.cube != 0, splat = !a.noSplat;  // uniform
+    const bool guard = a.N >= 2 * kGuard;
+    const bool tiled = splat && !cube;
+    const int tileDoubles = tiled ? 2 * (guard ? guard_row(a.N) : a.N) * a.C : 0;
+    double *tile = smem;
+    int32_t *lds_stack = (int32_t *)(smem + tileDoubles);
+    TailEvent *evl = (TailEvent *)(lds_stack + kStackShort * kWfBlock) + (tid >> 6) * kTailEvents;
+    for (int j = tid; j < tileDoubles; j += kWfBlock) tile[j] = 0.0;
+    __syncthreads();
+    int32_t spill[kStackSpill];
+    Stack st;
+    st.spill = spill;
+    st.set_column(lds_stack + tid);
+    st.sp = 0;
+    uint32_t cn = 0, ct = 0, cev = 0;
+    int pending = 0;  // events in the wave's list (wave-uniform)
+    const unsigned int n = w.cnt[K].n_ext.v;
+    const int32_t *queue = (K & 1) ? w.q_ext1 : w.q_ext0;
+    WfCounter *heads = w.cnt[K + 1].head_ext;  // depth K + 1 has no stages of its own
+    const int nrx = min(a.rxCount, a.C);
+    const int64_t P = w.P;
+    PathState p;
+    int64_t i = 0;      // the lane's path slot
+    int32_t hx = -1;    // slot of the triangle its current ray hit, -1 none
+    float ht = 0.0f;    // the hit's distance (the traversal's)
+    bool active = false;
+    Claim cl = claim_begin();
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        if (!cl.dry && __popcll(idle) >= kRefillMin) {
+            const unsigned my = claim_items(heads, n, idle, lane, cl);
+            if (my != ~0u) {
+                i = queue[my];
+                load_chain(a, w, i, p, K);
+                load_field(w, i, p);
+                if (a.motion) load_motion(a, w, i, p, K);
+                load_record(w, i, p, false);  // K >= 1: the depth-K gather stored the guiding weight
+                p.freq = w.sf[F_FREQ * P + i];
+                if (kRef) {  // the FP32 chain is the ray that was traced
+                    const float4 o = w.ray_o[i], d = w.ray_d[i];
+                    p.c32.o = {o.x, o.y, o.z};
+                    p.c32.r = o.w;
+                    p.c32.d = {d.x, d.y, d.z};
+                }
+                const int2 h = w.hit[i];
+                hx = h.x;
+                ht = __int_as_float(h.y);
+                active = true;
+            }
+        }
+        if (!__any(active)) {
+            if (cl.dry) break;
+            continue;
+        }
+        // the hit of this bounce and the new vertex (the shading stage)
+        bool vertex = false, ends = false;
+        const bool rec = active && p.local < a.recPaths;
+        if (active) {
+            if (a.recHits && rec)
+                a.recHits[p.local * a.maxDepth + p.depth] = hx >= 0 ? __float_as_int(a.tris[3 * hx].w) : -1;
+            vertex = hx >= 0 && apply_hit<kRef>(a, p, hx, ht);
+            ends = vertex && path_ends<kRef>(a, p);
+        }
+        // NEE at the new vertex towards every receiver (the shading stage's set-up, the shadow
+        // stage's packets of four receivers, the gather stage)
+        for (int c0 = 0; c0 < nrx; c0 += kPacket) {
+            const int rays = min(kPacket, nrx - c0);
+            uint32_t vis = (1u << rays) - 1u;
+            if (vertex && !a.poDiffraction) vis = tail_visibility<kRef, kCount>(a, p, c0, rays, st, cn, ct);
+            for (int r = 0; r < rays; ++r) {
+                const int ch = c0 + r;
+                bool has = false;
+                Event ev{0, 0.0f, {0.0f, 0.0f}, 0};
+                if (vertex) {
+                    const NeeSetup ns = nee_setup<kRef>(a, p, ch);
+                    int8_t flag = 1;
+                    if (vis & (1u << r)) {
+                        flag = 2;
+                        if (ns.gw1.re != 0.0f || ns.gw1.im != 0.0f) p.gw = p.gw + ns.gw1;
+                        if (ns.gwAdd) p.gw = p.gw + ns.v;
+                        if (ns.splat) {
+                            flag = 3;
+                            if (splat) {
+                                ev = make_event(a, ns.tn, ns.v, a.sampleWeight, ns.dv);
+                                has = true;
+                                if (kCount) ++cev;
+                            }
+                        }
+                    }
+                    if (a.recNee && rec)
+                        a.recNee[(p.local * a.maxDepth + (p.depth - 1)) * a.C + a.chBase + ch] = flag;
+                }
+                if (!splat) continue;
+                if (cube) {
+                    cube_event(has, ev, a.chBase + ch, a, lane);
+                    continue;
+                }
+                // the wave's event list, splatted 64 at a time
+                const uint64_t m = __ballot(has);
+                if (has) {
+                    TailEvent &t = evl[pending + __popcll(m & ((1ull << lane) - 1ull))];
+                    t.center = ev.center;
+                    t.ch = a.chBase + ch;
+                    t.s = ev.s;
+                    t.are = ev.A.re;
+                    t.aim = ev.A.im;
+                    t.delta = ev.delta;
+                }
+                pending += __popcll(m);
+                if (pending >= 64) {
+                    pending -= 64;
+                    const TailEvent t = evl[pending + lane];
+                    const Event e{t.center, t.s, {t.are, t.aim}, t.delta};
+                    if (guard)
+                        flush_events_g(true, e, t.ch, tile, a.N, a.C, lane);
+                    else
+                        flush_events(true, e, t.ch, tile, a.N, a.C, lane);
+                }
+            }
+        }
+        // the path's end (its guiding record) or its next ray
+        if (active) {
+            if (!vertex || ends) {
+                finish_record(a, w, true, i, p);
+                active = false;
+            } else {
+                const f3 ro = kRef ? p.c32.o : cast3<float>(p.o);
+                const f3 rd = kRef ? p.c32.d : cast3<float>(p.d);
+                float t;
+                int32_t slot;
+                const int32_t id = closest_hit<kCount>(a.nodes, a.tris, ro, rd, st, t, slot, cn, ct);
+                hx = id >= 0 ? slot : -1;
+                ht = t;
+            }
+        }
+    }
+    if (kCount) {
+        wave_count(&a.counters[7], cn, lane);
+        wave_count(&a.counters[8], ct, lane);
+        wave_count(&a.counters[4], cev, lane);
+    }
+    if (!tiled) return;
+    if (lane < pending) {  // the rest of the wave's list
+        const TailEvent t = evl[lane];
+        const Event e{t.center, t.s, {t.are, t.aim}, t.delta};
+        if (guard)
+            flush_events_g(true, e, t.ch, tile, a.N, a.C, lane);
+        else
+            flush_events(true, e, t.ch, tile, a.N, a.C, lane);
+    }
+    __syncthreads();
+    const int NC = a.N * a.C;
+    if (guard)
+        tile_flush_g(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
+    else
+        tile_flush(tile, partials + (size_t)blockIdx.x * NC, a.N, a.C, tid, kWfBlock);
+}
+
 // ------------------------------------------------------------------------ cube splat
 // The listed events of a cube frame into the frame through LDS: workgroup blockIdx / G owns
 // the tile of channel c and chirps [m0, m0 + Mt) (all samples), blockIdx % G = g its share
@@ -1242,10 +1471,28 @@ int wavefront_trav_blocks_per_cu(bool count) {
     return e == hipSuccess && n > 0 ? n : 1;
 }
 
-hipError_t prepare_wavefront(int N, int C, bool cube) {
+size_t tail_lds_bytes(int N, int C, bool noTile) {
+    return (noTile ? 0 : gather_lds_bytes(N, C)) + sizeof(int32_t) * kStackShort * kWfBlock +
+           sizeof(TailEvent) * kTailEvents * (kWfBlock / 64);
+}
+
+hipError_t prepare_wavefront(int N, int C, bool cube, bool noTile) {
     const size_t glds = cube ? 0 : gather_lds_bytes(N, C);
-    if (glds <= 65536) return hipSuccess;
-    return hipFuncSetAttribute((const void *)k_wf_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
+    if (glds > 65536) {
+        const hipError_t e =
+            hipFuncSetAttribute((const void *)k_wf_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
+        if (e != hipSuccess) return e;
+    }
+    const size_t tlds = tail_lds_bytes(N, C, noTile);
+    if (tlds > 65536) {
+        const void *fs[4] = {(const void *)k_wf_tail<false, false>, (const void *)k_wf_tail<false, true>,
+                             (const void *)k_wf_tail<true, false>, (const void *)k_wf_tail<true, true>};
+        for (const void *f : fs) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tlds);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
 }
 
 // ------------------------------------------------------------------------ batch plans
@@ -1282,6 +1529,14 @@ static void step_params(const BatchStep &st, const BatchArgs &h, StepParams &sp)
             sp.p[1] = w;
             sp.p[2] = &sp.i0;
             break;
+        case BatchStep::kTail:  // (TraceArgs, WfArgs, int K, double2 *partials)
+            sp.i0 = (int)st.x[0];
+            sp.d2p = h.partials;
+            sp.p[0] = a;
+            sp.p[1] = w;
+            sp.p[2] = &sp.i0;
+            sp.p[3] = &sp.d2p;
+            break;
         default:  // kGather: (TraceArgs, WfArgs, int depth, double2 *partials, bool count)
             sp.i0 = (int)st.x[0];
             sp.d2p = h.partials;
@@ -1297,7 +1552,7 @@ static void step_params(const BatchStep &st, const BatchArgs &h, StepParams &sp)
 
 BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
                          int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
-                         hipEvent_t *evShadow) {
+                         hipEvent_t *evShadow, int tailDepth) {
     BatchPlan plan;
     auto &S = plan.steps;
     const int maxDepth = a.maxDepth;
@@ -1378,6 +1633,16 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
         }
         if (k < maxDepth) {
             if (nee) marker(BatchStep::kJoin, 0, nullptr);
+            if (tailDepth >= 1 && k == tailDepth) {
+                // the fused tail from the depth-k hits on, in place of the stages from this
+                // depth's shading on
+                const void *f = a.refChain ? (count ? (const void *)k_wf_tail<true, true>
+                                                    : (const void *)k_wf_tail<true, false>)
+                                           : (count ? (const void *)k_wf_tail<false, true>
+                                                    : (const void *)k_wf_tail<false, false>);
+                kernel(0, f, gridGather, kWfBlock, tail_lds_bytes(a.N, a.C, a.cube || a.noSplat), BatchStep::kTail, k);
+                break;
+            }
             kernel(0, a.refChain ? (const void *)k_wf_shade<true> : (const void *)k_wf_shade<false>, gShade,
                    kWfBlock, 0, BatchStep::kAWi, k);
         }

@@ -270,6 +270,9 @@ typedef struct hsr_run_stats {
      * captured, since the statistics were last reset */
     int64_t graph_launches;
     int64_t graph_captures;
+    /* the fused tail's share of nodes_visited / tris_tested (HSR_OPT_TAIL, counters enabled) */
+    int64_t nodes_tail;
+    int64_t tris_tail;
 } hsr_run_stats;
 
 hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
@@ -328,6 +331,12 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        fails to build turns the option off for the backend; hsr_run_stats
  *                        counts the replays. */
 #define HSR_OPT_GRAPH 10
+/*  HSR_OPT_TAIL          wavefront pipeline: K >= 1 = the paths still going after the depth-K
+ *                        extension rays are finished by one fused kernel (hit update, NEE with
+ *                        per-ray shadow traversal, splat and further bounces per lane) instead
+ *                        of a stage launch per depth, when maxDepth > K (default 2); 0 = every
+ *                        depth has its stage launches. Results are those of the stages. */
+#define HSR_OPT_TAIL 11
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]
