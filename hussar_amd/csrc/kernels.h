@@ -233,6 +233,9 @@ constexpr int kStatLdsNodes = 5120;  // trees up to this size are summed in LDS 
 constexpr int64_t kStatRows = int64_t(1) << 21;  // doubles of partial rows a backend holds
 hipError_t launch_stat_reduce(const double *statV, const int32_t *statL, int64_t n, int nodes, double *rows,
                               int64_t rowsCap, double *stat, hipStream_t stream);
+// LDS of one chirp's frame tile of N samples x C channels, and of a gather stage's workgroup
+// (the tile).
+size_t tile_lds_bytes(int N, int C);
 size_t gather_lds_bytes(int N, int C);
 // partials [W][M] doubles -> stage [S][M] (sum, comp) -> frame (sum, comp) += Kahan
 hipError_t launch_reduce(const double *partials, int W, int M, double *stage_sum, double *stage_comp, int S,

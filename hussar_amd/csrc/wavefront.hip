@@ -824,6 +824,51 @@ __global__ __launch_bounds__(kWfBlock, HSR_NEE_WAVES) void k_wf_nee0(TraceArgs a
     }
 }
 
+// ------------------------------------------------------------------------ wave event lists
+// A wave's events waiting for the LDS tile splat. An LDS atomic instruction costs the same for
+// one active lane as for 64, so the fused tail, whose waves have few lanes with an event at a
+// time, lists its events per wave and splats them 64 at a time, one per lane (flush_events_g
+// takes each lane's own channel row). The gather stage splats at once: nearly all of its lanes
+// have an event per receiver, and a list there measured slower (more instructions per event).
+constexpr int kWaveEvents = 128;  // entries per wave: 64 pending at most, plus one append of 64
+// One listed event in 16 bytes (one LDS store and one load): the centre bin (bits 0-19), the
+// frame channel (bits 20-27) and the delta flag (bit 28), the fractional offset, the amplitude.
+struct alignas(16) WaveEvent {
+    int32_t key;
+    float s, are, aim;
+};
+constexpr size_t kWaveEventBytes = sizeof(WaveEvent) * kWaveEvents * (kWfBlock / 64);  // per workgroup
+
+__device__ inline void splat_listed(const WaveEvent &t, double *tile, int N, int C, bool guard, int lane) {
+    const Event e{t.key & 0xFFFFF, t.s, {t.are, t.aim}, (t.key >> 28) & 1};
+    const int ch = (t.key >> 20) & 0xFF;
+    if (guard)
+        flush_events_g(true, e, ch, tile, N, C, lane);
+    else
+        flush_events(true, e, ch, tile, N, C, lane);
+}
+
+// Appends the lane's event (if has) of frame channel ch to the wave's list and splats 64 of
+// them once 64 are pending; pending is wave-uniform. All lanes of the wave call it.
+__device__ inline void list_event(bool has, const Event &ev, int ch, WaveEvent *list, int &pending, double *tile,
+                                  int N, int C, bool guard, int lane) {
+    const uint64_t m = __ballot(has);
+    if (m == 0) return;
+    if (has)
+        list[pending + __popcll(m & ((1ull << lane) - 1ull))] =
+            WaveEvent{ev.center | (ch << 20) | (ev.delta ? 1 << 28 : 0), ev.s, ev.A.re, ev.A.im};
+    pending += __popcll(m);
+    if (pending >= 64) {
+        pending -= 64;
+        splat_listed(list[pending + lane], tile, N, C, guard, lane);
+    }
+}
+// The wave's remaining events (fewer than 64).
+__device__ inline void list_flush(const WaveEvent *list, int pending, double *tile, int N, int C, bool guard,
+                                  int lane) {
+    if (lane < pending) splat_listed(list[lane], tile, N, C, guard, lane);
+}
+
 // ------------------------------------------------------------------------ gather + splat
 __global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceArgs a, WfArgs w, int depth,
                                                                           double2 *partials, bool count) {
@@ -929,15 +974,8 @@ __global__ __launch_bounds__(kWfBlock, HSR_GATHER_WAVES) void k_wf_gather(TraceA
 // receivers, event), in the stages' order, so hits, visibility, records, guiding weights and
 // events equal the stages'. The loop is wave-uniform per bounce; idle lanes take further paths
 // from the queue through the consumer heads of depth K + 1, whose stages do not run. Events go
-// to a list per wave in LDS and are splatted 64 at a time (an LDS atomic costs the same for one
-// lane as for 64), the rest at the end; the workgroup's frame tile is then added into its
+// to the wave's event list (list_event); the workgroup's frame tile is then added into its
 // partial frame (blockIdx.x < the gather grid) like a gather stage's.
-constexpr int kTailEvents = 128;  // events listed per wave (64 pending at most, plus one append)
-struct TailEvent {
-    int32_t center, ch;
-    float s, are, aim;
-    int32_t delta;
-};
 
 // Any-hit visibility of receivers [c0, c0 + rays) of the lane's vertex p: one packet of the
 // shadow stage (the same rays, the same traversal), bit r of the result = receiver c0 + r is
@@ -994,7 +1032,7 @@ __global__ __launch_bounds__(kWfBlock, 2) void k_wf_tail(TraceArgs a, WfArgs w, 
     const int tileDoubles = tiled ? 2 * (guard ? guard_row(a.N) : a.N) * a.C : 0;
     double *tile = smem;
     int32_t *lds_stack = (int32_t *)(smem + tileDoubles);
-    TailEvent *evl = (TailEvent *)(lds_stack + kStackShort * kWfBlock) + (tid >> 6) * kTailEvents;
+    WaveEvent *list = (WaveEvent *)(lds_stack + kStackShort * kWfBlock) + (tid >> 6) * kWaveEvents;
     for (int j = tid; j < tileDoubles; j += kWfBlock) tile[j] = 0.0;
     __syncthreads();
     int32_t spill[kStackSpill];
@@ -1085,27 +1123,7 @@ __global__ __launch_bounds__(kWfBlock, 2) void k_wf_tail(TraceArgs a, WfArgs w, 
                     cube_event(has, ev, a.chBase + ch, a, lane);
                     continue;
                 }
-                // the wave's event list, splatted 64 at a time
-                const uint64_t m = __ballot(has);
-                if (has) {
-                    TailEvent &t = evl[pending + __popcll(m & ((1ull << lane) - 1ull))];
-                    t.center = ev.center;
-                    t.ch = a.chBase + ch;
-                    t.s = ev.s;
-                    t.are = ev.A.re;
-                    t.aim = ev.A.im;
-                    t.delta = ev.delta;
-                }
-                pending += __popcll(m);
-                if (pending >= 64) {
-                    pending -= 64;
-                    const TailEvent t = evl[pending + lane];
-                    const Event e{t.center, t.s, {t.are, t.aim}, t.delta};
-                    if (guard)
-                        flush_events_g(true, e, t.ch, tile, a.N, a.C, lane);
-                    else
-                        flush_events(true, e, t.ch, tile, a.N, a.C, lane);
-                }
+                list_event(has, ev, a.chBase + ch, list, pending, tile, a.N, a.C, guard, lane);
             }
         }
         // the path's end (its guiding record) or its next ray
@@ -1130,14 +1148,7 @@ __global__ __launch_bounds__(kWfBlock, 2) void k_wf_tail(TraceArgs a, WfArgs w, 
         wave_count(&a.counters[4], cev, lane);
     }
     if (!tiled) return;
-    if (lane < pending) {  // the rest of the wave's list
-        const TailEvent t = evl[lane];
-        const Event e{t.center, t.s, {t.are, t.aim}, t.delta};
-        if (guard)
-            flush_events_g(true, e, t.ch, tile, a.N, a.C, lane);
-        else
-            flush_events(true, e, t.ch, tile, a.N, a.C, lane);
-    }
+    list_flush(list, pending, tile, a.N, a.C, guard, lane);
     __syncthreads();
     const int NC = a.N * a.C;
     if (guard)
@@ -1381,9 +1392,11 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.statL = (int32_t *)take(sizeof(int32_t) * P);
 }
 
-size_t gather_lds_bytes(int N, int C) {
+size_t tile_lds_bytes(int N, int C) {
     return (size_t)(N >= 2 * kGuard ? N + 2 * kGuard : N) * C * 2 * sizeof(double);
 }
+
+size_t gather_lds_bytes(int N, int C) { return tile_lds_bytes(N, C); }
 
 std::vector<int32_t> coherence_cells() {
     // cell (u, v): the first 3 base-5 digits and the first 5 base-3 digits of the primary
@@ -1472,8 +1485,7 @@ int wavefront_trav_blocks_per_cu(bool count) {
 }
 
 size_t tail_lds_bytes(int N, int C, bool noTile) {
-    return (noTile ? 0 : gather_lds_bytes(N, C)) + sizeof(int32_t) * kStackShort * kWfBlock +
-           sizeof(TailEvent) * kTailEvents * (kWfBlock / 64);
+    return (noTile ? 0 : tile_lds_bytes(N, C)) + sizeof(int32_t) * kStackShort * kWfBlock + kWaveEventBytes;
 }
 
 hipError_t prepare_wavefront(int N, int C, bool cube, bool noTile) {
