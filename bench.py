@@ -455,6 +455,8 @@ def urban_bench(args):
     pt.configureFrame(fc)
     b = H.HipBackend(mesh, pt)
     b.set_velocities(vel)
+    if args.tail >= 0:
+        b.set_option(OPT_TAIL, args.tail)
     run_step(b, pt, sc, sched)  # warm-up, and the trained tree of the accuracy sample
     child, p0 = pt.guiding.sampling()
     sample = 1 << 13

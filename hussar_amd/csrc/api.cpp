@@ -27,6 +27,8 @@
 static constexpr int kGatherPerCU = HSR_GATHER_PER_CU;
 // cube frames: at most this many doubles of partial cubes for the tile splat (1 GiB), and
 // the partial cubes a serving slot reserves (frames' worth)
+// the fused tail starts at the first depth with at most this many extension rays (adaptive)
+static constexpr int64_t kTailPaths = int64_t(1) << 17;
 static constexpr int64_t kCubePartDoubles = int64_t(1) << 27;
 namespace hsr {
 constexpr int64_t kCubeReserveParts = 4;
@@ -112,6 +114,7 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     HSR_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     HSR_HIP(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
     for (auto &e : b->ev) HSR_HIP(hipEventCreate(&e));
+    HSR_HIP(hipHostMalloc((void **)&b->h_ext, hsr_backend::kTrackDepths * sizeof(uint32_t), hipHostMallocDefault));
 
     hsr::FlatBVH bvh;
     const auto t0 = std::chrono::steady_clock::now();
@@ -170,6 +173,7 @@ void hsr_hip_destroy(hsr_backend *b) {
     if (b->d_wf) (void)hipFree(b->d_wf);
     for (auto &g : b->graphs) hsr::destroy_graph(g.g);
     b->graphs.clear();
+    if (b->h_ext) (void)hipHostFree(b->h_ext);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : b->evTrav) (void)hipEventDestroy(e);
@@ -393,7 +397,7 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
             return HSR_OK;
         case HSR_OPT_CUBE_EVENTS:
             if (value < 0 || value > (int64_t)hsr::kCubeEvents)
-                return fail(HSR_ERR_INVALID, "cube event list capacity must lie in [0, 2^22]");
+                return fail(HSR_ERR_INVALID, "cube event list capacity must lie in [0, 2^24]");
             b->optCubeEvents = value;
             return HSR_OK;
         case HSR_OPT_TRAV_WAVES:
@@ -406,7 +410,7 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
             b->optPhaseMode = (int32_t)value;
             return HSR_OK;
         case HSR_OPT_TAIL:
-            if (value < 0 || value > 64) return fail(HSR_ERR_INVALID, "tail depth must lie in [0, 64]");
+            if (value < -1 || value > 64) return fail(HSR_ERR_INVALID, "tail depth must lie in [-1, 64]");
             b->optTail = (int32_t)value;
             return HSR_OK;
         case HSR_OPT_GRAPH:
@@ -635,7 +639,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         if (s != HSR_OK) return s;
         a.events = b->d_events;
         a.evCount = b->d_evCount;
-        a.evCap = std::min(b->evCap, evWant);
+        a.evCap = std::min(b->evCap, evWant) / (unsigned)std::min(scene->rxCount, C);  // per receiver
     }
 
     // grid: as many workgroups as fit at once, fewer for small budgets (>= 8 paths per lane)
@@ -759,13 +763,23 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             b->evFork.push_back(f);
             b->evJoin.push_back(j);
         }
+        // the fused tail from depth K on, when the batch has bounces past K and its LDS fits: K =
+        // optTail, or (-1) the first depth k >= 2 whose extension rays in this backend's last
+        // batch, scaled to this batch's paths, were at most kTailPaths (depth 2 at the first)
+        int tailDepth = b->optTail;
+        if (tailDepth < 0) {
+            tailDepth = 2;
+            if (b->lastBatchPaths > 0)
+                while (tailDepth < knobs->maxDepth && tailDepth < b->lastBatchDepths &&
+                       (double)b->lastExt[tailDepth] * (double)n / (double)b->lastBatchPaths > (double)kTailPaths)
+                    ++tailDepth;
+        }
+        if (tailDepth < 1 || tailDepth >= knobs->maxDepth ||
+            hsr::tail_lds_bytes(N, C, cube || noSplat) > (size_t)b->ldsPerBlock)
+            tailDepth = 0;
+        ++b->stats.tail_batches[std::min(tailDepth, 7)];
         // the batch's launch sequence: replayed from the graph of its launch shape with the
         // batch's arguments, or stage by stage
-        // the fused tail from depth optTail on, when the batch has bounces past it and its LDS fits
-        const int tailDepth = b->optTail >= 1 && b->optTail < knobs->maxDepth &&
-                                      hsr::tail_lds_bytes(N, C, cube || noSplat) <= (size_t)b->ldsPerBlock
-                                  ? b->optTail
-                                  : 0;
         HSR_HIP(hsr::prepare_wavefront(N, C, cube, cube || noSplat));
         hsr::BatchArgs host;
         host.a = a;
@@ -795,9 +809,15 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             nClosest = p.nClosest;
             nShadow = p.nShadow;
         }
+        // the batch's queue sizes to the host (read once the batch is over): the extension rays
+        // of each depth
+        const int depths = std::min(knobs->maxDepth + 1, hsr_backend::kTrackDepths);
+        HSR_HIP(hipMemcpy2DAsync(b->h_ext, sizeof(uint32_t), &w.cnt[0].n_ext.v, sizeof(hsr::WfCounts),
+                                 sizeof(uint32_t), depths, hipMemcpyDeviceToHost, b->stream));
         if (a.events && !noSplat)
-            HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, chirps, N, C, cubeMt, cubeG,
-                                           (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));
+            HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, a.chBase, std::min(scene->rxCount, C), chirps,
+                                           N, C, cubeMt, cubeG, (double2 *)b->d_cubeParts, b->d_frame_sum,
+                                           b->stream));
         if (training)
             HSR_HIP(hsr::launch_stat_reduce(w.statV, w.statL, n, guide->host.training.size(), b->d_statRows,
                                             hsr::kStatRows, guide->d_tStat, b->stream));
@@ -817,6 +837,9 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
         // the traversal events all precede ev[1] on the stream, and so the statistics' copy
         HSR_HIP(event_wait(statsFirst ? guide->evDl : b->ev[3]));
+        for (int k = 0; k < depths; ++k) b->lastExt[k] = b->h_ext[k];
+        b->lastBatchDepths = depths;
+        b->lastBatchPaths = n;
         float ms1 = 0.0f;
         HSR_HIP(hipEventElapsedTime(&ms1, b->ev[0], b->ev[1]));
         trace_ms += ms1;

@@ -109,7 +109,14 @@ struct hsr_backend {
     int64_t optPathsPerLaunch = int64_t(1) << 24;
     int32_t optPhaseMode = 0;
     int32_t optChain = 0;  // HSR_OPT_CHAIN
-    int32_t optTail = 2;   // HSR_OPT_TAIL: first depth of the fused tail, 0 = none
+    int32_t optTail = -1;  // HSR_OPT_TAIL: first depth of the fused tail, 0 = none, -1 = adaptive
+    // the extension rays of each depth of the last wavefront batch (h_ext: pinned landing area
+    // of their copy), for the adaptive tail depth
+    static constexpr int kTrackDepths = 64;
+    uint32_t *h_ext = nullptr;
+    int64_t lastExt[kTrackDepths] = {};
+    int lastBatchDepths = 0;
+    int64_t lastBatchPaths = 0;
     int32_t optPipeline = 1;
     int32_t optTravWaves = 0;  // resident waves per CU of the traversal kernels, 0 = all that fit
     int64_t optCubeEvents = -1;  // cube event list capacity (HSR_OPT_CUBE_EVENTS), -1 = the default

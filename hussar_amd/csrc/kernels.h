@@ -25,7 +25,7 @@ struct CubeEvent {
     int32_t flags;   // bit 0 delta, bit 1 deltaC, bits 2.. frame channel
     int32_t pad;
 };
-constexpr unsigned kCubeEvents = 1u << 22;  // capacity of a backend's event list (128 MiB)
+constexpr unsigned kCubeEvents = 1u << 24;  // capacity of a backend's event lists (512 MiB)
 
 // Copies of the guiding tree's training statistics: records of one leaf from every workgroup
 // would otherwise queue on one address (device atomics on one address are serialised); the
@@ -91,8 +91,9 @@ struct TraceArgs {
     const float4 *vel;  // per triangle slot velocity (xyz) or null
     double *frameAcc;   // cube: the frame sum [M][N][C] complex, accumulated with atomics
     int32_t cube;       // 1: splat into frameAcc (M > 1, or one chirp too large for an LDS tile)
-    // cube: the event list for k_cube_splat (null: every event splats with device atomics);
-    // events past evCap are splatted with device atomics at once
+    // cube: the event lists for k_cube_splat, one per receiver r of the launch (frame channel
+    // chBase + r) at events + r * evCap with its count at evCount[r] (null: every event splats
+    // with device atomics); events past a list's evCap are splatted with device atomics at once
     CubeEvent *events;
     unsigned *evCount;
     unsigned evCap;
@@ -246,11 +247,12 @@ hipError_t launch_zero2(double *p, double *q, int64_t n, hipStream_t stream);  /
 hipError_t launch_stat_fold(double *stat, int64_t stride, int64_t n, hipStream_t stream);
 // Cube frames: chirps per LDS tile of one channel (0: a chirp of N samples does not fit).
 int cube_tile_chirps(int M, int N, size_t ldsBytes);
-// Adds the listed events (count on the device) into the frame cube [M][N][C] through LDS
-// tiles of Mt chirps of one channel, G workgroups per tile and G partial cubes in parts
-// (G * M * N * C complex), then empties the list.
-hipError_t launch_cube_splat(const CubeEvent *events, unsigned *count, unsigned cap, int M, int N, int C, int Mt,
-                             int G, double2 *parts, double *frame, hipStream_t stream);
+// Adds the listed events of frame channels [chBase, chBase + lists) (list r at events + r * cap,
+// its count on the device at counts[r]) into the frame cube [M][N][C] through LDS tiles of Mt
+// chirps of one channel, G workgroups per tile and G partial cubes in parts (G * M * N * C
+// complex, the listed channels' cells written), then empties the lists.
+hipError_t launch_cube_splat(const CubeEvent *events, unsigned *counts, unsigned cap, int chBase, int lists, int M,
+                             int N, int C, int Mt, int G, double2 *parts, double *frame, hipStream_t stream);
 size_t trace_lds_bytes(int N, int C);
 
 // ---------------------------------------------------------------- frame post-processing

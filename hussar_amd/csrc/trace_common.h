@@ -1064,9 +1064,9 @@ __device__ inline void flush_events_cube(bool has, const Event &e, int ch, doubl
     }
 }
 
-// A cube event of this lane (if any) into the event list for the LDS tile splat, or, without
-// a list or past its capacity, straight into the frame cube with device atomics. All lanes
-// of the wave call it.
+// A cube event of this lane (if any) of frame channel ch (wave-uniform) into that channel's
+// event list (list ch - chBase) for the LDS tile splat, or, without a list or past its
+// capacity, straight into the frame cube with device atomics. All lanes of the wave call it.
 __device__ inline void cube_event(bool has, const Event &e, int ch, const TraceArgs &a, int lane) {
     bool direct = has;
     if (a.events) {
@@ -1074,7 +1074,7 @@ __device__ inline void cube_event(bool has, const Event &e, int ch, const TraceA
         if (m) {
             const int leader = __builtin_ctzll(m);
             unsigned base = 0;
-            if (lane == leader) base = atomicAdd(a.evCount, (unsigned)__popcll(m));
+            if (lane == leader) base = atomicAdd(a.evCount + (ch - a.chBase), (unsigned)__popcll(m));
             base = __shfl(base, leader, 64);
             const unsigned pos = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
             if (has && pos < a.evCap) {
@@ -1087,7 +1087,7 @@ __device__ inline void cube_event(bool has, const Event &e, int ch, const TraceA
                 c.sc = e.sc;
                 c.flags = (e.delta ? 1 : 0) | (e.deltaC ? 2 : 0) | (ch << 2);
                 c.pad = 0;
-                a.events[pos] = c;
+                a.events[(size_t)(ch - a.chBase) * a.evCap + pos] = c;
                 direct = false;
             }
         }

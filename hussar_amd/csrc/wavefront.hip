@@ -1159,34 +1159,37 @@ __global__ __launch_bounds__(kWfBlock, 2) void k_wf_tail(TraceArgs a, WfArgs w, 
 
 // ------------------------------------------------------------------------ cube splat
 // The listed events of a cube frame into the frame through LDS: workgroup blockIdx / G owns
-// the tile of channel c and chirps [m0, m0 + Mt) (all samples), blockIdx % G = g its share
-// of the list. A wave loads 64 events, keeps those of its channel, and adds each one's taps
-// inside the tile with LDS atomics, spread over its lanes (the taps and their weights are
-// those of flush_events_cube). The tile is then stored into partial cube g, of which every
-// cell is written by exactly one workgroup, and k_cube_reduce adds the G partial cubes into
-// the frame in a fixed order: no device atomics at all.
-__global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__restrict__ ev, const unsigned *count,
-                                                         unsigned cap, int M, int N, int C, int Mt, int G,
-                                                         double2 *__restrict__ parts) {
+// the tile of list r (frame channel c = chBase + r) and chirps [m0, m0 + Mt) (all samples),
+// blockIdx % G = g its share of the list. A wave loads 64 events and adds each one's taps inside the tile with
+// LDS atomics, spread over its lanes (the taps and their weights are those of
+// flush_events_cube). An event spread over both axes has 33 x 33 taps: each lane holds one
+// sample tap k = (lane mod 32) - 16 and its weight 1/(s - k), and the wave covers two chirp
+// rows per instruction (lanes 0-31 row l, lanes 32-63 row l + 1), then the sample tap 16 of all
+// 33 rows; the weight of a tap is 1/(sc - l) times 1/(s - k), as in flush_events_cube. The tile
+// is then stored into partial cube g, of which every cell is written by exactly one workgroup,
+// and k_cube_reduce adds the G partial cubes into the frame in a fixed order: no device atomics
+// at all.
+__global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__restrict__ events,
+                                                         const unsigned *counts, unsigned cap, int chBase, int M,
+                                                         int N, int C, int Mt, int G, double2 *__restrict__ parts) {
     extern __shared__ double t[];  // re [Mt * N], then im [Mt * N]
     const int mTiles = (M + Mt - 1) / Mt;
     const int tile = blockIdx.x / G, part = blockIdx.x % G;
-    const int c = tile / mTiles, m0 = (tile % mTiles) * Mt;
+    const int r = tile / mTiles, c = chBase + r, m0 = (tile % mTiles) * Mt;
     const int mt = min(Mt, M - m0);
     const int TN = Mt * N;
     for (int j = threadIdx.x; j < 2 * TN; j += kWfBlock) t[j] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned n = min(*count, cap);
+    const CubeEvent *ev = events + (size_t)r * cap;  // channel c's list
+    const unsigned n = min(counts[r], cap);
     const unsigned e0 = (unsigned)(((uint64_t)n * part) / G), e1 = (unsigned)(((uint64_t)n * (part + 1)) / G);
+    const int half = lane >> 5, kl = (lane & 31) - 16;  // two-row layout: row of the pair, sample tap
     for (unsigned b = e0 + wave * 64; b < e1; b += kWfBlock) {
         const unsigned i = b + lane;
         CubeEvent e{};
-        bool mine = false;
-        if (i < e1) {
-            e = ev[i];
-            mine = (e.flags >> 2) == c;
-        }
+        const bool mine = i < e1;
+        if (mine) e = ev[i];
         uint64_t m = __ballot(mine);
         while (m) {
             const int src = __builtin_ctzll(m);
@@ -1199,6 +1202,31 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__rest
             const float sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.sc), src));
             const int flags = __builtin_amdgcn_readlane(e.flags, src);
             const bool delta = flags & 1, deltaC = flags & 2;
+            if (!delta && !deltaC) {
+                const float vk = __builtin_amdgcn_rcpf(s - (float)kl);
+                const int bk = wrap_bin(center + kl, N);
+                for (int it = 0; it < 17; ++it) {
+                    const int l = 2 * it + half - 16;  // rows -16 .. 17; row 17 is none
+                    const int rel = wrap_bin(cc + l, M) - m0;
+                    if (l > 16 || rel < 0 || rel >= mt) continue;
+                    const float coef = __builtin_amdgcn_rcpf(sc - (float)l) * vk;
+                    const int j = rel * N + bk;
+                    atomicAdd(&t[j], (double)(Are * coef));
+                    atomicAdd(&t[TN + j], (double)(Aim * coef));
+                }
+                if (lane < 33) {  // sample tap 16 of row lane - 16
+                    const int l = lane - 16;
+                    const int rel = wrap_bin(cc + l, M) - m0;
+                    if (rel >= 0 && rel < mt) {
+                        const float coef = __builtin_amdgcn_rcpf(sc - (float)l) * __builtin_amdgcn_rcpf(s - 16.0f);
+                        const int j = rel * N + wrap_bin(center + 16, N);
+                        atomicAdd(&t[j], (double)(Are * coef));
+                        atomicAdd(&t[TN + j], (double)(Aim * coef));
+                    }
+                }
+                continue;
+            }
+            // one axis or none spread (a fractional offset within 1e-4 of a bin): at most 33 taps
             const int ns = delta ? 1 : 33, nc = deltaC ? 1 : 33;
             for (int tt = lane; tt < ns * nc; tt += 64) {
                 const int l = deltaC ? 0 : tt / ns - 16;
@@ -1220,15 +1248,18 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__rest
         dst[((int64_t)(m0 + j / N) * N + j % N) * C + c] = make_double2(t[j], t[TN + j]);
 }
 
-// frame += the G partial cubes (in order), and the event list emptied
+// frame += the G partial cubes (in order) on the cells of channels [chBase, chBase + lists) of
+// the cube [M][N][C] (cells: M * N * lists of them), and their event lists emptied
 __global__ void k_cube_reduce(double2 *__restrict__ frame, const double2 *__restrict__ parts, int G, int64_t cells,
-                              unsigned *count) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *count = 0u;
-    if (i >= cells) return;
+                              unsigned *counts, int C, int chBase, int lists) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < lists) counts[q] = 0u;
+    if (q >= cells) return;
+    const int64_t i = (q / lists) * C + chBase + q % lists;  // the cell in the cube
+    const int64_t stride = cells / lists * C;                // cells of one partial cube
     double2 s = make_double2(0.0, 0.0);
     for (int g = 0; g < G; ++g) {
-        const double2 v = parts[(int64_t)g * cells + i];
+        const double2 v = parts[(int64_t)g * stride + i];
         s.x += v.x;
         s.y += v.y;
     }
@@ -1437,21 +1468,21 @@ int cube_tile_chirps(int M, int N, size_t ldsBytes) {
     return (int)std::min<int64_t>(M, rows);
 }
 
-hipError_t launch_cube_splat(const CubeEvent *events, unsigned *count, unsigned cap, int M, int N, int C, int Mt,
-                             int G, double2 *parts, double *frame, hipStream_t s) {
+hipError_t launch_cube_splat(const CubeEvent *events, unsigned *counts, unsigned cap, int chBase, int lists, int M,
+                             int N, int C, int Mt, int G, double2 *parts, double *frame, hipStream_t s) {
     (void)hipGetLastError();
-    const int tiles = C * ((M + Mt - 1) / Mt);
+    const int tiles = lists * ((M + Mt - 1) / Mt);
     const size_t lds = (size_t)2 * Mt * N * sizeof(double);
     if (lds > 65536) {
         const hipError_t e =
             hipFuncSetAttribute((const void *)k_cube_splat, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_cube_splat, dim3(tiles * G), dim3(kWfBlock), lds, s, events, count, cap, M, N, C, Mt, G,
-                       parts);
-    const int64_t cells = (int64_t)M * N * C;
+    hipLaunchKernelGGL(k_cube_splat, dim3(tiles * G), dim3(kWfBlock), lds, s, events, counts, cap, chBase, M, N, C, Mt,
+                       G, parts);
+    const int64_t cells = (int64_t)M * N * lists;
     hipLaunchKernelGGL(k_cube_reduce, dim3((unsigned)((cells + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s,
-                       (double2 *)frame, (const double2 *)parts, G, cells, count);
+                       (double2 *)frame, (const double2 *)parts, G, cells, counts, C, chBase, lists);
     return hipGetLastError();
 }
 

@@ -251,7 +251,7 @@ class HipBackend:
     def stats(self) -> dict:
         s = c_run_stats()
         check(lib().hsr_hip_last_stats(self._h, C.byref(s)))
-        return {f: getattr(s, f) for f, _ in s._fields_}
+        return {f: (list(v) if isinstance(v, C.Array) else v) for f, v in ((f, getattr(s, f)) for f, _ in s._fields_)}
 
     def hit_record(self, paths: int, maxDepth: int, rxCount: int):
         hits = np.empty((paths, maxDepth), np.int32)

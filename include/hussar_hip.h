@@ -273,6 +273,8 @@ typedef struct hsr_run_stats {
     /* the fused tail's share of nodes_visited / tris_tested (HSR_OPT_TAIL, counters enabled) */
     int64_t nodes_tail;
     int64_t tris_tail;
+    /* wavefront batches by the depth their fused tail started at (0: none; 7: 7 or deeper) */
+    int64_t tail_batches[8];
 } hsr_run_stats;
 
 hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
@@ -305,9 +307,11 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        receivers). */
 #define HSR_OPT_BATCH_PATHS 7
 /*  HSR_OPT_CUBE_EVENTS   wavefront pipeline, frames with several chirps (or one chirp above
- *                        the LDS frame tile): capacity of the event list whose events are
- *                        accumulated through LDS tiles (default 2^22 = 128 MiB; 0 = none). Events
- *                        beyond it are added to the frame directly with device atomics. */
+ *                        the LDS frame tile): capacity of the event lists whose events are
+ *                        accumulated through LDS tiles, split evenly into one list per receiver
+ *                        of the transmitter being traced (default 2^24 = 512 MiB in all; 0 =
+ *                        none). Events beyond their receiver's list are added to the frame
+ *                        directly with device atomics. */
 #define HSR_OPT_CUBE_EVENTS 8
 /*  HSR_OPT_CHAIN         geometry chain of the rays (wavefront pipeline): 0 = FP64 (default:
  *                        sampled and reflected directions and hit points in double, each
@@ -333,9 +337,13 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
 #define HSR_OPT_GRAPH 10
 /*  HSR_OPT_TAIL          wavefront pipeline: K >= 1 = the paths still going after the depth-K
  *                        extension rays are finished by one fused kernel (hit update, NEE with
- *                        per-ray shadow traversal, splat and further bounces per lane) instead
- *                        of a stage launch per depth, when maxDepth > K (default 2); 0 = every
- *                        depth has its stage launches. Results are those of the stages. */
+ *                        the shadow stage's receiver packets, splat and further bounces per
+ *                        lane) instead of a stage launch per depth, when maxDepth > K; -1 =
+ *                        adaptive (default): K is the first depth >= 2 whose extension rays in
+ *                        the backend's last batch, scaled to the batch's paths, were at most
+ *                        2^17 (2 at the first batch); 0 = every depth has its stage launches.
+ *                        Results are those of the stages; hsr_run_stats counts the batches by
+ *                        their tail depth. */
 #define HSR_OPT_TAIL 11
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 

@@ -154,4 +154,4 @@ def test_cube_event_list_overflow(clutter):
     assert rel_l2(frames[2], frames[0]) < 1e-9
     b = H.HipBackend(mesh)
     with pytest.raises(HsrError, match="INVALID"):
-        b.set_option(OPT_CUBE_EVENTS, (1 << 22) + 1)
+        b.set_option(OPT_CUBE_EVENTS, (1 << 24) + 1)

@@ -52,14 +52,14 @@ def test_tail_equals_stage_launches(vehicle, chain):
     kn = H.Knobs(maxDepth=5, doGuiding=True)
     n = rec = 1 << 18
     z0, tw0, r0, s0 = run(b, sc, kn, n, 0, rec, chain, guide)
-    assert s0["nodes_tail"] == 0
+    assert s0["nodes_tail"] == 0 and s0["tail_batches"][0] == 1
     print(f"paths with hits at depths 0..4: {(r0[0] >= 0).sum(0).tolist()}", flush=True)
     assert (r0[0][:, 3] >= 0).sum() > 0  # some paths bounce a fourth time
     for tail in (1, 2, 3, 4):
         z, tw, r, st = run(b, sc, kn, n, tail, rec, chain, guide)
         print(f"tail from depth {tail} (chain {chain}): tail visits {st['nodes_tail']} of {st['nodes_visited']}",
               flush=True)
-        assert st["nodes_tail"] > 0
+        assert st["nodes_tail"] > 0 and st["tail_batches"][tail] == 1
         assert tw == tw0
         np.testing.assert_array_equal(r[0], r0[0])
         np.testing.assert_array_equal(r[1], r0[1])
@@ -102,3 +102,23 @@ def test_tail_mimo_cube_with_motion():
     np.testing.assert_array_equal(r[0], r0[0])
     np.testing.assert_array_equal(r[1], r0[1])
     assert rel_l2(z, z0) < 1e-12
+
+
+def test_adaptive_tail_depth(vehicle):
+    """The default tail depth (-1): 2 at a backend's first batch, then the first depth >= 2 whose
+    extension rays in the last batch, scaled to this one, were at most 2^17; a larger batch of
+    the same scene moves the tail deeper, and the results are those of the stages."""
+    b, sc, guide = vehicle
+    kn = H.Knobs(maxDepth=5, doGuiding=True)
+    n = 1 << 18
+    z0, tw0, r0, _ = run(b, sc, kn, n, 0, 4096, 0, guide)
+    depths = []
+    for rep in range(2):
+        z, tw, r, st = run(b, sc, kn, n, -1, 4096, 0, guide)
+        depths.append(st["tail_batches"].index(1))
+        np.testing.assert_array_equal(r[0], r0[0])
+        np.testing.assert_array_equal(r[1], r0[1])
+        assert rel_l2(z, z0) < 1e-12
+    print(f"adaptive tail depths: {depths}", flush=True)
+    assert 2 <= depths[1] < 5
+
