@@ -622,24 +622,27 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.vel = mesh->d_vel;
     a.frameAcc = cube ? b->d_frame_sum : nullptr;
     a.cube = cube;
-    // cube: events go through LDS tiles of cubeMt chirps of one channel when a chirp fits,
-    // cubeG workgroups (and partial cubes) per tile, at most kCubePartDoubles of partials
-    const int cubeMt = cube ? hsr::cube_tile_chirps(chirps, N, (size_t)b->ldsPerBlock) : 0;
-    const int cubeTiles = cubeMt > 0 ? C * ((chirps + cubeMt - 1) / cubeMt) : 1;
+    // cube: events go through the LDS tiles of cubePlan when one fits, cubeG workgroups (and
+    // partial cubes) per tile, at most kCubePartDoubles of partials
+    const int lists = std::min(scene->rxCount, C);
+    const hsr::CubeSplat cubePlan =
+        cube ? hsr::cube_splat_plan(chirps, N, (size_t)b->ldsPerBlock, (size_t)b->ldsPerCU) : hsr::CubeSplat{};
+    const int cubeTiles = cubePlan.lds > 0 ? lists * cubePlan.tilesPerList : 1;
     const int64_t cubeCells = (int64_t)chirps * N * C;
+    // doubles of one partial cube and its share of the guard rows
+    const int64_t perPart = 2 * cubeCells + hsr::cube_guard_doubles(cubePlan, chirps, lists, 1);
     int cubeG = (int)std::min<int64_t>(std::max(1, (2 * b->cu + cubeTiles - 1) / cubeTiles),
-                                       kCubePartDoubles / (2 * cubeCells));
+                                       kCubePartDoubles / perPart);
     const unsigned evWant = b->optCubeEvents < 0 ? (b->evCap ? b->evCap : hsr::kCubeEvents)
                                                  : (unsigned)b->optCubeEvents;
-    if (cubeMt > 0 && cubeG >= 1 && b->optPipeline == 1 && evWant > 0) {
+    if (cubePlan.lds > 0 && cubeG >= 1 && b->optPipeline == 1 && evWant > 0) {
         // a slot that reserved less room for partial cubes uses fewer of them
-        if (b->cubePartsCap >= 2 * cubeCells)
-            cubeG = (int)std::min<int64_t>(cubeG, b->cubePartsCap / (2 * cubeCells));
-        hsr_status s = hsr::backend_events(b, (int64_t)cubeG * 2 * cubeCells, evWant);
+        if (b->cubePartsCap >= perPart) cubeG = (int)std::min<int64_t>(cubeG, b->cubePartsCap / perPart);
+        hsr_status s = hsr::backend_events(b, (int64_t)cubeG * perPart, evWant);
         if (s != HSR_OK) return s;
         a.events = b->d_events;
         a.evCount = b->d_evCount;
-        a.evCap = std::min(b->evCap, evWant) / (unsigned)std::min(scene->rxCount, C);  // per receiver
+        a.evCap = std::min(b->evCap, evWant) / (unsigned)lists;  // per receiver
     }
 
     // grid: as many workgroups as fit at once, fewer for small budgets (>= 8 paths per lane)
@@ -815,9 +818,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         HSR_HIP(hipMemcpy2DAsync(b->h_ext, sizeof(uint32_t), &w.cnt[0].n_ext.v, sizeof(hsr::WfCounts),
                                  sizeof(uint32_t), depths, hipMemcpyDeviceToHost, b->stream));
         if (a.events && !noSplat)
-            HSR_HIP(hsr::launch_cube_splat(a.events, a.evCount, a.evCap, a.chBase, std::min(scene->rxCount, C), chirps,
-                                           N, C, cubeMt, cubeG, (double2 *)b->d_cubeParts, b->d_frame_sum,
-                                           b->stream));
+            HSR_HIP(hsr::launch_cube_splat(cubePlan, a.events, a.evCount, a.evCap, a.chBase, lists, chirps, N, C, cubeG,
+                                           (double2 *)b->d_cubeParts, b->d_frame_sum, b->stream));
         if (training)
             HSR_HIP(hsr::launch_stat_reduce(w.statV, w.statL, n, guide->host.training.size(), b->d_statRows,
                                             hsr::kStatRows, guide->d_tStat, b->stream));

@@ -245,14 +245,29 @@ hipError_t launch_zero(double *p, int64_t n, hipStream_t stream);
 hipError_t launch_zero2(double *p, double *q, int64_t n, hipStream_t stream);  // both arrays
 // Folds the kStatCopies copies of n training statistics (stride apart) into the first.
 hipError_t launch_stat_fold(double *stat, int64_t stride, int64_t n, hipStream_t stream);
-// Cube frames: chirps per LDS tile of one channel (0: a chirp of N samples does not fit).
-int cube_tile_chirps(int M, int N, size_t ldsBytes);
+// The LDS tiles of a cube frame's splat, two per CU where they fit. Sample-tiled (samples): a
+// tile holds every chirp of a slab of about N / T samples and 16 guard bins on each side, and
+// splats the events centred in its slab, each event in one tile; used when slabs at least 16
+// samples wide fit (M small enough). Chirp-tiled otherwise: a tile holds Mt chirps of every
+// sample, and each event is splatted by every tile its chirp taps reach. lds = 0: no tile fits.
+struct CubeSplat {
+    bool samples = false;
+    int T = 0;             // sample slabs per channel (sample-tiled)
+    int Mt = 0;            // chirps per tile (chirp-tiled)
+    int tilesPerList = 0;  // tiles per channel
+    size_t lds = 0;        // LDS bytes of a tile
+};
+CubeSplat cube_splat_plan(int M, int N, size_t ldsPerBlock, size_t ldsPerCU);
+// Doubles of the guard rows a sample-tiled splat of `lists` channels and G partial cubes holds
+// after its G partial cubes (0 when chirp-tiled).
+int64_t cube_guard_doubles(const CubeSplat &plan, int M, int lists, int G);
 // Adds the listed events of frame channels [chBase, chBase + lists) (list r at events + r * cap,
-// its count on the device at counts[r]) into the frame cube [M][N][C] through LDS tiles of Mt
-// chirps of one channel, G workgroups per tile and G partial cubes in parts (G * M * N * C
-// complex, the listed channels' cells written), then empties the lists.
-hipError_t launch_cube_splat(const CubeEvent *events, unsigned *counts, unsigned cap, int chBase, int lists, int M,
-                             int N, int C, int Mt, int G, double2 *parts, double *frame, hipStream_t stream);
+// its count on the device at counts[r]) into the frame cube [M][N][C] through the LDS tiles of
+// plan, G workgroups per tile and G partial cubes in parts (G * M * N * C complex, the listed
+// channels' cells written, then the guard rows), then empties the lists.
+hipError_t launch_cube_splat(const CubeSplat &plan, const CubeEvent *events, unsigned *counts, unsigned cap,
+                             int chBase, int lists, int M, int N, int C, int G, double2 *parts, double *frame,
+                             hipStream_t stream);
 size_t trace_lds_bytes(int N, int C);
 
 // ---------------------------------------------------------------- frame post-processing

@@ -1206,8 +1206,11 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__rest
                 const float vk = __builtin_amdgcn_rcpf(s - (float)kl);
                 const int bk = wrap_bin(center + kl, N);
                 for (int it = 0; it < 17; ++it) {
+                    // a pair of rows with neither in this tile is skipped by the whole wave
+                    const int r0 = wrap_bin(cc + 2 * it - 16, M) - m0, r1 = wrap_bin(cc + 2 * it - 15, M) - m0;
+                    if ((r0 < 0 || r0 >= mt) && (r1 < 0 || r1 >= mt || it == 16)) continue;
                     const int l = 2 * it + half - 16;  // rows -16 .. 17; row 17 is none
-                    const int rel = wrap_bin(cc + l, M) - m0;
+                    const int rel = half ? r1 : r0;
                     if (l > 16 || rel < 0 || rel >= mt) continue;
                     const float coef = __builtin_amdgcn_rcpf(sc - (float)l) * vk;
                     const int j = rel * N + bk;
@@ -1260,6 +1263,147 @@ __global__ void k_cube_reduce(double2 *__restrict__ frame, const double2 *__rest
     double2 s = make_double2(0.0, 0.0);
     for (int g = 0; g < G; ++g) {
         const double2 v = parts[(int64_t)g * stride + i];
+        s.x += v.x;
+        s.y += v.y;
+    }
+    double2 f = frame[i];
+    f.x += s.x;
+    f.y += s.y;
+    frame[i] = f;
+}
+
+// The sample-tiled splat. Workgroup blockIdx / G owns the tile of list r (frame channel
+// c = chBase + r) and sample slab j: every chirp of the samples [s0, s1) plus kGuard guard bins
+// on each side; blockIdx % G = g its share of the list. It takes the events centred in its
+// slab, so each event is splatted by one tile, all of its 33 x 33 taps inside it (the chirp
+// axis wraps within the tile, the sample axis runs into the guard bins), in the two-row layout
+// of k_cube_splat. The slab's cells go to partial cube g (each cell written by one
+// workgroup), the guard bins to their own partial rows, which k_cube_reduce_s adds to the
+// neighbouring slabs' cells (circularly) in a fixed order.
+__device__ inline int slab_lo(int j, int N, int T) { return (int)(((int64_t)j * N) / T); }
+
+__global__ __launch_bounds__(kWfBlock) void k_cube_splat_s(const CubeEvent *__restrict__ events,
+                                                           const unsigned *counts, unsigned cap, int chBase, int lists,
+                                                           int M, int N, int C, int T, int G,
+                                                           double2 *__restrict__ parts, double2 *__restrict__ guards) {
+    extern __shared__ double t[];  // re [M][W], then im [M][W]
+    const int tile = blockIdx.x / G, part = blockIdx.x % G;
+    const int r = tile / T, j = tile % T, c = chBase + r;
+    const int s0 = slab_lo(j, N, T), s1 = slab_lo(j + 1, N, T), ns = s1 - s0, W = ns + 2 * kGuard;
+    const int TW = M * W;
+    for (int q = threadIdx.x; q < 2 * TW; q += kWfBlock) t[q] = 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const CubeEvent *ev = events + (size_t)r * cap;
+    const unsigned n = min(counts[r], cap);
+    const unsigned e0 = (unsigned)(((uint64_t)n * part) / G), e1 = (unsigned)(((uint64_t)n * (part + 1)) / G);
+    const int half = lane >> 5, kl = (lane & 31) - 16;
+    for (unsigned b = e0 + wave * 64; b < e1; b += kWfBlock) {
+        const unsigned i = b + lane;
+        CubeEvent e{};
+        bool mine = false;
+        if (i < e1) {
+            e = ev[i];
+            const int c0 = e.center >= N ? e.center - N : e.center;  // round(N * frac) reaches N
+            mine = c0 >= s0 && c0 < s1;
+        }
+        uint64_t m = __ballot(mine);
+        while (m) {
+            const int src = __builtin_ctzll(m);
+            m &= m - 1;
+            const int center = __builtin_amdgcn_readlane(e.center, src);
+            const float s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.s), src));
+            const float Are = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.are), src));
+            const float Aim = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.aim), src));
+            const int cc = __builtin_amdgcn_readlane(e.cc, src);
+            const float sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e.sc), src));
+            const int flags = __builtin_amdgcn_readlane(e.flags, src);
+            const bool delta = flags & 1, deltaC = flags & 2;
+            const int gc = (center >= N ? center - N : center) - s0 + kGuard;  // the centre's column
+            if (!delta && !deltaC) {
+                const float vk = __builtin_amdgcn_rcpf(s - (float)kl);
+                const int g = gc + kl;
+                for (int it = 0; it < 17; ++it) {
+                    const int l = 2 * it + half - 16;  // rows -16 .. 17; row 17 is none
+                    if (l > 16) continue;
+                    const float coef = __builtin_amdgcn_rcpf(sc - (float)l) * vk;
+                    const int q = wrap_bin(cc + l, M) * W + g;
+                    atomicAdd(&t[q], (double)(Are * coef));
+                    atomicAdd(&t[TW + q], (double)(Aim * coef));
+                }
+                if (lane < 33) {  // sample tap 16 of row lane - 16
+                    const int l = lane - 16;
+                    const float coef = __builtin_amdgcn_rcpf(sc - (float)l) * __builtin_amdgcn_rcpf(s - 16.0f);
+                    const int q = wrap_bin(cc + l, M) * W + gc + 16;
+                    atomicAdd(&t[q], (double)(Are * coef));
+                    atomicAdd(&t[TW + q], (double)(Aim * coef));
+                }
+                continue;
+            }
+            const int nsamp = delta ? 1 : 33, nc = deltaC ? 1 : 33;
+            for (int tt = lane; tt < nsamp * nc; tt += 64) {
+                const int l = deltaC ? 0 : tt / nsamp - 16;
+                const int k = delta ? 0 : tt % nsamp - 16;
+                float coef = 1.0f;
+                if (!deltaC) coef = __builtin_amdgcn_rcpf(sc - (float)l);
+                if (!delta) coef = coef * __builtin_amdgcn_rcpf(s - (float)k);
+                const int q = wrap_bin(cc + l, M) * W + gc + k;
+                atomicAdd(&t[q], (double)(Are * coef));
+                atomicAdd(&t[TW + q], (double)(Aim * coef));
+            }
+        }
+    }
+    __syncthreads();
+    const int64_t cells = (int64_t)M * N * C;
+    double2 *dst = parts + (int64_t)part * cells;
+    double2 *gl = guards + ((((int64_t)part * lists + r) * T + j) * 2) * M * kGuard;  // [2][M][kGuard]
+    for (int q = threadIdx.x; q < TW; q += kWfBlock) {
+        const int mm = q / W, g = q % W;
+        const double2 v = make_double2(t[q], t[TW + q]);
+        if (g < kGuard)
+            gl[mm * kGuard + g] = v;  // left guard: samples s0 - 16 .. s0 - 1
+        else if (g >= kGuard + ns)
+            gl[(M + mm) * kGuard + (g - kGuard - ns)] = v;  // right guard: samples s1 .. s1 + 15
+        else
+            dst[((int64_t)mm * N + s0 + g - kGuard) * C + c] = v;
+    }
+}
+
+// frame += the G partial cubes of the sample-tiled splat, each with the guard bins its
+// neighbouring slabs hold for its cells (in a fixed order), on the cells of channels
+// [chBase, chBase + lists) (cells: M * N * lists of them); their event lists emptied.
+__global__ void k_cube_reduce_s(double2 *__restrict__ frame, const double2 *__restrict__ parts,
+                                const double2 *__restrict__ guards, int G, int M, int N, int C, int T, int chBase,
+                                int lists, unsigned *counts) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < lists) counts[q] = 0u;
+    const int64_t cells = (int64_t)M * N * lists;
+    if (q >= cells) return;
+    const int r = (int)(q % lists);
+    const int64_t mb = q / lists;
+    const int mm = (int)(mb / N), b = (int)(mb % N);
+    const int64_t i = mb * C + chBase + r;  // the cell in the cube
+    int j = (int)(((int64_t)b * T) / N);
+    while (j + 1 < T && slab_lo(j + 1, N, T) <= b) ++j;
+    while (j > 0 && slab_lo(j, N, T) > b) --j;
+    const int s0 = slab_lo(j, N, T), s1 = slab_lo(j + 1, N, T);
+    const int jl = (j + T - 1) % T, jr = (j + 1) % T;
+    const bool fromLeft = b - s0 < kGuard, fromRight = s1 - b <= kGuard;
+    const int64_t stride = (int64_t)M * N * C;
+    double2 s = make_double2(0.0, 0.0);
+    for (int g = 0; g < G; ++g) {
+        double2 v = parts[(int64_t)g * stride + i];
+        const int64_t base = ((int64_t)g * lists + r) * T;
+        if (fromLeft) {  // the right guard of the slab before
+            const double2 u = guards[((base + jl) * 2 + 1) * M * kGuard + (int64_t)mm * kGuard + (b - s0)];
+            v.x += u.x;
+            v.y += u.y;
+        }
+        if (fromRight) {  // the left guard of the slab after
+            const double2 u = guards[((base + jr) * 2) * M * kGuard + (int64_t)mm * kGuard + (b - (s1 - kGuard))];
+            v.x += u.x;
+            v.y += u.y;
+        }
         s.x += v.x;
         s.y += v.y;
     }
@@ -1463,26 +1607,57 @@ std::vector<int32_t> coherence_cells() {
     return out;
 }
 
-int cube_tile_chirps(int M, int N, size_t ldsBytes) {
-    const int64_t rows = (int64_t)(ldsBytes / (2 * sizeof(double))) / N;
-    return (int)std::min<int64_t>(M, rows);
+CubeSplat cube_splat_plan(int M, int N, size_t ldsPerBlock, size_t ldsPerCU) {
+    CubeSplat p{};
+    const int64_t col = (int64_t)(2 * sizeof(double)) * M;  // LDS of one sample of every chirp
+    const int64_t cols = (int64_t)(ldsPerCU / 2) / col;      // two tiles per CU
+    if (N >= 2 * kGuard && cols >= 3 * kGuard) {
+        const int64_t T = (N + (cols - 2 * kGuard) - 1) / (cols - 2 * kGuard);
+        if (N / T >= kGuard) {  // every slab as wide as a guard
+            p.samples = true;
+            p.T = (int)T;
+            p.tilesPerList = (int)T;
+            p.lds = (size_t)(col * ((N + T - 1) / T + 2 * kGuard));
+            return p;
+        }
+    }
+    const int64_t row = (int64_t)(2 * sizeof(double)) * N;  // LDS of one chirp of every sample
+    int64_t rows = (int64_t)(ldsPerCU / 2) / row;
+    if (rows < 1) rows = (int64_t)ldsPerBlock / row;
+    if (rows < 1) return p;  // a chirp does not fit
+    const int64_t tiles = (M + rows - 1) / rows;
+    p.Mt = (int)((M + tiles - 1) / tiles);  // tiles of even height
+    p.tilesPerList = (M + p.Mt - 1) / p.Mt;
+    p.lds = (size_t)(row * p.Mt);
+    return p;
 }
 
-hipError_t launch_cube_splat(const CubeEvent *events, unsigned *counts, unsigned cap, int chBase, int lists, int M,
-                             int N, int C, int Mt, int G, double2 *parts, double *frame, hipStream_t s) {
+int64_t cube_guard_doubles(const CubeSplat &p, int M, int lists, int G) {
+    return p.samples ? (int64_t)G * lists * p.T * 2 * M * kGuard * 2 : 0;
+}
+
+hipError_t launch_cube_splat(const CubeSplat &p, const CubeEvent *events, unsigned *counts, unsigned cap, int chBase,
+                             int lists, int M, int N, int C, int G, double2 *parts, double *frame, hipStream_t s) {
     (void)hipGetLastError();
-    const int tiles = lists * ((M + Mt - 1) / Mt);
-    const size_t lds = (size_t)2 * Mt * N * sizeof(double);
-    if (lds > 65536) {
-        const hipError_t e =
-            hipFuncSetAttribute((const void *)k_cube_splat, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const void *f = p.samples ? (const void *)k_cube_splat_s : (const void *)k_cube_splat;
+    if (p.lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_cube_splat, dim3(tiles * G), dim3(kWfBlock), lds, s, events, counts, cap, chBase, M, N, C, Mt,
-                       G, parts);
     const int64_t cells = (int64_t)M * N * lists;
-    hipLaunchKernelGGL(k_cube_reduce, dim3((unsigned)((cells + kWfBlock - 1) / kWfBlock)), dim3(kWfBlock), 0, s,
-                       (double2 *)frame, (const double2 *)parts, G, cells, counts, C, chBase, lists);
+    const dim3 grid((unsigned)(lists * p.tilesPerList * G)), rgrid((unsigned)((cells + kWfBlock - 1) / kWfBlock));
+    if (p.samples) {
+        double2 *guards = parts + (int64_t)G * M * N * C;  // after the partial cubes
+        hipLaunchKernelGGL(k_cube_splat_s, grid, dim3(kWfBlock), p.lds, s, events, counts, cap, chBase, lists, M, N,
+                           C, p.T, G, parts, guards);
+        hipLaunchKernelGGL(k_cube_reduce_s, rgrid, dim3(kWfBlock), 0, s, (double2 *)frame, (const double2 *)parts,
+                           (const double2 *)guards, G, M, N, C, p.T, chBase, lists, counts);
+    } else {
+        hipLaunchKernelGGL(k_cube_splat, grid, dim3(kWfBlock), p.lds, s, events, counts, cap, chBase, M, N, C, p.Mt,
+                           G, parts);
+        hipLaunchKernelGGL(k_cube_reduce, rgrid, dim3(kWfBlock), 0, s, (double2 *)frame, (const double2 *)parts, G,
+                           cells, counts, C, chBase, lists);
+    }
     return hipGetLastError();
 }
 
