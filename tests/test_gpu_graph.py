@@ -3,7 +3,7 @@
 A batch's launch sequence (set-up, generation, per depth the traversals, shading and gather
 stages with their stream fork / join and timing events) is built into a HIP graph at the
 second batch of its launch shape (the first one launches stage by stage) and replayed from
-then on, its arguments copied to the device before each replay. Bar, for every replay: hit
+then on, each kernel node given the batch's own arguments before the replay. Bar, for every replay: hit
 ids and shadow outcomes equal those of the
 same batch launched stage by stage, the frame within 1e-12 relative L2 of it (only the order
 of the LDS tile's FP64 additions differs), the same total weight; between replays the
