@@ -40,6 +40,11 @@ __device__ inline d3 posd(const DevAntenna &a) { return {(double)a.pos[0], (doub
 // The LDS column is typed as LDS (address space 3) so that its accesses are ds_read/ds_write
 // and are not merged with the spill's into one flat access.
 typedef __attribute__((address_space(3))) int32_t lds_int32;
+typedef __attribute__((address_space(3))) double lds_double;
+// FP64 add to an LDS word without a return value (ds_add_f64).
+__device__ inline void lds_add(lds_double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 struct Stack {
     lds_int32 *lds;  // column of this thread: lds[i * kTraceBlock]
     int32_t *spill;  // [kStackSpill] private
@@ -56,6 +61,22 @@ struct Stack {
         if (__builtin_expect(sp < kStackShort, 1)) v = lds[sp * kTraceBlock];
         else v = spill[sp - kStackShort];
         return v;
+    }
+    // Pushes the children v1..v(n-1) of a visited node, sorted nearest first (n = 1..4 entered
+    // children, v0 taken next), farthest first, so that v1 ends on top. While three entries fit
+    // in the LDS part it writes sp, sp + 1 and sp + 2 without branches (entries past the new top
+    // are dead) and advances sp by n - 1.
+    __device__ inline void push_sorted(int n, int32_t v1, int32_t v2, int32_t v3) {
+        if (__builtin_expect(sp + 3 <= kStackShort, 1)) {
+            lds[sp * kTraceBlock] = n == 4 ? v3 : (n == 3 ? v2 : v1);
+            lds[(sp + 1) * kTraceBlock] = n == 4 ? v2 : v1;
+            lds[(sp + 2) * kTraceBlock] = v1;
+            sp += n - 1;
+        } else {
+            if (n == 4) push(v3);
+            if (n >= 3) push(v2);
+            if (n >= 2) push(v1);
+        }
     }
 };
 
@@ -140,12 +161,14 @@ __device__ inline uint32_t slab4(const Node4 &q, const SlabRay &r, float tb, flo
 // far planes (no min/max). b carries two roundings, so t is exact to within one rounding of
 // itself plus 2.00001 u |b| (u = 2^-24): the near planes use b - err and the far planes
 // b + err, err = 2^-20 |b| per axis, and the exit keeps the robust factor 1 + 2 gamma_3 (Ize
-// 2013), which makes the test a superset of the exact one. The real boxes origin + q*2^e lie
+// 2013), which makes the test a superset of the exact one. The bound must stay per axis: an
+// axis whose direction component is tiny has a huge |b|, and its error would open every box
+// along the other axes. The real boxes origin + q*2^e lie
 // within one rounding of the decoded boxes the builder checked, and it widened every box by
 // far more. invd is finite (safe_rcp), so no product is NaN.
 struct QuadHit {
     uint32_t m;     // bit i: child i is entered
-    float te[4];    // entry distances
+    float k[4];     // sort keys: the entry distance of an entered child, +inf otherwise
     int32_t c[4];   // child refs
 };
 
@@ -185,8 +208,9 @@ __device__ inline QuadHit slab4q(const float4 *__restrict__ nodes, int32_t n, co
         const float fz = __builtin_fmaf((float)((fzw >> (8 * i)) & 0xFFu), az, bfz);
         const float t0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tnear));
         const float t1 = fminf(fminf(fx, fy), fz) * 1.0000004f;
-        qh.te[i] = t0;
-        if (t0 <= t1 && t0 <= tb && qh.c[i] != kNoChild) qh.m |= 1u << i;
+        const bool hit = t0 <= t1 && t0 <= tb && qh.c[i] != kNoChild;
+        qh.k[i] = hit ? t0 : __builtin_inff();
+        if (hit) qh.m |= 1u << i;
     }
     return qh;
 }
@@ -201,39 +225,31 @@ __device__ inline int32_t child_ref(const int4 &c, int i) {
     return i == 0 ? c.x : (i == 1 ? c.y : (i == 2 ? c.z : c.w));
 }
 
+// Compare-exchange of (key, child) pairs as selects (no branch): the smaller key first.
 __device__ inline void cswap(float &ka, int32_t &va, float &kb, int32_t &vb) {
-    if (kb < ka) {
-        const float tk = ka;
-        ka = kb;
-        kb = tk;
-        const int32_t tv = va;
-        va = vb;
-        vb = tv;
-    }
+    const bool s = kb < ka;
+    const float k0 = s ? kb : ka, k1 = s ? ka : kb;
+    const int32_t v0 = s ? vb : va, v1 = s ? va : vb;
+    ka = k0;
+    kb = k1;
+    va = v0;
+    vb = v1;
 }
 
 // Closest-hit order from a visited inner node with hit mask m != 0: the hit children sorted by
-// entry distance (a five-comparator network over four keys, misses keyed +inf), the nearest
-// returned as the next node, the others pushed farthest first so that they pop nearest first.
+// entry distance (a five-comparator network over four keys, misses keyed +inf; entry distances
+// of entered children are finite), the nearest returned as the next node, the others pushed
+// farthest first so that they pop nearest first. The same instructions run for any number of
+// entered children, so the lanes of a wave do not diverge here.
 __device__ inline int32_t order_children(const QuadHit &q, Stack &st) {
-    const uint32_t m = q.m;
-    if ((m & (m - 1)) == 0) {
-        const int i = __builtin_ctz(m);
-        return i == 0 ? q.c[0] : (i == 1 ? q.c[1] : (i == 2 ? q.c[2] : q.c[3]));
-    }
-    const float inf = __builtin_inff();
-    float k0 = (m & 1u) ? q.te[0] : inf, k1 = (m & 2u) ? q.te[1] : inf;
-    float k2 = (m & 4u) ? q.te[2] : inf, k3 = (m & 8u) ? q.te[3] : inf;
+    float k0 = q.k[0], k1 = q.k[1], k2 = q.k[2], k3 = q.k[3];
     int32_t v0 = q.c[0], v1 = q.c[1], v2 = q.c[2], v3 = q.c[3];
     cswap(k0, v0, k1, v1);
     cswap(k2, v2, k3, v3);
     cswap(k0, v0, k2, v2);
     cswap(k1, v1, k3, v3);
     cswap(k1, v1, k2, v2);
-    const int n = __popc(m);
-    if (n == 4) st.push(v3);
-    if (n >= 3) st.push(v2);
-    st.push(v1);
+    st.push_sorted(__popc(q.m), v1, v2, v3);
     return v0;
 }
 
@@ -986,25 +1002,29 @@ __device__ inline void flush_events_g(bool has, const Event &e, int ch, double *
     const int R = guard_row(N);
     int c0 = e.center;  // in [0, N]: round(N * frac) reaches N
     if (c0 >= N) c0 -= N;
-    double *re = tile + ch * R + kGuard + c0, *im = tile + R * C + ch * R + kGuard + c0;
+    // the event's centre bin in the LDS tile (tap k at re[k], k in [-16, 16] within the row)
+    lds_double *re = (lds_double *)tile + (ch * R + kGuard + c0);
+    const int imOff = R * C;
     if (e.delta) {
-        atomicAdd(re, (double)e.A.re);
-        atomicAdd(im, (double)e.A.im);
+        lds_add(re, (double)e.A.re);
+        lds_add(re + imOff, (double)e.A.im);
         return;
     }
-    int k = ((lane - c0 + 16) & 31) - 16;  // c0 + k = lane (mod 32)
+    // tap of step it: k = the 5-bit two's-complement value of (lane - c0 + it), so that
+    // c0 + k = lane + it (mod 32) and k runs from its first value up to 15, then from -16
+    const int x0 = lane - c0;
 #if HSR_TAP_UNROLL > 0
 #pragma unroll HSR_TAP_UNROLL
 #endif
     for (int it = 0; it < 32; ++it) {
+        const int k = __builtin_amdgcn_sbfe(x0 + it, 0, 5);
         const float coef = __builtin_amdgcn_rcpf(e.s - (float)k);
-        atomicAdd(&re[k], (double)(e.A.re * coef));
-        atomicAdd(&im[k], (double)(e.A.im * coef));
-        k = k == 15 ? -16 : k + 1;
+        lds_add(re + k, (double)(e.A.re * coef));
+        lds_add(re + k + imOff, (double)(e.A.im * coef));
     }
     const float coef = __builtin_amdgcn_rcpf(e.s - 16.0f);
-    atomicAdd(&re[16], (double)(e.A.re * coef));
-    atomicAdd(&im[16], (double)(e.A.im * coef));
+    lds_add(re + 16, (double)(e.A.re * coef));
+    lds_add(re + 16 + imOff, (double)(e.A.im * coef));
 }
 
 // Adds the guard-band tile into the workgroup's partial frame [N][C] (complex, frame order):

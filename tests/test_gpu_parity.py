@@ -14,7 +14,7 @@ import pytest
 
 import hussar_amd as H
 from hussar_amd import scenes
-from hussar_amd._lib import HsrError, OPT_COUNTERS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE, OPT_RECORD_HITS
+from hussar_amd._lib import HsrError, OPT_COUNTERS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE, OPT_RECORD_HITS, OPT_TAIL
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -305,6 +305,31 @@ def test_counters_are_deterministic(dihedral):
         b.set_option(OPT_COUNTERS, 0)
     for k in ("nodes_visited", "tris_tested", "rays_closest", "rays_shadow", "events"):
         assert s1[k] == s2[k] > 0, k
+
+
+def test_traversal_visits_per_ray_stay_small():
+    """Node visits per ray of the closest-hit and packet traversals on the vehicle clutter
+    (deterministic counters, stage launches). A box test that turned too conservative would
+    still return the right hits while opening most of the tree: this bounds the visits, at a few
+    times what the 50k-triangle tree needs (1.1 per closest-hit ray, most of the unguided primary
+    rays missing the clutter, and 12 per shadow packet)."""
+    mesh, _ = scenes.vehicle_mesh(seed=3, triangles=50_000)
+    b = H.HipBackend(mesh)
+    b._configure(H.FrameConfig(1, 256, 4))
+    sc = scenes.radar_array(scenes.dihedral_rf(), channels=4, position=(0.0, 0.75, 0.0))
+    b.set_option(OPT_TAIL, 0)
+    b.set_option(OPT_COUNTERS, 1)
+    try:
+        gpu_run(b, sc, H.Knobs(maxDepth=4, doGuiding=False), 1 << 18)
+        st = b.stats()
+    finally:
+        b.set_option(OPT_COUNTERS, 0)
+        b.set_option(OPT_TAIL, -1)
+    closest = (st["nodes_visited"] - st["nodes_shadow"]) / st["rays_closest"]
+    shadow = st["nodes_shadow"] / (st["rays_shadow"] / 4)
+    print(f"node visits per closest-hit ray {closest:.2f}, per shadow packet {shadow:.2f}", flush=True)
+    assert 1.0 < closest < 5.0
+    assert 1.0 < shadow < 30.0
 
 
 @pytest.mark.parametrize("size_cm", [5, 15])
