@@ -37,6 +37,10 @@ enum : int { I_FLAGS = 0, I_LOCAL, I_N };
 enum : int { S_SETUP = 1, S_TEST = 2, S_GW = 4, S_SPLAT = 8, S_GW1 = 16 };
 
 constexpr int kWfBlock = 256;
+// Threads per workgroup of the cube splats: their LDS tile (up to half a CU's LDS) limits them to
+// two workgroups per CU, so each holds 16 waves to keep the CU's SIMDs busy (they need few
+// registers).
+constexpr int kCubeBlock = 1024;
 #ifndef HSR_TRAV_STEPS
 #define HSR_TRAV_STEPS 256
 #endif
@@ -1169,7 +1173,7 @@ __global__ __launch_bounds__(kWfBlock, 2) void k_wf_tail(TraceArgs a, WfArgs w, 
 // is then stored into partial cube g, of which every cell is written by exactly one workgroup,
 // and k_cube_reduce adds the G partial cubes into the frame in a fixed order: no device atomics
 // at all.
-__global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__restrict__ events,
+__global__ __launch_bounds__(kCubeBlock) void k_cube_splat(const CubeEvent *__restrict__ events,
                                                          const unsigned *counts, unsigned cap, int chBase, int M,
                                                          int N, int C, int Mt, int G, double2 *__restrict__ parts) {
     extern __shared__ double t[];  // re [Mt * N], then im [Mt * N]
@@ -1178,14 +1182,14 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__rest
     const int r = tile / mTiles, c = chBase + r, m0 = (tile % mTiles) * Mt;
     const int mt = min(Mt, M - m0);
     const int TN = Mt * N;
-    for (int j = threadIdx.x; j < 2 * TN; j += kWfBlock) t[j] = 0.0;
+    for (int j = threadIdx.x; j < 2 * TN; j += kCubeBlock) t[j] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const CubeEvent *ev = events + (size_t)r * cap;  // channel c's list
     const unsigned n = min(counts[r], cap);
     const unsigned e0 = (unsigned)(((uint64_t)n * part) / G), e1 = (unsigned)(((uint64_t)n * (part + 1)) / G);
     const int half = lane >> 5, kl = (lane & 31) - 16;  // two-row layout: row of the pair, sample tap
-    for (unsigned b = e0 + wave * 64; b < e1; b += kWfBlock) {
+    for (unsigned b = e0 + wave * 64; b < e1; b += kCubeBlock) {
         const unsigned i = b + lane;
         CubeEvent e{};
         const bool mine = i < e1;
@@ -1247,7 +1251,7 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat(const CubeEvent *__rest
     }
     __syncthreads();
     double2 *dst = parts + (int64_t)part * M * N * C;
-    for (int j = threadIdx.x; j < mt * N; j += kWfBlock)
+    for (int j = threadIdx.x; j < mt * N; j += kCubeBlock)
         dst[((int64_t)(m0 + j / N) * N + j % N) * C + c] = make_double2(t[j], t[TN + j]);
 }
 
@@ -1282,7 +1286,7 @@ __global__ void k_cube_reduce(double2 *__restrict__ frame, const double2 *__rest
 // neighbouring slabs' cells (circularly) in a fixed order.
 __device__ inline int slab_lo(int j, int N, int T) { return (int)(((int64_t)j * N) / T); }
 
-__global__ __launch_bounds__(kWfBlock) void k_cube_splat_s(const CubeEvent *__restrict__ events,
+__global__ __launch_bounds__(kCubeBlock) void k_cube_splat_s(const CubeEvent *__restrict__ events,
                                                            const unsigned *counts, unsigned cap, int chBase, int lists,
                                                            int M, int N, int C, int T, int G,
                                                            double2 *__restrict__ parts, double2 *__restrict__ guards) {
@@ -1291,14 +1295,14 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat_s(const CubeEvent *__re
     const int r = tile / T, j = tile % T, c = chBase + r;
     const int s0 = slab_lo(j, N, T), s1 = slab_lo(j + 1, N, T), ns = s1 - s0, W = ns + 2 * kGuard;
     const int TW = M * W;
-    for (int q = threadIdx.x; q < 2 * TW; q += kWfBlock) t[q] = 0.0;
+    for (int q = threadIdx.x; q < 2 * TW; q += kCubeBlock) t[q] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const CubeEvent *ev = events + (size_t)r * cap;
     const unsigned n = min(counts[r], cap);
     const unsigned e0 = (unsigned)(((uint64_t)n * part) / G), e1 = (unsigned)(((uint64_t)n * (part + 1)) / G);
     const int half = lane >> 5, kl = (lane & 31) - 16;
-    for (unsigned b = e0 + wave * 64; b < e1; b += kWfBlock) {
+    for (unsigned b = e0 + wave * 64; b < e1; b += kCubeBlock) {
         const unsigned i = b + lane;
         CubeEvent e{};
         bool mine = false;
@@ -1357,7 +1361,7 @@ __global__ __launch_bounds__(kWfBlock) void k_cube_splat_s(const CubeEvent *__re
     const int64_t cells = (int64_t)M * N * C;
     double2 *dst = parts + (int64_t)part * cells;
     double2 *gl = guards + ((((int64_t)part * lists + r) * T + j) * 2) * M * kGuard;  // [2][M][kGuard]
-    for (int q = threadIdx.x; q < TW; q += kWfBlock) {
+    for (int q = threadIdx.x; q < TW; q += kCubeBlock) {
         const int mm = q / W, g = q % W;
         const double2 v = make_double2(t[q], t[TW + q]);
         if (g < kGuard)
@@ -1648,12 +1652,12 @@ hipError_t launch_cube_splat(const CubeSplat &p, const CubeEvent *events, unsign
     const dim3 grid((unsigned)(lists * p.tilesPerList * G)), rgrid((unsigned)((cells + kWfBlock - 1) / kWfBlock));
     if (p.samples) {
         double2 *guards = parts + (int64_t)G * M * N * C;  // after the partial cubes
-        hipLaunchKernelGGL(k_cube_splat_s, grid, dim3(kWfBlock), p.lds, s, events, counts, cap, chBase, lists, M, N,
+        hipLaunchKernelGGL(k_cube_splat_s, grid, dim3(kCubeBlock), p.lds, s, events, counts, cap, chBase, lists, M, N,
                            C, p.T, G, parts, guards);
         hipLaunchKernelGGL(k_cube_reduce_s, rgrid, dim3(kWfBlock), 0, s, (double2 *)frame, (const double2 *)parts,
                            (const double2 *)guards, G, M, N, C, p.T, chBase, lists, counts);
     } else {
-        hipLaunchKernelGGL(k_cube_splat, grid, dim3(kWfBlock), p.lds, s, events, counts, cap, chBase, M, N, C, p.Mt,
+        hipLaunchKernelGGL(k_cube_splat, grid, dim3(kCubeBlock), p.lds, s, events, counts, cap, chBase, M, N, C, p.Mt,
                            G, parts);
         hipLaunchKernelGGL(k_cube_reduce, rgrid, dim3(kWfBlock), 0, s, (double2 *)frame, (const double2 *)parts, G,
                            cells, counts, C, chBase, lists);
