@@ -702,7 +702,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         }
         hsr::WfArgs w;
         std::memset(&w, 0, sizeof(w));
-        hsr::wavefront_layout(w, b->d_wf, P, C, knobs->maxDepth);
+        hsr::WfSet sets[2];
+        hsr::wavefront_layout(w, sets, b->d_wf, P, C, knobs->maxDepth);
         w.cells = b->d_cells;
         w.base = done;
         w.n = (int32_t)n;
@@ -724,25 +725,6 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
                  : (int)std::max<size_t>(1, std::min<size_t>(kGatherPerCU, (size_t)b->ldsPerCU / tileBytes));
         const int gridGather = (int)std::max<int64_t>(
             1, std::min<int64_t>((int64_t)b->cu * tilesPerCU, (n + hsr::kTraceBlock - 1) / hsr::kTraceBlock));
-        const size_t needP = (size_t)gridGather * MD;
-        if (!cube && needP > b->partialCap) {
-            dfree(b->d_partials);
-            hsr_status s;
-            if ((s = dalloc(&b->d_partials, needP)) != HSR_OK) return s;
-            b->partialCap = needP;
-        }
-        const int S = std::min(64, gridGather);
-        if (!cube && (size_t)S * MD > b->stageCap) {
-            dfree(b->d_stage_sum);
-            dfree(b->d_stage_comp);
-            hsr_status s;
-            if ((s = dalloc(&b->d_stage_sum, (size_t)S * MD)) != HSR_OK) return s;
-            if ((s = dalloc(&b->d_stage_comp, (size_t)S * MD)) != HSR_OK) return s;
-            b->stageCap = (size_t)S * MD;
-        }
-        // the batch's set-up launch clears the partial frames (unless nothing is splatted) and
-        // the per-slot statistics of a training batch, summed per leaf in a fixed order after it
-        const int64_t partialDoubles = cube || noSplat ? 0 : (int64_t)gridGather * MD;
         const bool training = guide && knobs->doGuiding && !isFinalIteration;
         if (training && !b->d_statRows) {
             hsr_status s = dalloc(&b->d_statRows, (size_t)hsr::kStatRows);
@@ -759,7 +741,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         }
         int nClosest = 0, nShadow = 0;
         hipEvent_t *evC = b->evTrav.data(), *evS = b->evTrav.data() + nev / 2;
-        while ((int)b->evFork.size() < knobs->maxDepth + 1) {
+        while ((int)b->evFork.size() < knobs->maxDepth + 2) {
             hipEvent_t f, j;
             HSR_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
             HSR_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
@@ -781,16 +763,41 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             hsr::tail_lds_bytes(N, C, cube || noSplat) > (size_t)b->ldsPerBlock)
             tailDepth = 0;
         ++b->stats.tail_batches[std::min(tailDepth, 7)];
+        // a batch without guiding records runs its gather stages beside the path chain (the
+        // decoupled plan), and its fused tail adds its frame tiles into partial rows of its own
+        const bool decouple = !training;
+        const int tiles = gridGather * (decouple && tailDepth >= 1 ? 2 : 1);
+        const size_t needP = (size_t)tiles * MD;
+        if (!cube && needP > b->partialCap) {
+            dfree(b->d_partials);
+            hsr_status s;
+            if ((s = dalloc(&b->d_partials, needP)) != HSR_OK) return s;
+            b->partialCap = needP;
+        }
+        const int S = std::min(64, tiles);
+        if (!cube && (size_t)S * MD > b->stageCap) {
+            dfree(b->d_stage_sum);
+            dfree(b->d_stage_comp);
+            hsr_status s;
+            if ((s = dalloc(&b->d_stage_sum, (size_t)S * MD)) != HSR_OK) return s;
+            if ((s = dalloc(&b->d_stage_comp, (size_t)S * MD)) != HSR_OK) return s;
+            b->stageCap = (size_t)S * MD;
+        }
+        // the batch's set-up launch clears the partial frames (unless nothing is splatted) and
+        // the per-slot statistics of a training batch, summed per leaf in a fixed order after it
+        const int64_t partialDoubles = cube || noSplat ? 0 : (int64_t)tiles * MD;
         // the batch's launch sequence: replayed from the graph of its launch shape with the
         // batch's arguments, or stage by stage
         HSR_HIP(hsr::prepare_wavefront(N, C, cube, cube || noSplat));
         hsr::BatchArgs host;
         host.a = a;
         host.w = w;
+        host.set[0] = sets[0];
+        host.set[1] = sets[1];
         host.partials = (double2 *)b->d_partials;
         auto plan = [&] {
             return hsr::plan_wavefront(a, w, gridTrav, gridWork, gridGather, partialDoubles, training,
-                                       b->optCounters, evC, evS, tailDepth);
+                                       b->optCounters, evC, evS, tailDepth, decouple);
         };
         hsr_backend::GraphEntry *graph = nullptr;
         if (b->optGraph) {
@@ -834,7 +841,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         // LDS tiles: the workgroup tiles are reduced into the frame; cube: the gather kernel
         // has added into the frame directly
         if (!cube && !noSplat)
-            HSR_HIP(hsr::launch_reduce(b->d_partials, gridGather, MD, b->d_stage_sum, b->d_stage_comp, S,
+            HSR_HIP(hsr::launch_reduce(b->d_partials, tiles, MD, b->d_stage_sum, b->d_stage_comp, S,
                                        b->d_frame_sum, b->d_frame_comp, b->stream));
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
         // the traversal events all precede ev[1] on the stream, and so the statistics' copy

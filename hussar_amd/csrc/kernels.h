@@ -135,7 +135,8 @@ struct WfArgs {
     double *sd;
     float *sf;
     int32_t *si;
-    float4 *ray_o, *ray_d;
+    float4 *ray_o, *ray_d;  // the current vertex's rays: written by generation and shading, read by shadow
+    const float4 *tray_o, *tray_d;  // the rays being traced (closest-hit traversal, shading, tail)
     int2 *hit;
     float4 *s_d;
     double *s_tn;
@@ -157,15 +158,30 @@ constexpr int kCells = 125 * 243;  // cells of the coherence order: 5^3 x 3^5
 // The kCells residues of the coherence order, cells in Morton order (host).
 std::vector<int32_t> coherence_cells();
 
+// The buffers of one vertex parity: a vertex's rays (its origin, the extension ray leaving it),
+// NEE slots and gather and shadow queues. Vertex k uses set k & 1, so that the shading of
+// vertex k (which writes vertex k + 1's set) may run beside the shadow and gather stages of
+// vertex k.
+struct WfSet {
+    float4 *ray_o, *ray_d, *s_d;
+    double *s_tn, *s_dv;
+    float2 *s_v, *s_g1;
+    int8_t *s_st, *s_vis;
+    int32_t *q_shd, *q_gat;
+};
 // Bytes of device memory the wavefront buffers need for P paths and C channels.
 size_t wavefront_bytes(int64_t P, int C, int maxDepth);
-// Carves the buffers out of one allocation.
-void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth);
+// Carves the buffers out of one allocation: w with the buffers of set 0 in place, both sets in
+// set[2].
+void wavefront_layout(WfArgs &w, WfSet set[2], void *base, int64_t P, int C, int maxDepth);
+// w with the vertex buffers of set v and the traced rays of set t.
+WfArgs wf_with_sets(const WfArgs &w, const WfSet &v, const WfSet &t);
 // One wavefront batch's arguments: the stages receive TraceArgs and WfArgs by value (their
 // pointers then stay in the global address space), the gather and set-up stages the partials.
 struct BatchArgs {
     TraceArgs a;
     WfArgs w;
+    WfSet set[2];
     double2 *partials;  // [gatherGrid][N*C] partial frames, accumulated by the gather stages
 };
 // Allows the gather stage and the fused tail the LDS of an N x C tile (beyond 64 KiB); before
@@ -178,9 +194,12 @@ size_t tail_lds_bytes(int N, int C, bool noTile);
 // built from the batch's arguments and the step's own scalars x, after the layout sig), a
 // fork (stream 1 waits for stream 0's work so far), a join (the reverse) or a timing event.
 struct BatchStep {
-    enum : int8_t { kKernel, kFork, kJoin, kRecord };
+    // kSignal: record sync event x[0] of this stream; kWait: this stream waits for sync event
+    // x[0] of stream x[1] (recorded earlier in the plan)
+    enum : int8_t { kKernel, kFork, kJoin, kRecord, kSignal, kWait };
     enum : int8_t { kSetup, kAW, kAWi, kGather, kTail };
     int8_t kind = kKernel, stream = 0, sig = kAW;
+    int8_t vset = 0, tset = 0;  // kernels: the vertex set and the traced-ray set (WfSet) they use
     hipEvent_t event = nullptr;
     const void *func = nullptr;
     dim3 grid, block;
@@ -199,15 +218,25 @@ struct BatchStep {
 // takes the paths from the depth-K hits on, in place of the stages from the depth-K shading on
 // (its node visits and triangle tests are counted in counters[7] and [8]); otherwise (0) every
 // depth has its stages.
+// decouple (batches that take no guiding records): the chain of each path (closest-hit
+// traversal, shading, tail) runs on stream 1 and the NEE of each vertex (shadow, gather) on
+// stream 0, ordered by sync events only where they share data: a vertex's shadow stage after
+// the shading that set up its slots, a shading after the gather of two vertices back (whose
+// buffer set it reuses). A depth's gather then overlaps the next depth's shading and
+// traversal and the tail; the tail adds its frame tiles into the partials after the gather
+// stages' (from row gridGather on), and the batch ends on stream 0 once stream 1 is done.
+// Without records the shading stage needs nothing the gather computes (the guiding weight
+// only feeds the records).
 struct BatchPlan {
     std::vector<BatchStep> steps;
     int nClosest = 0, nShadow = 0;  // traversal launches (timed pairs)
+    int nSync[2] = {0, 0};          // sync events recorded on stream 0 / 1
 };
 BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
                          int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
-                         hipEvent_t *evShadow, int tailDepth);
+                         hipEvent_t *evShadow, int tailDepth, bool decouple);
 // The plan launched with the arguments h on stream (0) and stream2 (1), ordered by the events
-// evFork / evJoin (maxDepth + 1 each).
+// evFork / evJoin (maxDepth + 2 each; in a decoupled plan the sync events of stream 0 and 1).
 hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t stream, hipStream_t stream2,
                        hipEvent_t *evFork, hipEvent_t *evJoin);
 // The plan as a graph: its kernels as kernel nodes, its timing events as event-record nodes,

@@ -458,8 +458,8 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
             const unsigned int my = claim_items(heads, n, idle, lane, cl);
             if (my != ~0u) {
                 item = queue[my];
-                const float4 o = w.ray_o[item];
-                const float4 d = w.ray_d[item];
+                const float4 o = w.tray_o[item];
+                const float4 d = w.tray_d[item];
                 trav_begin(t, {o.x, o.y, o.z}, {d.x, d.y, d.z}, __builtin_inff(), st);
                 has = true;
                 if (kCount) ++cr;
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
                     load_field(w, i, p);
                     if (a.motion) load_motion(a, w, i, p, depth);
                     if (kRef) {  // the FP32 chain is the ray that was traced
-                        const float4 o = w.ray_o[i], d = w.ray_d[i];
+                        const float4 o = w.tray_o[i], d = w.tray_d[i];
                         p.c32.o = {o.x, o.y, o.z};
                         p.c32.r = o.w;
                         p.c32.d = {d.x, d.y, d.z};
@@ -1069,7 +1069,7 @@ __global__ __launch_bounds__(kWfBlock, 2) void k_wf_tail(TraceArgs a, WfArgs w, 
                 load_record(w, i, p, false);  // K >= 1: the depth-K gather stored the guiding weight
                 p.freq = w.sf[F_FREQ * P + i];
                 if (kRef) {  // the FP32 chain is the ray that was traced
-                    const float4 o = w.ray_o[i], d = w.ray_d[i];
+                    const float4 o = w.tray_o[i], d = w.tray_d[i];
                     p.c32.o = {o.x, o.y, o.z};
                     p.c32.r = o.w;
                     p.c32.d = {d.x, d.y, d.z};
@@ -1520,26 +1520,28 @@ __global__ __launch_bounds__(256) void k_stat_rows_fold(const double *__restrict
 // ------------------------------------------------------------------------ host side
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// one vertex set: rays, NEE slots, shadow and gather queues
+static size_t wf_set_bytes(int64_t P, int64_t S) {
+    return 2 * align256(sizeof(float4) * P) + align256(sizeof(float4) * S) + 2 * align256(sizeof(double) * S) +
+           2 * align256(sizeof(float2) * S) + 2 * align256(sizeof(int8_t) * S) + align256(sizeof(int32_t) * S) +
+           align256(sizeof(int32_t) * P);
+}
+
 size_t wavefront_bytes(int64_t P, int C, int maxDepth) {
     size_t b = 0;
     b += align256(sizeof(double) * D_N * P);
     b += align256(sizeof(float) * F_N * P);
     b += align256(sizeof(int32_t) * I_N * P);
-    b += 2 * align256(sizeof(float4) * P);
     b += align256(sizeof(int2) * P);
-    const int64_t S = P * C;
-    b += align256(sizeof(float4) * S);
-    b += 2 * align256(sizeof(double) * S);
-    b += 2 * align256(sizeof(float2) * S);
-    b += 2 * align256(sizeof(int8_t) * S);
-    b += 3 * align256(sizeof(int32_t) * P) + align256(sizeof(int32_t) * S);
+    b += 2 * wf_set_bytes(P, P * C);
+    b += 2 * align256(sizeof(int32_t) * P);
     b += align256(sizeof(WfCounts) * (maxDepth + 2));
     b += align256(sizeof(int32_t) * (kCells + 1));
     b += align256(sizeof(double) * P) + align256(sizeof(int32_t) * P);
     return b;
 }
 
-void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
+void wavefront_layout(WfArgs &w, WfSet set[2], void *base, int64_t P, int C, int maxDepth) {
     char *p = (char *)base;
     auto take = [&](size_t bytes) {
         char *r = p;
@@ -1551,24 +1553,46 @@ void wavefront_layout(WfArgs &w, void *base, int64_t P, int C, int maxDepth) {
     w.sd = (double *)take(sizeof(double) * D_N * P);
     w.sf = (float *)take(sizeof(float) * F_N * P);
     w.si = (int32_t *)take(sizeof(int32_t) * I_N * P);
-    w.ray_o = (float4 *)take(sizeof(float4) * P);
-    w.ray_d = (float4 *)take(sizeof(float4) * P);
     w.hit = (int2 *)take(sizeof(int2) * P);
-    w.s_d = (float4 *)take(sizeof(float4) * S);
-    w.s_tn = (double *)take(sizeof(double) * S);
-    w.s_dv = (double *)take(sizeof(double) * S);
-    w.s_v = (float2 *)take(sizeof(float2) * S);
-    w.s_g1 = (float2 *)take(sizeof(float2) * S);
-    w.s_st = (int8_t *)take(sizeof(int8_t) * S);
-    w.s_vis = (int8_t *)take(sizeof(int8_t) * S);
+    for (int v = 0; v < 2; ++v) {
+        WfSet &t = set[v];
+        t.ray_o = (float4 *)take(sizeof(float4) * P);
+        t.ray_d = (float4 *)take(sizeof(float4) * P);
+        t.s_d = (float4 *)take(sizeof(float4) * S);
+        t.s_tn = (double *)take(sizeof(double) * S);
+        t.s_dv = (double *)take(sizeof(double) * S);
+        t.s_v = (float2 *)take(sizeof(float2) * S);
+        t.s_g1 = (float2 *)take(sizeof(float2) * S);
+        t.s_st = (int8_t *)take(sizeof(int8_t) * S);
+        t.s_vis = (int8_t *)take(sizeof(int8_t) * S);
+        t.q_shd = (int32_t *)take(sizeof(int32_t) * S);
+        t.q_gat = (int32_t *)take(sizeof(int32_t) * P);
+    }
     w.q_ext0 = (int32_t *)take(sizeof(int32_t) * P);
     w.q_ext1 = (int32_t *)take(sizeof(int32_t) * P);
-    w.q_gat = (int32_t *)take(sizeof(int32_t) * P);
-    w.q_shd = (int32_t *)take(sizeof(int32_t) * S);
     w.cnt = (WfCounts *)take(sizeof(WfCounts) * (maxDepth + 2));
     w.cellStart = (int32_t *)take(sizeof(int32_t) * (kCells + 1));
     w.statV = (double *)take(sizeof(double) * P);
     w.statL = (int32_t *)take(sizeof(int32_t) * P);
+    w = wf_with_sets(w, set[0], set[0]);
+}
+
+WfArgs wf_with_sets(const WfArgs &w, const WfSet &v, const WfSet &t) {
+    WfArgs r = w;
+    r.ray_o = v.ray_o;
+    r.ray_d = v.ray_d;
+    r.s_d = v.s_d;
+    r.s_tn = v.s_tn;
+    r.s_dv = v.s_dv;
+    r.s_v = v.s_v;
+    r.s_g1 = v.s_g1;
+    r.s_st = v.s_st;
+    r.s_vis = v.s_vis;
+    r.q_shd = v.q_shd;
+    r.q_gat = v.q_gat;
+    r.tray_o = t.ray_o;
+    r.tray_d = t.ray_d;
+    return r;
 }
 
 size_t tile_lds_bytes(int N, int C) {
@@ -1725,10 +1749,12 @@ struct StepParams {
     bool b = false;
     double *dp = nullptr;
     double2 *d2p = nullptr;
+    WfArgs w;  // the batch's buffers with the step's vertex and traced-ray sets
     void *p[5] = {};
 };
 static void step_params(const BatchStep &st, const BatchArgs &h, StepParams &sp) {
-    void *a = (void *)&h.a, *w = (void *)&h.w;
+    sp.w = wf_with_sets(h.w, h.set[st.vset], h.set[st.tset]);
+    void *a = (void *)&h.a, *w = (void *)&sp.w;
     switch (st.sig) {
         case BatchStep::kSetup:  // (WfArgs, int nCounts, double *partials, int64 nPartials, int64 nStat)
             sp.i0 = (int)st.x[0];
@@ -1751,9 +1777,9 @@ static void step_params(const BatchStep &st, const BatchArgs &h, StepParams &sp)
             sp.p[1] = w;
             sp.p[2] = &sp.i0;
             break;
-        case BatchStep::kTail:  // (TraceArgs, WfArgs, int K, double2 *partials)
+        case BatchStep::kTail:  // (TraceArgs, WfArgs, int K, double2 *partials): its rows from x[1]
             sp.i0 = (int)st.x[0];
-            sp.d2p = h.partials;
+            sp.d2p = h.partials + st.x[1];
             sp.p[0] = a;
             sp.p[1] = w;
             sp.p[2] = &sp.i0;
@@ -1774,7 +1800,7 @@ static void step_params(const BatchStep &st, const BatchArgs &h, StepParams &sp)
 
 BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
                          int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
-                         hipEvent_t *evShadow, int tailDepth) {
+                         hipEvent_t *evShadow, int tailDepth, bool decouple) {
     BatchPlan plan;
     auto &S = plan.steps;
     const int maxDepth = a.maxDepth;
@@ -1789,8 +1815,9 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
     const int gShadow = fit(gridTrav, n * ((nrx + kPacket - 1) / kPacket), kWfBlock);
     const int gShade = fit(gridWork, n, kRange);
     const int gNee = fit(gridWork, n * nrx, kRange);
-    auto kernel = [&](int stream, const void *f, int grid, unsigned block, size_t lds, int8_t sig, int64_t x0 = 0,
-                      int64_t x1 = 0, int64_t x2 = 0) {
+    // a kernel step using the buffers of vertex set vset and the traced rays of set tset
+    auto kernel = [&](int stream, const void *f, int grid, unsigned block, size_t lds, int8_t sig, int vset,
+                      int tset, int64_t x0 = 0, int64_t x1 = 0, int64_t x2 = 0) {
         BatchStep st;
         st.kind = BatchStep::kKernel;
         st.stream = (int8_t)stream;
@@ -1799,6 +1826,8 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
         st.block = dim3(block);
         st.lds = (unsigned)lds;
         st.sig = sig;
+        st.vset = (int8_t)vset;
+        st.tset = (int8_t)tset;
         st.x[0] = x0;
         st.x[1] = x1;
         st.x[2] = x2;
@@ -1811,64 +1840,117 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
         st.event = e;
         S.push_back(st);
     };
+    auto signal = [&](int stream) {
+        BatchStep st;
+        st.kind = BatchStep::kSignal;
+        st.stream = (int8_t)stream;
+        st.x[0] = plan.nSync[stream]++;
+        S.push_back(st);
+        return (int)st.x[0];
+    };
+    auto wait = [&](int stream, int from, int idx) {
+        BatchStep st;
+        st.kind = BatchStep::kWait;
+        st.stream = (int8_t)stream;
+        st.x[0] = idx;
+        st.x[1] = from;
+        S.push_back(st);
+    };
     // set-up: one workgroup for the scan, the clears spread over up to 256 more
     {
         const int64_t nStat = zeroStats ? n : 0;
         const int64_t clears = std::max<int64_t>(maxDepth + 2, partialDoubles / 2 + nStat);
         const int gSetup = 1 + (int)std::min<int64_t>(256, (clears + 4 * kScanThreads - 1) / (4 * kScanThreads));
-        kernel(0, (const void *)k_wf_setup, gSetup, kScanThreads, 0, BatchStep::kSetup, maxDepth + 2,
+        kernel(0, (const void *)k_wf_setup, gSetup, kScanThreads, 0, BatchStep::kSetup, 0, 0, maxDepth + 2,
                partialDoubles, nStat);
     }
-    // generation: K cells per workgroup, about kRange paths each
+    // generation (vertex 0's rays and gather list): K cells per workgroup, about kRange paths each
     const int64_t q = n / kCells;
     const int K = (int)std::max<int64_t>(1, std::min<int64_t>(kGenCellsMax, kRange / (q + 1)));
     kernel(0, a.refChain ? (const void *)k_wf_gen<true> : (const void *)k_wf_gen<false>, (kCells + K - 1) / K,
-           kWfBlock, 0, BatchStep::kAWi, K);
+           kWfBlock, 0, BatchStep::kAWi, 0, 0, K);
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);  // prepare_wavefront allowed it
     if (!a.onlyIndirect)
         kernel(0, a.refChain ? (const void *)k_wf_nee0<true> : (const void *)k_wf_nee0<false>, gNee, kWfBlock, 0,
-               BatchStep::kAW);
-    for (int k = 0; k <= maxDepth; ++k) {
-        // the depth's closest-hit traversal on stream 1, beside its shadow and gather stages:
-        // it reads the extension queue and rays of depth k and writes only the hits, which the
-        // shading stage reads after the join. A depth without those stages (depth 0 when the
-        // direct paths are not traced) keeps it on stream 0: no fork and join to wait for.
-        const bool nee = k >= 1 || !a.onlyIndirect;
-        const int ts = nee ? 1 : 0;
-        if (k < maxDepth) {
-            if (nee) marker(BatchStep::kFork, 0, nullptr);
-            if (evClosest) marker(BatchStep::kRecord, ts, evClosest[2 * plan.nClosest]);
-            kernel(ts, count ? (const void *)k_wf_trav<true> : (const void *)k_wf_trav<false>, gClosest, kWfBlock,
-                   0, BatchStep::kAWi, k);
-            if (evClosest) marker(BatchStep::kRecord, ts, evClosest[2 * plan.nClosest + 1]);
-            ++plan.nClosest;
+               BatchStep::kAW, 0, 0);
+    const void *fTrav = count ? (const void *)k_wf_trav<true> : (const void *)k_wf_trav<false>;
+    const void *fShadow = count ? (const void *)k_wf_shadow<true> : (const void *)k_wf_shadow<false>;
+    const void *fShade = a.refChain ? (const void *)k_wf_shade<true> : (const void *)k_wf_shade<false>;
+    const void *fTail = a.refChain ? (count ? (const void *)k_wf_tail<true, true> : (const void *)k_wf_tail<true, false>)
+                                   : (count ? (const void *)k_wf_tail<false, true> : (const void *)k_wf_tail<false, false>);
+    // vertex k: its closest-hit traversal reads the rays of set k & 1, its shadow and gather
+    // stages that set's slots and queues, and its shading writes vertex k + 1's set
+    auto trav = [&](int stream, int k) {
+        if (evClosest) marker(BatchStep::kRecord, stream, evClosest[2 * plan.nClosest]);
+        kernel(stream, fTrav, gClosest, kWfBlock, 0, BatchStep::kAWi, k & 1, k & 1, k);
+        if (evClosest) marker(BatchStep::kRecord, stream, evClosest[2 * plan.nClosest + 1]);
+        ++plan.nClosest;
+    };
+    auto shadow_gather = [&](int k) {
+        if (!a.poDiffraction) {
+            if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
+            kernel(0, fShadow, gShadow, kWfBlock, 0, BatchStep::kAWi, k & 1, k & 1, k);
+            if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow + 1]);
+            ++plan.nShadow;
         }
-        if (nee) {
-            if (!a.poDiffraction) {
-                if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
-                kernel(0, count ? (const void *)k_wf_shadow<true> : (const void *)k_wf_shadow<false>, gShadow,
-                       kWfBlock, 0, BatchStep::kAWi, k);
-                if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow + 1]);
-                ++plan.nShadow;
+        kernel(0, (const void *)k_wf_gather, gridGather, kWfBlock, glds, BatchStep::kGather, k & 1, k & 1, k,
+               count ? 1 : 0);
+    };
+    // the fused tail from the depth-k hits on, in place of the stages from this depth's shading
+    // on; its frame tiles go to the partials from row `row`
+    auto tail = [&](int stream, int k, int64_t row) {
+        kernel(stream, fTail, gridGather, kWfBlock, tail_lds_bytes(a.N, a.C, a.cube || a.noSplat), BatchStep::kTail,
+               k & 1, k & 1, k, row * a.N * a.C);
+    };
+    auto shade = [&](int stream, int k) {
+        kernel(stream, fShade, gShade, kWfBlock, 0, BatchStep::kAWi, (k + 1) & 1, k & 1, k);
+    };
+    if (!decouple) {
+        for (int k = 0; k <= maxDepth; ++k) {
+            // the depth's closest-hit traversal on stream 1, beside its shadow and gather stages:
+            // it reads the extension queue and rays of depth k and writes only the hits, which the
+            // shading stage reads after the join. A depth without those stages (depth 0 when the
+            // direct paths are not traced) keeps it on stream 0: no fork and join to wait for.
+            const bool nee = k >= 1 || !a.onlyIndirect;
+            if (k < maxDepth) {
+                if (nee) marker(BatchStep::kFork, 0, nullptr);
+                trav(nee ? 1 : 0, k);
             }
-            kernel(0, (const void *)k_wf_gather, gridGather, kWfBlock, glds, BatchStep::kGather, k, count ? 1 : 0);
+            if (nee) shadow_gather(k);
+            if (k < maxDepth) {
+                if (nee) marker(BatchStep::kJoin, 0, nullptr);
+                if (tailDepth >= 1 && k == tailDepth) {
+                    tail(0, k, 0);
+                    break;
+                }
+                shade(0, k);
+            }
+        }
+        return plan;
+    }
+    // decoupled: the path chain on stream 1, the NEE of each vertex on stream 0
+    wait(1, 0, signal(0));  // generation (and the depth-0 NEE set-up) done
+    std::vector<int> gathered(maxDepth + 1, -1);  // stream-0 sync after vertex k's gather
+    int shaded = -1;                              // stream-1 sync after the last shading
+    for (int k = 0; k <= maxDepth; ++k) {
+        const bool nee = k >= 1 || !a.onlyIndirect;
+        if (k < maxDepth) trav(1, k);
+        if (nee) {
+            if (shaded >= 0) wait(0, 1, shaded);  // vertex k's slots, origins and queues
+            shadow_gather(k);
+            gathered[k] = signal(0);
         }
         if (k < maxDepth) {
-            if (nee) marker(BatchStep::kJoin, 0, nullptr);
             if (tailDepth >= 1 && k == tailDepth) {
-                // the fused tail from the depth-k hits on, in place of the stages from this
-                // depth's shading on
-                const void *f = a.refChain ? (count ? (const void *)k_wf_tail<true, true>
-                                                    : (const void *)k_wf_tail<true, false>)
-                                           : (count ? (const void *)k_wf_tail<false, true>
-                                                    : (const void *)k_wf_tail<false, false>);
-                kernel(0, f, gridGather, kWfBlock, tail_lds_bytes(a.N, a.C, a.cube || a.noSplat), BatchStep::kTail, k);
+                tail(1, k, gridGather);
                 break;
             }
-            kernel(0, a.refChain ? (const void *)k_wf_shade<true> : (const void *)k_wf_shade<false>, gShade,
-                   kWfBlock, 0, BatchStep::kAWi, k);
+            if (k >= 1 && gathered[k - 1] >= 0) wait(1, 0, gathered[k - 1]);  // set (k + 1) & 1 free
+            shade(1, k);
+            shaded = signal(1);
         }
     }
+    wait(0, 1, signal(1));
     return plan;
 }
 
@@ -1876,6 +1958,7 @@ hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t s,
                        hipEvent_t *evJoin) {
     (void)hipGetLastError();
     const hipStream_t st[2] = {s, s2};
+    hipEvent_t *const sync[2] = {evFork, evJoin};
     int forks = 0, joins = 0;
     hipError_t r = hipSuccess;
     for (const BatchStep &x : plan.steps) {
@@ -1896,6 +1979,8 @@ hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t s,
                 if (r == hipSuccess) r = hipStreamWaitEvent(s, evJoin[joins], 0);
                 ++joins;
                 break;
+            case BatchStep::kSignal: r = hipEventRecord(sync[x.stream][x.x[0]], st[x.stream]); break;
+            case BatchStep::kWait: r = hipStreamWaitEvent(st[x.stream], sync[x.x[1]][x.x[0]], 0); break;
             default: r = hipEventRecord(x.event, st[x.stream]); break;
         }
         if (r != hipSuccess) return r;
@@ -1909,9 +1994,11 @@ hipError_t graph_plan(BatchGraph *out, const BatchPlan &plan, const BatchArgs &h
     hipError_t r = hipGraphCreate(&g, 0);
     if (r != hipSuccess) return r;
     // each node depends on the last node of its stream; a fork hands stream 0's to stream 1,
-    // a join adds stream 1's to stream 0's
+    // a join adds stream 1's to stream 0's, a wait the nodes its sync event followed
     std::vector<hipGraphNode_t> deps[2];
     std::vector<hipGraphNode_t> nodes;
+    std::vector<std::vector<hipGraphNode_t>> sync[2];  // the stream's last nodes at each sync event
+    for (int i = 0; i < 2; ++i) sync[i].resize(plan.nSync[i]);
     for (const BatchStep &x : plan.steps) {
         hipGraphNode_t nd = nullptr;
         if (x.kind == BatchStep::kKernel) {
@@ -1929,6 +2016,12 @@ hipError_t graph_plan(BatchGraph *out, const BatchPlan &plan, const BatchArgs &h
             deps[1] = deps[0];
         } else if (x.kind == BatchStep::kJoin) {
             deps[0].insert(deps[0].end(), deps[1].begin(), deps[1].end());
+        } else if (x.kind == BatchStep::kSignal) {
+            sync[x.stream][x.x[0]] = deps[x.stream];
+        } else if (x.kind == BatchStep::kWait) {
+            for (hipGraphNode_t d : sync[x.x[1]][x.x[0]])
+                if (std::find(deps[x.stream].begin(), deps[x.stream].end(), d) == deps[x.stream].end())
+                    deps[x.stream].push_back(d);
         } else {
             r = hipGraphAddEventRecordNode(&nd, g, deps[x.stream].data(), deps[x.stream].size(), x.event);
         }
