@@ -543,7 +543,8 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured (hsr_hip_configure_frame)");
     if (budget < 0 || sampleIndexOffset < 0) return fail(HSR_ERR_INVALID, "negative budget or offset");
-    if (knobs->correctPhase) return fail(HSR_ERR_UNSUPPORTED, "correctPhase is not supported");
+    if (knobs->correctPhase && !knobs->filteringSphere)  // the reference's Assert, path.h:318
+        return fail(HSR_ERR_INVALID, "correctPhase needs filteringSphere");
     if (!knobs->useGeometricalOptics) return fail(HSR_ERR_UNSUPPORTED, "only geometrical optics is supported");
     if (knobs->maxDepth < 0) return fail(HSR_ERR_INVALID, "negative maxDepth");
     if (b->optChain && b->optPipeline != 1)
@@ -591,6 +592,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.filterGuiding = knobs->filterGuiding;
     a.primaryOrder = knobs->primaryOrder;
     a.produceDebugImage = knobs->produceDebugImage;
+    a.correctPhase = knobs->correctPhase;
     a.filteringMin = knobs->filteringMin;
     a.filteringMax = knobs->filteringMax;
     a.filteringRadius = knobs->filteringRadius;

@@ -596,7 +596,21 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
     }
     ns.tn = p.time + rr / kC;
     cf v = cdot(Hn, Hrx);  // measureH (geometry.h:138)
-    if (p.depth > 0) {
+    const bool hard = a.correctPhase && a.filteringSphere;  // the sphere filter cuts (path.h:360-368)
+    if (p.depth > 0 && a.correctPhase) {
+        // phase correction (path.h:316-329): the reflected ray itself measured at the receiver,
+        // its time the distance from the transmitter's mirror image (the vertex moved back by
+        // the path length along the reflected direction) to the receiver, its value
+        // measureH(H, Hrx)·π·dist / (π·R·λ)² / 4π
+        const d3 rxPos = p.o + rr * dn;
+        const double dist = norm((p.o - p.r * p.d) - rxPos);
+        ns.tn = dist / kC;  // setDistance (geometry.h:113)
+        const double den = kPiD * (double)a.filteringRadius * (kC / (double)p.freq);
+        v = cdot(p.H, Hrx);
+        v = v * (float)(kPiD * dist);
+        v = v / (float)(den * den);
+        v = v / fourPi;
+    } else if (p.depth > 0) {
         // GO density correction (path.h:330-341)
         v = v * (float)(p.r / p.cosTheta);
         v = v / fourPi;
@@ -605,6 +619,8 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
             ns.splat = false;
             return ns;
         }
+    }
+    if (p.depth > 0) {
         if (kRef) {
             // phase error and filter of the FP32 chain (path.h:343-387 in float)
             float dp;
@@ -620,7 +636,13 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
             if (a.doFiltering) {
                 if (a.filteringSphere) {
                     const float R = a.filteringRadius;
-                    if (dp > R) {
+                    if (dp > R && hard) {
+                        if (a.filterGuiding) {  // std::pow(float, int) is evaluated in double
+                            const double q = (double)(dp / R);
+                            ns.gw1 = v / (float)(q * q + 1.0);
+                        }
+                        v = cf{0.0f, 0.0f};
+                    } else if (dp > R) {
                         const float f = (float)(1.0 - 0.20 * (double)(dp / R - 1.0f));
                         v = v * fmaxf(f, 0.0f);
                     }
@@ -638,8 +660,7 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
             }
             ns.gwAdd = true;
             ns.v = v;
-            ns.splat = a.produceDebugImage ? !(a.sampleWeight == 0.0f || (v.re == 0.0f && v.im == 0.0f))
-                                           : !(a.sampleWeight == 0.0f || v.re == 0.0f || v.im == 0.0f);
+            ns.splat = a.produceDebugImage || !(a.sampleWeight == 0.0f || v.re == 0.0f || v.im == 0.0f);
             return ns;
         }
         double dphase;
@@ -655,7 +676,13 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
         if (a.doFiltering) {                      // path.h:357-387
             if (a.filteringSphere) {
                 const double R = (double)a.filteringRadius;
-                if (dphase > R) {
+                if (dphase > R && hard) {
+                    if (a.filterGuiding) {
+                        const double q = dphase / R;
+                        ns.gw1 = v / (float)(q * q + 1.0);
+                    }
+                    v = cf{0.0f, 0.0f};
+                } else if (dphase > R) {
                     const double f = 1.0 - 0.20 * (dphase / R - 1.0);
                     v = v * (float)fmax(f, 0.0);
                 }
@@ -675,9 +702,8 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
     }
     ns.v = v;
     // splat skip rule (integrator.h:121-124); with the debug image on, every contribution is
-    // splatted (an exactly zero one adds nothing and is skipped)
-    ns.splat = a.produceDebugImage ? !(a.sampleWeight == 0.0f || (v.re == 0.0f && v.im == 0.0f))
-                                   : !(a.sampleWeight == 0.0f || v.re == 0.0f || v.im == 0.0f);
+    // splatted, a zero one too (it adds nothing to the frame)
+    ns.splat = a.produceDebugImage || !(a.sampleWeight == 0.0f || v.re == 0.0f || v.im == 0.0f);
     return ns;
 }
 

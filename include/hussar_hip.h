@@ -93,8 +93,12 @@ typedef struct hsr_scene {
 } hsr_scene;
 
 /* PathTracer public knobs (path.h:153-169), same names and defaults (see
- * hsr_knobs_default). correctPhase=1 or useGeometricalOptics=0 return
- * HSR_ERR_UNSUPPORTED. primaryOrder fixes the unspecified evaluation order of
+ * hsr_knobs_default). useGeometricalOptics=0 returns HSR_ERR_UNSUPPORTED (the reference marks
+ * that branch "should always be true"); correctPhase=1 (path.h:316-329, 360-368) needs
+ * filteringSphere=1, as the reference asserts (path.h:318), else HSR_ERR_INVALID; its
+ * corrected values are real for real antenna patterns, so the splat skip rule
+ * (integrator.h:121-124) drops them unless produceDebugImage is set, as in the reference.
+ * primaryOrder fixes the unspecified evaluation order of
  * `Vector2f(get1D(), get1D())` (libhussar/include/hussar/samplers/halton.h:24-27):
  * 0 = right-to-left as g++ evaluates it (primary = (RI_5, RI_3)), 1 = left-to-right as
  * clang does (primary = (RI_3, RI_5)). */

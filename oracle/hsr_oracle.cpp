@@ -860,8 +860,9 @@ inline void splat_event(P *frame, int M, int N, int C, int ch, P delta_t, Cx<P> 
 }
 
 // ---------------------------------------------------------------------------------------
-// PathTracer::sample (path.h:236-451), defaults useGeometricalOptics = true,
-// correctPhase = false.
+// PathTracer::sample (path.h:236-451) with useGeometricalOptics = true (the reference's
+// default; its other branch draws extra samples and is not restated); correctPhase as the
+// knobs say.
 //
 // Four precision roles: T — BVH traversal and ray/triangle tests; Gd — the geometry chain
 // that fixes path lengths (sampled and reflected directions, hit points, NEE legs,
@@ -911,15 +912,22 @@ Gd phase_error(const Knobs &knobs, Gd r, V3<Gd> o, V3<Gd> d, V3<Gd> dn, Gd rr, V
     return dphase / lambda;  // path.h:355
 }
 
-// The filter of path.h:357-387: the soft sphere max(1 - 0.2·(dphase/R - 1), 0) beyond R; the
-// plane's linear ramp from filteringMin to filteringMax and zero beyond, whose cut value may
-// feed the guiding weight.
+// The filter of path.h:357-387: the soft sphere max(1 - 0.2·(dphase/R - 1), 0) beyond R (with
+// correctPhase a hard sphere, path.h:360-368, whose cut value may feed the guiding weight
+// through 1/((dphase/R)² + 1)); the plane's linear ramp from filteringMin to filteringMax and
+// zero beyond, whose cut value may feed the guiding weight.
 template <class Gd, class F>
 void filter_weight(const Knobs &knobs, Gd dphase, Cx<F> &v, Cx<F> &guidingWeight) {
     if (!knobs.doFiltering) return;
     if (knobs.filteringSphere) {
         const Gd R = Gd(knobs.filteringRadius);
-        if (dphase > R) {
+        if (dphase > R && knobs.correctPhase) {
+            if (knobs.filterGuiding) {  // std::pow(Float, int) is evaluated in double
+                const double q = (double)(dphase / R);
+                guidingWeight = guidingWeight + v / F(q * q + 1.0);
+            }
+            v = Cx<F>{F(0), F(0)};
+        } else if (dphase > R) {
             const Gd f = Gd(1.0 - 0.20 * (double)(dphase / R - Gd(1)));
             v = v * F(std::max(f, Gd(0)));
         }
@@ -1098,9 +1106,34 @@ template <class T, class Gd, class F, class P, bool kPromote, bool kDual = false
                         const Cx<F> green = Cx<F>{F(1) / std::max(rrF, F(1e-3)), k0} / ((F(4) * piF) * rrF);
                         Hn = Hn * green;
                     }
-                    const P tn = time + P(rr) / P(kSpeedOfLight);  // addDistance in connect
-                    Cx<F> v = cdot(Hn, Hrx);                       // measureH, geometry.h:138
-                    if (depth > 0) {
+                    P tn = time + P(rr) / P(kSpeedOfLight);  // addDistance in connect
+                    Cx<F> v = cdot(Hn, Hrx);                 // measureH, geometry.h:138
+                    if (depth > 0 && knobs->correctPhase) {
+                        // phase correction (path.h:316-329): the reflected ray itself measured at
+                        // the receiver, its time the distance from the transmitter's mirror image
+                        // o - r·d to the receiver (setDistance, geometry.h:113), its value
+                        // measureH(H, Hrx)·π·dist / (π·R·λ)² / 4π; then the phase error and the
+                        // hard sphere (the FP32 chain's in the dual instance), and gw += v
+                        const V3<Gd> rxPos = o + rr * dn;
+                        const Gd dist = norm((o - r * d) - rxPos);
+                        tn = P(dist) / P(kSpeedOfLight);
+                        const Gd piG = K<Gd>::pi();
+                        const double den = (double)(piG * Gd(knobs->filteringRadius) * lambda);
+                        v = cdot(H, Hrx);
+                        v = v * F(piG * dist);
+                        v = v / F(den * den);
+                        v = v / (F(4) * piF);
+                        if constexpr (kDual) {
+                            const float lambda32 = float(kSpeedOfLight) / float(freq);
+                            const float dp = phase_error<float>(*knobs, r32, o32, d32, dn32, rr32, d32_in, n32_in,
+                                                                lambda32);
+                            filter_weight<float, F>(*knobs, dp, v, guidingWeight);
+                        } else {
+                            const Gd dphase = phase_error<Gd>(*knobs, r, o, d, dn, rr, d_in, n_in, lambda);
+                            filter_weight<Gd, F>(*knobs, dphase, v, guidingWeight);
+                        }
+                        guidingWeight = guidingWeight + v;  // path.h:392-394
+                    } else if (depth > 0) {
                         if constexpr (kDual) {
                             // the GO weight from the Gd chain, the grazing skip, phase error and
                             // filter from the FP32 chain (as the ref32 instance)
@@ -1532,7 +1565,8 @@ int orc_render(void *mh, const SceneC *scene, const Knobs *knobs, const FrameCfg
                int64_t count, float sampleWeight, int isFinal, void *guide, int nthreads, double *frame_out,
                double *totalWeight, int32_t *hit_ids, int8_t *nee, int64_t record_paths, int64_t *counters) {
     if (!mh || !scene || !knobs || !fc || !frame_out) return 1;
-    if (knobs->correctPhase || !knobs->useGeometricalOptics) return 5;
+    if (!knobs->useGeometricalOptics) return 5;
+    if (knobs->correctPhase && !knobs->filteringSphere) return 1;  // path.h:318
     if (fc->samplesPerChirp <= 0 || fc->channelCount <= 0) return 1;
     Counters c;
     render_for(prec)(*(Mesh *)mh, *scene, *knobs, *fc, offset, count, sampleWeight, isFinal != 0, (Guide *)guide,

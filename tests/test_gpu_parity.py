@@ -209,7 +209,7 @@ def test_repeatable(dihedral):
 
 @pytest.mark.parametrize("pipeline", PIPELINES, ids=["fused", "wavefront"])
 @pytest.mark.parametrize("case", ["direct", "po_diffraction", "plane_filter", "no_filter", "clang_order",
-                                  "small_frame", "odd_budget"])
+                                  "small_frame", "odd_budget", "correct_phase", "correct_phase_cut"])
 def test_knob_and_shape_variants(oracle, case, pipeline):
     mesh = scenes.dihedral_mesh()
     sc = scenes.dihedral_pose(5.0)
@@ -233,6 +233,13 @@ def test_knob_and_shape_variants(oracle, case, pipeline):
         fc = H.FrameConfig(1, 24, 1)
     elif case == "odd_budget":
         n = 100_003
+    elif case.startswith("correct_phase"):
+        # path.h:316-329, 360-368; the corrected values are real for these antenna patterns, so
+        # the debug image keeps them from the splat skip rule; "cut": a radius of 20 wavelengths
+        # cuts the contributions whose reflected ray passes the receiver further away
+        kn.correctPhase = True
+        kn.produceDebugImage = True
+        kn.filteringRadius = 20.0 if case == "correct_phase_cut" else 1e4
     b = make_backend(mesh, fc, pipeline)
     om = oracle.OracleMesh.from_mesh(mesh)
     check_parity(oracle, om, b, sc, kn, fc, n, record=min(n, 1 << 16), fp64=True)
@@ -280,7 +287,9 @@ def test_errors_and_interrupt(dihedral):
     mesh, b, fc = dihedral
     sc = scenes.dihedral_pose(0.0)
     with pytest.raises(HsrError, match="UNSUPPORTED"):
-        b.run_raw(sc, H.Knobs(correctPhase=True, doGuiding=False), 0, 10)
+        b.run_raw(sc, H.Knobs(useGeometricalOptics=False, doGuiding=False), 0, 10)
+    with pytest.raises(HsrError, match="INVALID"):  # path.h:318 asserts the sphere filter
+        b.run_raw(sc, H.Knobs(correctPhase=True, filteringSphere=False, doGuiding=False), 0, 10)
     with pytest.raises(HsrError, match="INVALID"):
         b.run_raw(sc, H.Knobs(doGuiding=True), 0, 10)  # guiding without a guide
     flag = C.c_bool(True)

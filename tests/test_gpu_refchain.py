@@ -97,7 +97,8 @@ def test_vehicle_reference_chain_hits_bitexact(oracle, vehicle_guided):
     assert rel_l2(z0, zf) < rel_l2(z1, zf) + 1e-6  # and follows the exact geometry at least as well
 
 
-@pytest.mark.parametrize("case", ["plane_filter", "plane_filter_guiding", "no_filter", "po_diffraction", "clang_order"])
+@pytest.mark.parametrize("case", ["plane_filter", "plane_filter_guiding", "no_filter", "po_diffraction", "clang_order",
+                                  "correct_phase"])
 def test_reference_chain_knob_variants(oracle, vehicle_guided, case):
     mesh, om, b, pt, og, sc, fc = vehicle_guided
     kw = dict(maxDepth=4, doGuiding=True)
@@ -105,7 +106,10 @@ def test_reference_chain_knob_variants(oracle, vehicle_guided, case):
                "plane_filter_guiding": dict(filteringSphere=False, filteringMin=40.0, filteringMax=300.0,
                                             filterGuiding=True),
                "no_filter": dict(doFiltering=False),
-               "po_diffraction": dict(poDiffraction=True), "clang_order": dict(primaryOrder=1)}[case])
+               "po_diffraction": dict(poDiffraction=True), "clang_order": dict(primaryOrder=1),
+               # the hard sphere of the phase correction at the default radius; the debug image
+               # keeps the real corrected values from the splat skip rule
+               "correct_phase": dict(correctPhase=True, filterGuiding=True, produceDebugImage=True)}[case])
     kn = H.Knobs(**kw)
     n = 1 << 16
     print(f"reference chain: {case}", flush=True)
