@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 import hussar_amd as H  # noqa: E402
 from hussar_amd import scenes  # noqa: E402
-from hussar_amd._lib import OPT_COUNTERS, OPT_GRAPH, OPT_PIPELINE, OPT_RECORD_HITS, OPT_TAIL  # noqa: E402
+from hussar_amd._lib import OPT_COUNTERS, OPT_GRAPH, OPT_MARKERS, OPT_PIPELINE, OPT_RECORD_HITS, OPT_TAIL  # noqa: E402
 
 METRIC = "paths/s & frames/s at ≤1e-3 IF rel-L2 vs FP64; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -518,6 +518,9 @@ def main():
                     help="first depth of the fused tail (HSR_OPT_TAIL), 0 none, -1 the library's default")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="1 replay each batch shape's graph (HSR_OPT_GRAPH), 0 stage-by-stage launches")
+    ap.add_argument("--markers", action="store_true",
+                    help="ROCTx ranges around passes, batches and stage launches (HSR_OPT_MARKERS), for "
+                         "rocprofv3 --marker-trace")
     ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.counter_child:
@@ -551,6 +554,8 @@ def main():
     backend.set_option(OPT_GRAPH, args.graph)
     if args.tail >= 0:
         backend.set_option(OPT_TAIL, args.tail)
+    if args.markers:
+        backend.set_option(OPT_MARKERS, 1)
     bvh_ms = backend.stats()["bvh_build_ms"]
     setup_s = time.perf_counter() - t0
 

@@ -24,6 +24,7 @@ OPT_CUBE_EVENTS = 8
 OPT_CHAIN = 9
 OPT_GRAPH = 10
 OPT_TAIL = 11
+OPT_MARKERS = 12
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER

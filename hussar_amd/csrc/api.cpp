@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/hussar_hip.h"
 #include "backend.h"
 #include "bvh.h"
@@ -413,6 +415,10 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
             if (value < -1 || value > 64) return fail(HSR_ERR_INVALID, "tail depth must lie in [-1, 64]");
             b->optTail = (int32_t)value;
             return HSR_OK;
+        case HSR_OPT_MARKERS:
+            if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "markers must be 0 or 1");
+            b->optMarkers = value != 0;
+            return HSR_OK;
         case HSR_OPT_GRAPH:
             if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "graph must be 0 (stage launches) or 1 (replay)");
             b->optGraph = value != 0;
@@ -536,6 +542,17 @@ static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
 // schedule call instead of replacing them. discardFrame: the caller clears the frame before
 // anything can read it (a training iteration of a guiding schedule that clears before each
 // iteration), so the wavefront pipeline splats nothing and reduces no frame.
+// A ROCTx range (HSR_OPT_MARKERS) held for the object's lifetime.
+struct Marker {
+    bool on;
+    Marker(bool markers, const char *message) : on(markers) {
+        if (on) roctxRangePushA(message);
+    }
+    ~Marker() {
+        if (on) roctxRangePop();
+    }
+};
+
 static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
                            int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
                            const volatile uint8_t *interruptFlag, bool accumulate, bool queueStats = false,
@@ -543,6 +560,11 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (!b->configured) return fail(HSR_ERR_INVALID, "frame not configured (hsr_hip_configure_frame)");
     if (budget < 0 || sampleIndexOffset < 0) return fail(HSR_ERR_INVALID, "negative budget or offset");
+    char passName[96];
+    if (b->optMarkers)
+        std::snprintf(passName, sizeof(passName), "hsr pass: %lld paths from %lld%s", (long long)budget,
+                      (long long)sampleIndexOffset, isFinalIteration ? ", final" : "");
+    const Marker passMark(b->optMarkers, b->optMarkers ? passName : "");
     if (knobs->correctPhase && !knobs->filteringSphere)  // the reference's Assert, path.h:318
         return fail(HSR_ERR_INVALID, "correctPhase needs filteringSphere");
     if (!knobs->useGeometricalOptics) return fail(HSR_ERR_UNSUPPORTED, "only geometrical optics is supported");
@@ -693,6 +715,10 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         // wavefront batches: path state in HBM, all stage sizes on the device
         if (interruptFlag && *interruptFlag) return fail(HSR_ERR_INTERRUPTED, "interrupted");
         const int64_t n = std::min({budget - done, b->optPathsPerLaunch, b->wfCap});
+        char batchName[96];
+        if (b->optMarkers)
+            std::snprintf(batchName, sizeof(batchName), "hsr batch: tx %d, %lld paths", tx, (long long)n);
+        const Marker batchMark(b->optMarkers, b->optMarkers ? batchName : "");
         const int64_t P = std::min(budget, std::min(b->optPathsPerLaunch, b->wfCap));
         const size_t need = hsr::wavefront_bytes(P, C, knobs->maxDepth);
         if (need > b->wfBytes) {
@@ -817,7 +843,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         } else {
             const hsr::BatchPlan p = plan();
             HSR_HIP(hsr::launch_plan(p, host, b->stream, b->useStream2 ? b->stream2 : b->stream, b->evFork.data(),
-                                     b->evJoin.data()));
+                                     b->evJoin.data(), b->optMarkers));
             nClosest = p.nClosest;
             nShadow = p.nShadow;
         }
@@ -842,9 +868,11 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         }
         // LDS tiles: the workgroup tiles are reduced into the frame; cube: the gather kernel
         // has added into the frame directly
-        if (!cube && !noSplat)
+        if (!cube && !noSplat) {
+            const Marker reduceMark(b->optMarkers, "hsr reduce");
             HSR_HIP(hsr::launch_reduce(b->d_partials, tiles, MD, b->d_stage_sum, b->d_stage_comp, S,
                                        b->d_frame_sum, b->d_frame_comp, b->stream));
+        }
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
         // the traversal events all precede ev[1] on the stream, and so the statistics' copy
         HSR_HIP(event_wait(statsFirst ? guide->evDl : b->ev[3]));

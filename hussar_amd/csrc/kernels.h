@@ -201,6 +201,7 @@ struct BatchStep {
     int8_t kind = kKernel, stream = 0, sig = kAW;
     int8_t vset = 0, tset = 0;  // kernels: the vertex set and the traced-ray set (WfSet) they use
     hipEvent_t event = nullptr;
+    const char *name = nullptr;  // kernels: the stage's name (ROCTx ranges)
     const void *func = nullptr;
     dim3 grid, block;
     unsigned lds = 0;
@@ -236,9 +237,10 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
                          int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
                          hipEvent_t *evShadow, int tailDepth, bool decouple);
 // The plan launched with the arguments h on stream (0) and stream2 (1), ordered by the events
-// evFork / evJoin (maxDepth + 2 each; in a decoupled plan the sync events of stream 0 and 1).
+// evFork / evJoin (maxDepth + 2 each; in a decoupled plan the sync events of stream 0 and 1);
+// markers: a ROCTx range around each kernel launch, named after its stage.
 hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t stream, hipStream_t stream2,
-                       hipEvent_t *evFork, hipEvent_t *evJoin);
+                       hipEvent_t *evFork, hipEvent_t *evJoin, bool markers = false);
 // The plan as a graph: its kernels as kernel nodes, its timing events as event-record nodes,
 // each depending on the last node of its stream (built node by node: no stream is captured, so
 // other threads' HIP calls are unaffected), instantiated twice. A replay sets the kernel

@@ -25,6 +25,8 @@
 #include <utility>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "trace_common.h"
 
 namespace hsr {
@@ -1840,6 +1842,7 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
         st.event = e;
         S.push_back(st);
     };
+    auto named = [&](const char *name) { S.back().name = name; };
     auto signal = [&](int stream) {
         BatchStep st;
         st.kind = BatchStep::kSignal;
@@ -1863,16 +1866,20 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
         const int gSetup = 1 + (int)std::min<int64_t>(256, (clears + 4 * kScanThreads - 1) / (4 * kScanThreads));
         kernel(0, (const void *)k_wf_setup, gSetup, kScanThreads, 0, BatchStep::kSetup, 0, 0, maxDepth + 2,
                partialDoubles, nStat);
+        named("hsr setup");
     }
     // generation (vertex 0's rays and gather list): K cells per workgroup, about kRange paths each
     const int64_t q = n / kCells;
     const int K = (int)std::max<int64_t>(1, std::min<int64_t>(kGenCellsMax, kRange / (q + 1)));
     kernel(0, a.refChain ? (const void *)k_wf_gen<true> : (const void *)k_wf_gen<false>, (kCells + K - 1) / K,
            kWfBlock, 0, BatchStep::kAWi, 0, 0, K);
+    named("hsr generation");
     const size_t glds = a.cube ? 0 : gather_lds_bytes(a.N, a.C);  // prepare_wavefront allowed it
-    if (!a.onlyIndirect)
+    if (!a.onlyIndirect) {
         kernel(0, a.refChain ? (const void *)k_wf_nee0<true> : (const void *)k_wf_nee0<false>, gNee, kWfBlock, 0,
                BatchStep::kAW, 0, 0);
+        named("hsr nee0");
+    }
     const void *fTrav = count ? (const void *)k_wf_trav<true> : (const void *)k_wf_trav<false>;
     const void *fShadow = count ? (const void *)k_wf_shadow<true> : (const void *)k_wf_shadow<false>;
     const void *fShade = a.refChain ? (const void *)k_wf_shade<true> : (const void *)k_wf_shade<false>;
@@ -1883,6 +1890,7 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
     auto trav = [&](int stream, int k) {
         if (evClosest) marker(BatchStep::kRecord, stream, evClosest[2 * plan.nClosest]);
         kernel(stream, fTrav, gClosest, kWfBlock, 0, BatchStep::kAWi, k & 1, k & 1, k);
+        named("hsr closest-hit traversal");
         if (evClosest) marker(BatchStep::kRecord, stream, evClosest[2 * plan.nClosest + 1]);
         ++plan.nClosest;
     };
@@ -1890,20 +1898,24 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
         if (!a.poDiffraction) {
             if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
             kernel(0, fShadow, gShadow, kWfBlock, 0, BatchStep::kAWi, k & 1, k & 1, k);
+            named("hsr shadow");
             if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow + 1]);
             ++plan.nShadow;
         }
         kernel(0, (const void *)k_wf_gather, gridGather, kWfBlock, glds, BatchStep::kGather, k & 1, k & 1, k,
                count ? 1 : 0);
+        named("hsr gather");
     };
     // the fused tail from the depth-k hits on, in place of the stages from this depth's shading
     // on; its frame tiles go to the partials from row `row`
     auto tail = [&](int stream, int k, int64_t row) {
         kernel(stream, fTail, gridGather, kWfBlock, tail_lds_bytes(a.N, a.C, a.cube || a.noSplat), BatchStep::kTail,
                k & 1, k & 1, k, row * a.N * a.C);
+        named("hsr tail");
     };
     auto shade = [&](int stream, int k) {
         kernel(stream, fShade, gShade, kWfBlock, 0, BatchStep::kAWi, (k + 1) & 1, k & 1, k);
+        named("hsr shade");
     };
     if (!decouple) {
         for (int k = 0; k <= maxDepth; ++k) {
@@ -1955,7 +1967,7 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
 }
 
 hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t s, hipStream_t s2, hipEvent_t *evFork,
-                       hipEvent_t *evJoin) {
+                       hipEvent_t *evJoin, bool markers) {
     (void)hipGetLastError();
     const hipStream_t st[2] = {s, s2};
     hipEvent_t *const sync[2] = {evFork, evJoin};
@@ -1966,7 +1978,9 @@ hipError_t launch_plan(const BatchPlan &plan, const BatchArgs &h, hipStream_t s,
             case BatchStep::kKernel: {
                 StepParams sp;
                 step_params(x, h, sp);
+                if (markers && x.name) roctxRangePushA(x.name);
                 r = hipLaunchKernel(x.func, x.grid, x.block, sp.p, x.lds, st[x.stream]);
+                if (markers && x.name) roctxRangePop();
                 break;
             }
             case BatchStep::kFork:

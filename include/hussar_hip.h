@@ -349,6 +349,12 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        Results are those of the stages; hsr_run_stats counts the batches by
  *                        their tail depth. */
 #define HSR_OPT_TAIL 11
+/*  HSR_OPT_MARKERS       1 = the host marks its work with ROCTx ranges (rocprofv3
+ *                        --marker-trace shows them): one per PathTracer::run iteration
+ *                        ("hsr pass"), per wavefront batch ("hsr batch"), around each stage
+ *                        launch of a batch (the kernel's name) and around the frame reduction
+ *                        ("hsr reduce"); 0 = none (default). */
+#define HSR_OPT_MARKERS 12
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]

@@ -14,7 +14,8 @@ import pytest
 
 import hussar_amd as H
 from hussar_amd import scenes
-from hussar_amd._lib import HsrError, OPT_COUNTERS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE, OPT_RECORD_HITS, OPT_TAIL
+from hussar_amd._lib import (HsrError, OPT_COUNTERS, OPT_MARKERS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE,
+                             OPT_RECORD_HITS, OPT_TAIL)
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -339,6 +340,26 @@ def test_traversal_visits_per_ray_stay_small():
     print(f"node visits per closest-hit ray {closest:.2f}, per shadow packet {shadow:.2f}", flush=True)
     assert 1.0 < closest < 5.0
     assert 1.0 < shadow < 30.0
+
+
+def test_markers_leave_results_alone(dihedral):
+    """ROCTx ranges around passes, batches and stage launches (HSR_OPT_MARKERS) only mark the
+    host's work: the same run gives the same frame and records with and without them."""
+    mesh, b, fc = dihedral
+    sc = scenes.dihedral_pose(10.0)
+    kn = H.Knobs(maxDepth=3, doGuiding=False)
+    z0, tw0, r0 = gpu_run(b, sc, kn, 1 << 16, record=4096)
+    b.set_option(OPT_MARKERS, 1)
+    try:
+        z1, tw1, r1 = gpu_run(b, sc, kn, 1 << 16, record=4096)
+    finally:
+        b.set_option(OPT_MARKERS, 0)
+    assert tw0 == tw1
+    np.testing.assert_array_equal(r0[0], r1[0])
+    np.testing.assert_array_equal(r0[1], r1[1])
+    assert rel_l2(z1, z0) < 1e-12
+    with pytest.raises(HsrError, match="INVALID"):
+        b.set_option(OPT_MARKERS, 2)
 
 
 @pytest.mark.parametrize("size_cm", [5, 15])
