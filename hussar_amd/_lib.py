@@ -25,6 +25,10 @@ OPT_CHAIN = 9
 OPT_GRAPH = 10
 OPT_TAIL = 11
 OPT_MARKERS = 12
+OPT_PRIMARY = 13
+OPT_PRIMARY_MIN = 14
+OPT_PRIMARY_RES = 15
+OPT_PRIMARY_CAP = 16
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhussar_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "wavefront.hip", "post.hip", "api.cpp", "serve.cpp", "bvh.cpp", "guide.cpp"]
+SOURCES = ["kernels.hip", "wavefront.hip", "primary.hip", "post.hip", "api.cpp", "serve.cpp", "bvh.cpp", "guide.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-result", "-Wno-unused-command-line-argument"]

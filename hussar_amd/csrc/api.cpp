@@ -136,6 +136,24 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     HSR_HIP(hipMemcpy(b->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(float), hipMemcpyHostToDevice));
     b->slotIds.resize(bvh.slotCount);
     for (int32_t s2 = 0; s2 < bvh.slotCount; ++s2) std::memcpy(&b->slotIds[s2], &bvh.tris[12 * (size_t)s2 + 3], 4);
+    {
+        // one slot of every triangle that has one (its first in leaf order), for the primary
+        // visibility lists: every slot of a triangle holds the same whole triangle
+        std::vector<int32_t> first((size_t)triangleCount, -1), ts;
+        for (int32_t s2 = 0; s2 < bvh.slotCount; ++s2) {
+            const int32_t id = b->slotIds[s2];
+            if (id >= 0 && id < triangleCount && first[id] < 0) first[id] = s2;
+        }
+        for (int32_t v : first)
+            if (v >= 0) ts.push_back(v);
+        b->pvTriangles = (int64_t)ts.size();
+        if (!ts.empty()) {
+            if ((s = dalloc(&b->d_triSlot, ts.size())) != HSR_OK) return s;
+            HSR_HIP(hipMemcpy(b->d_triSlot, ts.data(), ts.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+    }
+    HSR_HIP(hipHostMalloc((void **)&b->h_pvMeta, 4 * sizeof(int32_t), hipHostMallocDefault));
+    for (auto &e : b->evPv) HSR_HIP(hipEventCreate(&e));
     if ((s = dalloc(&b->d_counters, 16)) != HSR_OK) return s;
     {
         static const std::vector<int32_t> cells = hsr::coherence_cells();
@@ -171,11 +189,16 @@ void hsr_hip_destroy(hsr_backend *b) {
     dfree(b->d_vel);
     dfree(b->d_post);
     dfree(b->d_post2);
+    dfree(b->d_triSlot);
+    if (b->d_pv) (void)hipFree(b->d_pv);
     if (b->d_postScratch) (void)hipFree(b->d_postScratch);
     if (b->d_wf) (void)hipFree(b->d_wf);
     for (auto &g : b->graphs) hsr::destroy_graph(g.g);
     b->graphs.clear();
     if (b->h_ext) (void)hipHostFree(b->h_ext);
+    if (b->h_pvMeta) (void)hipHostFree(b->h_pvMeta);
+    for (auto &e : b->evPv)
+        if (e) (void)hipEventDestroy(e);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : b->evTrav) (void)hipEventDestroy(e);
@@ -428,8 +451,146 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
                 return fail(HSR_ERR_UNSUPPORTED, "chain must be 0 (FP64) or 1 (the reference's FP32 chain)");
             b->optChain = (int32_t)value;
             return HSR_OK;
+        case HSR_OPT_PRIMARY:
+            if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "primary must be 0 (BVH) or 1 (visibility lists)");
+            // the lists' first run on the device faulted and the cause is not found: disabled
+            if (value == 1) return fail(HSR_ERR_UNSUPPORTED, "primary visibility lists are disabled");
+            b->optPrimary = (int32_t)value;
+            return HSR_OK;
+        case HSR_OPT_PRIMARY_MIN:
+            if (value < 0) return fail(HSR_ERR_INVALID, "primary minimum paths must not be negative");
+            b->optPrimaryMin = value;
+            return HSR_OK;
+        case HSR_OPT_PRIMARY_RES:
+            if (value < 4 || value > hsr::kPvMaxRes) return fail(HSR_ERR_INVALID, "primary resolution must lie in [4, 80]");
+            if ((int32_t)value != b->optPrimaryRes) b->pvBuilt = false;
+            b->optPrimaryRes = (int32_t)value;
+            return HSR_OK;
+        case HSR_OPT_PRIMARY_CAP:
+            if (value < 0 || value >= (int64_t(1) << 31))
+                return fail(HSR_ERR_INVALID, "primary list capacity must lie in [0, 2^31)");
+            b->pvCap = value;
+            b->pvCapSet = value > 0;
+            b->pvBuilt = false;
+            return HSR_OK;
         default: return fail(HSR_ERR_INVALID, "unknown option");
     }
+}
+
+// A ROCTx range (HSR_OPT_MARKERS) held for the object's lifetime.
+struct Marker {
+    bool on;
+    Marker(bool markers, const char *message) : on(markers) {
+        if (on) roctxRangePushA(message);
+    }
+    ~Marker() {
+        if (on) roctxRangePop();
+    }
+};
+
+static size_t pv_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The primary visibility lists of transmitter position pos over this run's mesh, built on the
+// backend's stream unless the lists of the same mesh, position and frame epoch are in place;
+// their pointers into a. A build's meta is copied to the host: a build over the lists'
+// capacity is found after the batch (the depth-0 kernel then traversed the BVH) and the
+// capacity grows for the next build.
+static hsr_status pv_prepare(hsr_backend *b, const hsr_backend *mesh, const float pos[3], hsr::TraceArgs &a) {
+    const int R = b->optPrimaryRes;
+    const int64_t T = mesh->pvTriangles;
+    const bool same = b->pvBuilt && b->pvMesh == mesh && b->pvEpoch == b->epoch && b->pv.R == R &&
+                      std::memcmp(b->pvPos, pos, sizeof(b->pvPos)) == 0;
+    if (!same) {
+        const int64_t cells = 6 * (int64_t)R * R;
+        const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>(256, T / 256));
+        if (!b->pvCapSet) b->pvCap = std::max<int64_t>(b->pvCap, std::max<int64_t>(int64_t(1) << 23, 64 * T));
+        b->pvCap = std::max<int64_t>(b->pvCap, 1);
+        b->pvCap = std::min<int64_t>(b->pvCap, (int64_t(1) << 31) - 1);
+        const size_t sortBytes = hsr::pv_sort_bytes(T);
+        const size_t sz[] = {(size_t)T * 4,
+                             (size_t)T * 4,
+                             (size_t)T * 4,
+                             (size_t)T * 4,
+                             (size_t)hsr::pv_prep_blocks(T) * 25 * 4,
+                             6 * 16,
+                             16,
+                             (size_t)chunks * cells * 4,
+                             (size_t)(cells + 1) * 4,
+                             (size_t)b->pvCap * 8,
+                             sortBytes};
+        size_t total = 0;
+        for (size_t v : sz) total += pv_align(v);
+        if (total > b->pvBytes) {
+            if (b->d_pv) {
+                HSR_HIP(hipStreamSynchronize(b->stream));
+                (void)hipFree(b->d_pv);
+            }
+            b->d_pv = nullptr;
+            b->pvBytes = 0;
+            HSR_HIP(hipMalloc(&b->d_pv, total));
+            b->pvBytes = total;
+        }
+        char *p = (char *)b->d_pv;
+        void *ptr[11];
+        for (int i = 0; i < 11; ++i) {
+            ptr[i] = p;
+            p += pv_align(sz[i]);
+        }
+        hsr::PvBuild &v = b->pv;
+        v.tris = mesh->d_tris;
+        v.triSlot = mesh->d_triSlot;
+        v.T = T;
+        std::memcpy(v.o, pos, sizeof(v.o));
+        v.R = R;
+        v.chunks = chunks;
+        v.cap = b->pvCap;
+        v.keys = (uint32_t *)ptr[0];
+        v.keys2 = (uint32_t *)ptr[1];
+        v.slots = (int32_t *)ptr[2];
+        v.slots2 = (int32_t *)ptr[3];
+        v.part = (float *)ptr[4];
+        v.faces = (float4 *)ptr[5];
+        v.meta = (int32_t *)ptr[6];
+        v.matrix = (int32_t *)ptr[7];
+        v.start = (int32_t *)ptr[8];
+        v.list = (int2 *)ptr[9];
+        v.sortTemp = ptr[10];
+        v.sortBytes = sortBytes;
+        const Marker mark(b->optMarkers, "hsr primary lists");
+        HSR_HIP(hipEventRecord(b->evPv[0], b->stream));
+        HSR_HIP(hsr::launch_pv_build(v, b->stream));
+        HSR_HIP(hipEventRecord(b->evPv[1], b->stream));
+        HSR_HIP(hipMemcpyAsync(b->h_pvMeta, v.meta, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, b->stream));
+        b->pvBuilt = true;
+        b->pvCheck = true;
+        b->pvMesh = mesh;
+        b->pvEpoch = b->epoch;
+        std::memcpy(b->pvPos, pos, sizeof(b->pvPos));
+        ++b->stats.primary_builds;
+    }
+    a.pvMeta = b->pv.meta;
+    a.pvFaces = b->pv.faces;
+    a.pvKeys = b->pv.keys2;
+    a.pvSlots = b->pv.slots2;
+    a.pvStart = b->pv.start;
+    a.pvList = b->pv.list;
+    a.pvR = R;
+    return HSR_OK;
+}
+
+// After a batch that followed a build: its device time, and the capacity check.
+static hsr_status pv_settle(hsr_backend *b) {
+    if (!b->pvCheck) return HSR_OK;
+    b->pvCheck = false;
+    float ms = 0.0f;
+    HSR_HIP(hipEventElapsedTime(&ms, b->evPv[0], b->evPv[1]));
+    b->stats.primary_ms += ms;
+    if (b->h_pvMeta[1] != 0) {
+        const int64_t need = (int64_t)(uint32_t)b->h_pvMeta[2];
+        b->pvCap = std::max<int64_t>(2 * b->pvCap, need + need / 4);
+        b->pvBuilt = false;
+    }
+    return HSR_OK;
 }
 
 static hipError_t event_wait(hipEvent_t e);
@@ -542,17 +703,6 @@ static void fill_scene(hsr::TraceArgs &a, const hsr_scene *sc) {
 // schedule call instead of replacing them. discardFrame: the caller clears the frame before
 // anything can read it (a training iteration of a guiding schedule that clears before each
 // iteration), so the wavefront pipeline splats nothing and reduces no frame.
-// A ROCTx range (HSR_OPT_MARKERS) held for the object's lifetime.
-struct Marker {
-    bool on;
-    Marker(bool markers, const char *message) : on(markers) {
-        if (on) roctxRangePushA(message);
-    }
-    ~Marker() {
-        if (on) roctxRangePop();
-    }
-};
-
 static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t sampleIndexOffset,
                            int64_t budget, float sampleWeight, int32_t isFinalIteration, hsr_guide *guide,
                            const volatile uint8_t *interruptFlag, bool accumulate, bool queueStats = false,
@@ -711,6 +861,17 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     std::memcpy(a.tx.pos, txa.position, sizeof(a.tx.pos));
     std::memcpy(a.tx.rot, txa.rotation, sizeof(a.tx.rot));
     a.chBase = tx * scene->rxCount;
+    a.pvMeta = nullptr;
+    a.pvFaces = nullptr;
+    a.pvKeys = nullptr;
+    a.pvSlots = nullptr;
+    a.pvStart = nullptr;
+    a.pvList = nullptr;
+    a.pvR = 0;
+    if (b->optPipeline == 1 && b->optPrimary && budget >= b->optPrimaryMin && budget > 0 && mesh->d_triSlot) {
+        hsr_status s = pv_prepare(b, mesh, txa.position, a);
+        if (s != HSR_OK) return s;
+    }
     for (int64_t done = 0; done < budget && b->optPipeline == 1;) {
         // wavefront batches: path state in HBM, all stage sizes on the device
         if (interruptFlag && *interruptFlag) return fail(HSR_ERR_INTERRUPTED, "interrupted");
@@ -832,7 +993,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
             const std::vector<int64_t> key = {n, gridTrav, gridWork, gridGather, partialDoubles, training,
                                               b->optCounters, a.maxDepth, a.onlyIndirect, a.poDiffraction,
                                               a.refChain, a.cube, a.rxCount, a.C, a.N, (int64_t)(intptr_t)evC,
-                                              tailDepth, noSplat};
+                                              tailDepth, noSplat, a.pvList != nullptr};
             graph = batch_graph(b, key, plan, host);
         }
         if (graph) {
@@ -876,6 +1037,10 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         HSR_HIP(hipEventRecord(b->ev[3], b->stream));
         // the traversal events all precede ev[1] on the stream, and so the statistics' copy
         HSR_HIP(event_wait(statsFirst ? guide->evDl : b->ev[3]));
+        {
+            hsr_status s = pv_settle(b);
+            if (s != HSR_OK) return s;
+        }
         for (int k = 0; k < depths; ++k) b->lastExt[k] = b->h_ext[k];
         b->lastBatchDepths = depths;
         b->lastBatchPaths = n;
@@ -1316,6 +1481,7 @@ hsr_status hsr_guide_get_training(hsr_guide *g, int32_t *child, double *stat, in
 hsr_status hsr_path_tracer_run(hsr_backend *b, const hsr_scene *scene, const hsr_knobs *knobs, int64_t samples,
                                hsr_guide *guide, const volatile uint8_t *interruptFlag) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
+    ++b->epoch;  // a new frame: its primary visibility lists are built again
     hsr_status s = hsr_hip_clear_frame(b);  // setup() + clearFrame(), path.h:173-174
     if (s != HSR_OK) return s;
     if ((s = reset_stats(b)) != HSR_OK) return s;
@@ -1358,6 +1524,7 @@ hsr_status hsr_path_tracer_run_schedule(hsr_backend *b, const hsr_scene *scene, 
                                         hsr_guide *guide, const volatile uint8_t *interruptFlag) {
     if (!b || !scene || !knobs) return fail(HSR_ERR_INVALID, "null argument");
     if (trainingPasses < 0 || trainingPaths < 0 || finalPaths < 0) return fail(HSR_ERR_INVALID, "negative schedule");
+    ++b->epoch;  // a new frame: its primary visibility lists are built again
     hsr_status s;
     const bool guiding = knobs->doGuiding && guide;
     if (knobs->doGuiding && !guide) return fail(HSR_ERR_INVALID, "doGuiding requires a guide");

@@ -75,6 +75,8 @@ struct hsr_backend {
     double bvh_ms = 0.0;
     std::vector<int32_t> slotIds;  // triangle id of every slot (leaf order)
     float4 *d_vel = nullptr;       // per slot velocity, null for a static mesh
+    int32_t *d_triSlot = nullptr;  // one slot of each triangle that has one (pvTriangles)
+    int64_t pvTriangles = 0;
     // when set, runs trace this backend's mesh instead of their own (a serving slot running
     // a request on a resident scene; the server keeps the scene alive meanwhile)
     const hsr_backend *meshFrom = nullptr;
@@ -161,6 +163,24 @@ struct hsr_backend {
     static constexpr int kGraphs = 8;
     std::vector<GraphEntry> graphs;
     uint64_t graphClock = 0;
+    // primary visibility lists (HSR_OPT_PRIMARY): depth-0 closest hits from the lists of the
+    // traced transmitter's position, built on the device at most once per position and frame
+    // (epoch: advanced by every PathTracer::run) for runs of at least optPrimaryMin paths. Off:
+    // switching them on is refused (their first GPU run faulted, cause not found)
+    int32_t optPrimary = 0;
+    int64_t optPrimaryMin = int64_t(1) << 20;
+    int32_t optPrimaryRes = 64;
+    void *d_pv = nullptr;
+    size_t pvBytes = 0;
+    int64_t pvCap = 0;      // list entries (grows when a build exceeds it)
+    bool pvCapSet = false;  // HSR_OPT_PRIMARY_CAP set it
+    hsr::PvBuild pv{};
+    int32_t *h_pvMeta = nullptr;  // pinned copy of the last build's meta (near, over, entries)
+    bool pvBuilt = false, pvCheck = false;
+    const hsr_backend *pvMesh = nullptr;
+    float pvPos[3] = {0.0f, 0.0f, 0.0f};
+    uint64_t pvEpoch = 0, epoch = 0;
+    hipEvent_t evPv[2] = {nullptr, nullptr};
     hsr_run_stats stats{};
 };
 

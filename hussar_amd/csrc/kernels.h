@@ -97,6 +97,16 @@ struct TraceArgs {
     CubeEvent *events;
     unsigned *evCount;
     unsigned evCap;
+    // primary visibility lists of the traced transmitter (primary.h), read by the depth-0
+    // closest-hit kernel; null: depth 0 traverses the BVH
+    const int32_t *pvMeta;   // [0] triangles near the transmitter (they lead pvKeys / pvSlots),
+                             // [1] 1 = the lists exceeded their capacity (the BVH is used)
+    const float4 *pvFaces;   // [6] face grids (u0, v0, su, sv)
+    const uint32_t *pvKeys;  // triangles in key order: keys (float bits) and slots
+    const int32_t *pvSlots;
+    const int32_t *pvStart;  // [6 R R + 1] first entry of each cell
+    const int2 *pvList;      // (slot, key bits) per cell in key order
+    int32_t pvR;
 };
 
 hipError_t launch_trace(const TraceArgs &args, int grid, bool count, hipStream_t stream);
@@ -300,6 +310,33 @@ hipError_t launch_cube_splat(const CubeSplat &plan, const CubeEvent *events, uns
                              int chBase, int lists, int M, int N, int C, int G, double2 *parts, double *frame,
                              hipStream_t stream);
 size_t trace_lds_bytes(int N, int C);
+
+// ---------------------------------------------------------------- primary visibility lists
+// Build of the lists of one transmitter position o (primary.hip); every buffer is the caller's:
+// keys, keys2, slots, slots2 [T]; part [pv_prep_blocks(T) * 25]; faces [6]; meta [4]; matrix
+// [chunks * 6 R R]; start [6 R R + 1]; list [cap]; sortTemp [sortBytes >= pv_sort_bytes(T)].
+// R <= kPvMaxRes (the count and fill workgroups hold 6 R R counters in LDS).
+constexpr int kPvMaxRes = 80;
+struct PvBuild {
+    const float4 *tris;
+    const int32_t *triSlot;  // [T] one slot of each triangle
+    int64_t T;
+    float o[3];
+    int R, chunks;
+    int64_t cap;
+    uint32_t *keys, *keys2;
+    int32_t *slots, *slots2;
+    float *part;
+    float4 *faces;
+    int32_t *meta;
+    int32_t *matrix, *start;
+    int2 *list;
+    void *sortTemp;
+    size_t sortBytes;
+};
+size_t pv_sort_bytes(int64_t T);
+int pv_prep_blocks(int64_t T);
+hipError_t launch_pv_build(const PvBuild &p, hipStream_t stream);
 
 // ---------------------------------------------------------------- frame post-processing
 // post = (sum - comp) * inv, n doubles

@@ -27,6 +27,7 @@
 
 #include <rocprofiler-sdk-roctx/roctx.h>
 
+#include "primary.h"
 #include "trace_common.h"
 
 namespace hsr {
@@ -478,6 +479,79 @@ __global__ __launch_bounds__(kWfBlock, HSR_TRAV_OCC) void k_wf_trav(TraceArgs a,
                 has = false;
             }
         }
+    }
+    if (kCount) {
+        wave_count(&a.counters[0], cn, lane);
+        wave_count(&a.counters[1], ct, lane);
+        wave_count(&a.counters[2], cr, lane);
+    }
+}
+
+// Closest hits of the depth-0 extension rays through the traced transmitter's primary
+// visibility lists (primary.h, built by primary.hip): the triangles near the transmitter, then
+// the list of the ray's direction cell, each in key order until a key exceeds the best hit so
+// far. The triangle test and the tie rule are those of the BVH traversal, so the hits are the
+// same. Lists over their capacity (pvMeta[1] != 0) fall back to the BVH traversal.
+__device__ inline void pv_test(const float4 *__restrict__ tris, int32_t s, f3 o, f3 d, float &best, int32_t &bid,
+                               int32_t &bslot) {
+    const float4 p = tris[3 * (int64_t)s + 0];
+    const float4 e1 = tris[3 * (int64_t)s + 1];
+    const float4 e2 = tris[3 * (int64_t)s + 2];
+    const float t = tri_intersect({p.x, p.y, p.z}, {e1.x, e1.y, e1.z}, {e2.x, e2.y, e2.z}, o, d, kEps,
+                                  __builtin_inff());
+    const int32_t id = __float_as_int(p.w);
+    if (t < __builtin_inff() && (bid < 0 || t < best || (t == best && id < bid))) {
+        best = t;
+        bid = id;
+        bslot = s;
+    }
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kWfBlock) void k_wf_trav0(TraceArgs a, WfArgs w, int depth) {
+    __shared__ int32_t lds_stack[kStackShort * kWfBlock];
+    const int lane = threadIdx.x & 63;
+    const unsigned n = w.cnt[depth].n_ext.v;
+    const int32_t *queue = (depth & 1) ? w.q_ext1 : w.q_ext0;
+    const bool lists = a.pvMeta[1] == 0;
+    const int near = a.pvMeta[0];
+    int32_t spill[kStackSpill];
+    Stack st;
+    st.spill = spill;
+    st.set_column(lds_stack + threadIdx.x);
+    st.sp = 0;
+    uint32_t cn = 0, ct = 0, cr = 0;
+    for (unsigned i = blockIdx.x * kWfBlock + threadIdx.x; i < n; i += gridDim.x * kWfBlock) {
+        const int32_t item = queue[i];
+        const float4 o4 = w.tray_o[item];
+        const float4 d4 = w.tray_d[item];
+        const f3 o{o4.x, o4.y, o4.z}, d{d4.x, d4.y, d4.z};
+        float best = __builtin_inff();
+        int32_t bid = -1, bslot = -1;
+        if (kCount) ++cr;
+        if (lists) {
+            for (int j = 0; j < near; ++j) {
+                if (__uint_as_float(a.pvKeys[j]) > best) break;
+                if (kCount) ++ct;
+                pv_test(a.tris, a.pvSlots[j], o, d, best, bid, bslot);
+            }
+            float u, v;
+            const int f = pv::ray_face(d.x, d.y, d.z, u, v);
+            const float4 g = a.pvFaces[f];
+            const int c = pv::ray_cell(f, u, v, pv::FaceGrid{g.x, g.y, g.z, g.w}, a.pvR);
+            if (c >= 0) {
+                const int32_t j1 = a.pvStart[c + 1];
+                for (int32_t j = a.pvStart[c]; j < j1; ++j) {
+                    const int2 e = a.pvList[j];
+                    if (__int_as_float(e.y) > best) break;
+                    if (kCount) ++ct;
+                    pv_test(a.tris, e.x, o, d, best, bid, bslot);
+                }
+            }
+        } else {
+            bid = closest_hit<kCount>(a.nodes, a.tris, o, d, st, best, bslot, cn, ct);
+        }
+        w.hit[item] = make_int2(bslot, __float_as_int(best));
     }
     if (kCount) {
         wave_count(&a.counters[0], cn, lane);
@@ -1887,9 +1961,11 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
                                    : (count ? (const void *)k_wf_tail<false, true> : (const void *)k_wf_tail<false, false>);
     // vertex k: its closest-hit traversal reads the rays of set k & 1, its shadow and gather
     // stages that set's slots and queues, and its shading writes vertex k + 1's set
+    // depth 0 through the transmitter's primary visibility lists when the batch has them
+    const void *fTrav0 = count ? (const void *)k_wf_trav0<true> : (const void *)k_wf_trav0<false>;
     auto trav = [&](int stream, int k) {
         if (evClosest) marker(BatchStep::kRecord, stream, evClosest[2 * plan.nClosest]);
-        kernel(stream, fTrav, gClosest, kWfBlock, 0, BatchStep::kAWi, k & 1, k & 1, k);
+        kernel(stream, k == 0 && a.pvList ? fTrav0 : fTrav, gClosest, kWfBlock, 0, BatchStep::kAWi, k & 1, k & 1, k);
         named("hsr closest-hit traversal");
         if (evClosest) marker(BatchStep::kRecord, stream, evClosest[2 * plan.nClosest + 1]);
         ++plan.nClosest;

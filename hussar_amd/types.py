@@ -71,7 +71,8 @@ class c_run_stats(C.Structure):
                 ("closest_ms", C.c_double), ("shadow_ms", C.c_double), ("closest_launches", C.c_int64),
                 ("shadow_launches", C.c_int64), ("nodes_shadow", C.c_int64), ("tris_shadow", C.c_int64),
                 ("graph_launches", C.c_int64), ("graph_captures", C.c_int64), ("nodes_tail", C.c_int64),
-                ("tris_tail", C.c_int64), ("tail_batches", C.c_int64 * 8)]
+                ("tris_tail", C.c_int64), ("tail_batches", C.c_int64 * 8),
+                ("primary_builds", C.c_int64), ("primary_ms", C.c_double)]
 
 
 class c_device_info(C.Structure):

@@ -279,6 +279,9 @@ typedef struct hsr_run_stats {
     int64_t tris_tail;
     /* wavefront batches by the depth their fused tail started at (0: none; 7: 7 or deeper) */
     int64_t tail_batches[8];
+    /* primary visibility lists (HSR_OPT_PRIMARY) built, and their device time */
+    int64_t primary_builds;
+    double primary_ms;
 } hsr_run_stats;
 
 hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
@@ -355,6 +358,30 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        launch of a batch (the kernel's name) and around the frame reduction
  *                        ("hsr reduce"); 0 = none (default). */
 #define HSR_OPT_MARKERS 12
+/*  HSR_OPT_PRIMARY       wavefront pipeline: 1 = the depth-0 closest hits of a run of
+ *                        at least HSR_OPT_PRIMARY_MIN paths come from primary visibility lists
+ *                        of the transmitter's position instead of a BVH traversal: the
+ *                        directions around the transmitter cut into the cells of a cube map
+ *                        (HSR_OPT_PRIMARY_RES^2 per face over the part each face sees of the
+ *                        mesh), each cell listing the triangles it sees in order of a lower
+ *                        bound of their distance; a ray tests its cell's triangles in that order
+ *                        until the bound passes its best hit. The same triangle test and tie
+ *                        rule give the same hits as the traversal. The lists are built on the
+ *                        device once per transmitter position and PathTracer::run
+ *                        (hsr_path_tracer_run, hsr_path_tracer_run_schedule); plain runs
+ *                        (hsr_hip_run) reuse the lists of the same position since the last such
+ *                        run. 0 = the BVH traversal at every depth (the default). Setting 1 is refused with
+ *                        HSR_ERR_UNSUPPORTED in this build: the lists' first device run faulted
+ *                        and the cause has not been found. */
+#define HSR_OPT_PRIMARY 13
+/*  HSR_OPT_PRIMARY_MIN   fewest paths of a run that builds or uses the lists (default 2^20). */
+#define HSR_OPT_PRIMARY_MIN 14
+/*  HSR_OPT_PRIMARY_RES   cells per side of each cube face of the lists, 4 to 80 (default 64). */
+#define HSR_OPT_PRIMARY_RES 15
+/*  HSR_OPT_PRIMARY_CAP   list entries the lists may hold (0 = the default: at least 2^23 and
+ *                        64 per triangle). A build that needs more leaves the BVH traversal to
+ *                        that run's depth-0 rays, and the capacity grows for the next build. */
+#define HSR_OPT_PRIMARY_CAP 16
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]
