@@ -367,6 +367,36 @@ __global__ __launch_bounds__(kScanThreads) void k_wf_setup(WfArgs w, int nCounts
 // stages. A sampling tree of up to kGuideLds nodes is copied into LDS first: the guiding warp
 // descends it with one dependent load per level.
 constexpr int kGuideLds = 2048;
+// Order of one cell's paths. The cell's Halton indices are h = H0 + j kCells (j < cnt), and the
+// next base-5 digit of RI_5(h) (floor(h / 125) mod 5 = (A + 3 j) mod 5, kCells = 125 * 243) and
+// the next base-3 digit of RI_3(h) (floor(h / 243) mod 3 = (B + 2 j) mod 3) cut the cell into
+// 5 x 3 sub-cells that depend on j mod 15 alone. Slot p of the cell takes the j of the p-th
+// path when the sub-cells are taken in a serpentine order (rows of the base-3 digit, the base-5
+// digit alternately up and down) and each sub-cell's paths in index order, so that a wave's
+// rays come from one or two neighbouring sub-cells.
+#ifndef HSR_SUBCELLS
+#define HSR_SUBCELLS 1
+#endif
+__device__ inline int32_t subcell_j(int32_t p, int32_t cnt, int64_t H0) {
+#if HSR_SUBCELLS
+    const int A = (int)((H0 / 125) % 5), Bv = (int)((H0 / 243) % 3);
+    int32_t base = 0;
+    for (int s = 0; s < 15; ++s) {
+        const int dv = s / 5, du = (dv & 1) ? 4 - s % 5 : s % 5;
+        // j mod 5 with (A + 3 j) mod 5 = du (3^-1 = 2 mod 5), j mod 3 with (B + 2 j) mod 3 = dv
+        // (2^-1 = 2 mod 3), combined into j mod 15 (6 = 1 mod 5 and 0 mod 3, 10 = 0 mod 5 and 1 mod 3)
+        const int r5 = (2 * (du - A + 5)) % 5, r3 = (2 * (dv - Bv + 3)) % 3;
+        const int r = (6 * r5 + 10 * r3) % 15;
+        const int32_t n = r < cnt ? (cnt - r + 14) / 15 : 0;
+        if (p < base + n) return r + 15 * (p - base);
+        base += n;
+    }
+#endif
+    (void)cnt;
+    (void)H0;
+    return p;
+}
+
 constexpr int kGenCellsMax = kRange;  // cells per workgroup (K)
 template <bool kRef>
 __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int K) {
@@ -407,7 +437,9 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_gen(TraceArgs a, WfArgs w, int 
                     if (cstart[mid] <= item) lo = mid;
                     else hi = mid - 1;
                 }
-                const int64_t m = cell_first(w.cells[c0 + lo], w.b0mod) + (int64_t)(item - cstart[lo]) * kCells;
+                const int64_t t = cell_first(w.cells[c0 + lo], w.b0mod);
+                const int32_t j = subcell_j(item - cstart[lo], cstart[lo + 1] - cstart[lo], B0 + t);
+                const int64_t m = t + (int64_t)j * kCells;
                 p.local = w.base + m;
                 generate<kRef>(a, (uint64_t)(B0 + m), p, gsChild, gsP0);
             }
