@@ -35,7 +35,8 @@ sys.path.insert(0, ROOT)
 
 import hussar_amd as H  # noqa: E402
 from hussar_amd import scenes  # noqa: E402
-from hussar_amd._lib import OPT_COUNTERS, OPT_GRAPH, OPT_MARKERS, OPT_PIPELINE, OPT_RECORD_HITS, OPT_TAIL  # noqa: E402
+from hussar_amd._lib import (OPT_COUNTERS, OPT_GRAPH, OPT_MARKERS, OPT_PIPELINE, OPT_RECORD_HITS,  # noqa: E402
+                             OPT_SHADOW_SPLAT, OPT_TAIL)
 
 METRIC = "paths/s & frames/s at ≤1e-3 IF rel-L2 vs FP64; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -89,6 +90,9 @@ def counter_child(args):
     pt.configureFrame(fc)
     backend = H.HipBackend(mesh, pt)
     backend.set_option(OPT_PIPELINE, args.pipeline)
+    backend.set_option(OPT_SHADOW_SPLAT, args.shadow_splat)
+    if args.tail >= 0:
+        backend.set_option(OPT_TAIL, args.tail)
     run_step(backend, pt, scene, sched)
     backend.fetch()
 
@@ -106,7 +110,7 @@ def collect_counters(args):
         d = tempfile.mkdtemp(prefix="hsr_pmc_", dir="/tmp")
         cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", *ctrs, "--output-format", "csv", "-d", d, "-o", "pmc",
                "--", sys.executable, os.path.abspath(__file__), "--counter-child", "--workload", args.workload,
-               "--pipeline", str(args.pipeline)]
+               "--pipeline", str(args.pipeline), "--shadow-splat", str(args.shadow_splat), "--tail", str(args.tail)]
         t0 = time.perf_counter()
         r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
         rows = []
@@ -518,6 +522,9 @@ def main():
                     help="first depth of the fused tail (HSR_OPT_TAIL), 0 none, -1 the library's default")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="1 replay each batch shape's graph (HSR_OPT_GRAPH), 0 stage-by-stage launches")
+    ap.add_argument("--shadow-splat", type=int, default=1, choices=(0, 1),
+                    help="1 the shadow stages of batches without guiding records splat (HSR_OPT_SHADOW_SPLAT), "
+                         "0 a gather stage after each")
     ap.add_argument("--markers", action="store_true",
                     help="ROCTx ranges around passes, batches and stage launches (HSR_OPT_MARKERS), for "
                          "rocprofv3 --marker-trace")
@@ -552,6 +559,7 @@ def main():
     backend = H.HipBackend(mesh, pt)
     backend.set_option(OPT_PIPELINE, args.pipeline)
     backend.set_option(OPT_GRAPH, args.graph)
+    backend.set_option(OPT_SHADOW_SPLAT, args.shadow_splat)
     if args.tail >= 0:
         backend.set_option(OPT_TAIL, args.tail)
     if args.markers:

@@ -29,6 +29,7 @@ OPT_PRIMARY = 13
 OPT_PRIMARY_MIN = 14
 OPT_PRIMARY_RES = 15
 OPT_PRIMARY_CAP = 16
+OPT_SHADOW_SPLAT = 17
 
 # Every exported symbol and its signature: (restype, argtypes)
 _P = C.POINTER

@@ -466,6 +466,10 @@ hsr_status hsr_hip_set_option(hsr_backend *b, int32_t option, int64_t value) {
             if ((int32_t)value != b->optPrimaryRes) b->pvBuilt = false;
             b->optPrimaryRes = (int32_t)value;
             return HSR_OK;
+        case HSR_OPT_SHADOW_SPLAT:
+            if (value < 0 || value > 1) return fail(HSR_ERR_INVALID, "shadow splat must be 0 or 1");
+            b->optShadowSplat = value != 0;
+            return HSR_OK;
         case HSR_OPT_PRIMARY_CAP:
             if (value < 0 || value >= (int64_t(1) << 31))
                 return fail(HSR_ERR_INVALID, "primary list capacity must lie in [0, 2^31)");
@@ -955,6 +959,11 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         // a batch without guiding records runs its gather stages beside the path chain (the
         // decoupled plan), and its fused tail adds its frame tiles into partial rows of its own
         const bool decouple = !training;
+        // its shadow stages splat (no gather stage) when their frame tile, event lists and
+        // traversal stack fit a workgroup's LDS (HSR_OPT_SHADOW_SPLAT)
+        const bool fuseSplat = b->optShadowSplat && !cube &&
+                               hsr::shadow_splat_lds_bytes(N, C) + hsr::kStackShort * 256 * sizeof(int32_t) <=
+                                   (size_t)b->ldsPerBlock;
         const int tiles = gridGather * (decouple && tailDepth >= 1 ? 2 : 1);
         const size_t needP = (size_t)tiles * MD;
         if (!cube && needP > b->partialCap) {
@@ -986,14 +995,14 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         host.partials = (double2 *)b->d_partials;
         auto plan = [&] {
             return hsr::plan_wavefront(a, w, gridTrav, gridWork, gridGather, partialDoubles, training,
-                                       b->optCounters, evC, evS, tailDepth, decouple);
+                                       b->optCounters, evC, evS, tailDepth, decouple, fuseSplat);
         };
         hsr_backend::GraphEntry *graph = nullptr;
         if (b->optGraph) {
             const std::vector<int64_t> key = {n, gridTrav, gridWork, gridGather, partialDoubles, training,
                                               b->optCounters, a.maxDepth, a.onlyIndirect, a.poDiffraction,
                                               a.refChain, a.cube, a.rxCount, a.C, a.N, (int64_t)(intptr_t)evC,
-                                              tailDepth, noSplat, a.pvList != nullptr};
+                                              tailDepth, noSplat, a.pvList != nullptr, fuseSplat};
             graph = batch_graph(b, key, plan, host);
         }
         if (graph) {

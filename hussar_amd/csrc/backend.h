@@ -113,6 +113,7 @@ struct hsr_backend {
     int32_t optChain = 0;  // HSR_OPT_CHAIN
     int32_t optTail = -1;  // HSR_OPT_TAIL: first depth of the fused tail, 0 = none, -1 = adaptive
     bool optMarkers = false;  // HSR_OPT_MARKERS: ROCTx ranges around passes, batches and stage launches
+    bool optShadowSplat = true;  // HSR_OPT_SHADOW_SPLAT: decoupled batches splat in their shadow stages
     // the extension rays of each depth of the last wavefront batch (h_ext: pinned landing area
     // of their copy), for the adaptive tail depth
     static constexpr int kTrackDepths = 64;

@@ -245,7 +245,11 @@ struct BatchPlan {
 };
 BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
                          int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
-                         hipEvent_t *evShadow, int tailDepth, bool decouple);
+                         hipEvent_t *evShadow, int tailDepth, bool decouple, bool fuseSplat = false);
+// LDS (dynamic) of the fused shadow + splat stage of decoupled batches (frame tile and per-wave
+// event lists; the traversal stack is static); the stage is used when it and the stack fit a
+// workgroup's LDS.
+size_t shadow_splat_lds_bytes(int N, int C);
 // The plan launched with the arguments h on stream (0) and stream2 (1), ordered by the events
 // evFork / evJoin (maxDepth + 2 each; in a decoupled plan the sync events of stream 0 and 1);
 // markers: a ROCTx range around each kernel launch, named after its stage.

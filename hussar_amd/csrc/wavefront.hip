@@ -711,6 +711,50 @@ __device__ inline void packet_run(const float4 *__restrict__ nodes, const float4
     }
 }
 
+// A shadow packet from shadow queue entry e (path * 4 + group): the path's vertex as the shared
+// origin, the group's receivers' directions and extents from their slots (slot0 = the first),
+// the range of their reciprocal directions and whether they share their direction signs.
+__device__ inline void packet_begin(const TraceArgs &a, const WfArgs &w, int32_t e, Packet &pk, int64_t &slot0,
+                                    int &rays) {
+    const int nrx = min(a.rxCount, a.C);
+    const int64_t i = e >> 2;
+    const int c0 = (e & 3) * kPacket;
+    rays = min(kPacket, nrx - c0);
+    slot0 = i * a.C + c0;
+    pk.alive = 0;
+    uint32_t sg[kPacket];
+    const float4 o = w.ray_o[i];
+    pk.o = {o.x, o.y, o.z};
+    const float inf = __builtin_inff();
+    pk.ivlo = {inf, inf, inf};
+    pk.ivhi = {-inf, -inf, -inf};
+    pk.tmax = 0.0f;
+#pragma unroll
+    for (int r = 0; r < kPacket; ++r) {
+        f3 iv = {0.0f, 0.0f, 1.0f};
+        if (r < rays) {
+            const float4 d = w.s_d[slot0 + r];
+            pk.d[r] = {d.x, d.y, d.z};
+            iv = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
+            pk.tfar[r] = d.w;
+            pk.alive |= 1u << r;
+            pk.ivlo = {fminf(pk.ivlo.x, iv.x), fminf(pk.ivlo.y, iv.y), fminf(pk.ivlo.z, iv.z)};
+            pk.ivhi = {fmaxf(pk.ivhi.x, iv.x), fmaxf(pk.ivhi.y, iv.y), fmaxf(pk.ivhi.z, iv.z)};
+            pk.tmax = fmaxf(pk.tmax, d.w);
+        } else {
+            pk.d[r] = {0.0f, 0.0f, 1.0f};
+            pk.tfar[r] = 0.0f;
+        }
+        sg[r] = (iv.x >= 0.0f ? 1u : 0u) | (iv.y >= 0.0f ? 2u : 0u) | (iv.z >= 0.0f ? 4u : 0u);
+    }
+    pk.sgn = sg[0];
+    pk.same = true;
+#pragma unroll
+    for (int r = 1; r < kPacket; ++r)
+        if (r < rays && sg[r] != sg[0]) pk.same = false;
+    pk.node = 0;
+}
+
 template <bool kCount>
 __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, int depth) {
     __shared__ int32_t lds_stack[kStackShort * kWfBlock];
@@ -724,7 +768,6 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
     WfCounts *cnt = &w.cnt[depth];
     const unsigned int n = cnt->n_shd.v;
     WfCounter *heads = cnt->head_shd;
-    const int nrx = min(a.rxCount, a.C);
     uint32_t cn = 0, ct = 0, cr = 0;
     Packet pk;
     int64_t slot0 = 0;
@@ -736,43 +779,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shadow(TraceArgs a, WfArgs w, i
         if (!cl.dry && __popcll(idle) >= kRefillMin) {
             const unsigned int my = claim_items(heads, n, idle, lane, cl);
             if (my != ~0u) {
-                const int32_t e = w.q_shd[my];
-                const int64_t i = e >> 2;
-                const int c0 = (e & 3) * kPacket;
-                rays = min(kPacket, nrx - c0);
-                slot0 = i * a.C + c0;
-                pk.alive = 0;
-                uint32_t sg[kPacket];
-                const float4 o = w.ray_o[i];
-                pk.o = {o.x, o.y, o.z};
-                const float inf = __builtin_inff();
-                pk.ivlo = {inf, inf, inf};
-                pk.ivhi = {-inf, -inf, -inf};
-                pk.tmax = 0.0f;
-#pragma unroll
-                for (int r = 0; r < kPacket; ++r) {
-                    f3 iv = {0.0f, 0.0f, 1.0f};
-                    if (r < rays) {
-                        const float4 d = w.s_d[slot0 + r];
-                        pk.d[r] = {d.x, d.y, d.z};
-                        iv = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
-                        pk.tfar[r] = d.w;
-                        pk.alive |= 1u << r;
-                        pk.ivlo = {fminf(pk.ivlo.x, iv.x), fminf(pk.ivlo.y, iv.y), fminf(pk.ivlo.z, iv.z)};
-                        pk.ivhi = {fmaxf(pk.ivhi.x, iv.x), fmaxf(pk.ivhi.y, iv.y), fmaxf(pk.ivhi.z, iv.z)};
-                        pk.tmax = fmaxf(pk.tmax, d.w);
-                    } else {
-                        pk.d[r] = {0.0f, 0.0f, 1.0f};
-                        pk.tfar[r] = 0.0f;
-                    }
-                    sg[r] = (iv.x >= 0.0f ? 1u : 0u) | (iv.y >= 0.0f ? 2u : 0u) | (iv.z >= 0.0f ? 4u : 0u);
-                }
-                pk.sgn = sg[0];
-                pk.same = true;
-#pragma unroll
-                for (int r = 1; r < kPacket; ++r)
-                    if (r < rays && sg[r] != sg[0]) pk.same = false;
-                pk.node = 0;
+                packet_begin(a, w, w.q_shd[my], pk, slot0, rays);
                 st.sp = 0;
                 has = true;
                 if (kCount) cr += rays;
@@ -979,6 +986,121 @@ __device__ inline void list_event(bool has, const Event &ev, int ch, WaveEvent *
 __device__ inline void list_flush(const WaveEvent *list, int pending, double *tile, int N, int C, bool guard,
                                   int lane) {
     if (lane < pending) splat_listed(list[lane], tile, N, C, guard, lane);
+}
+
+// ------------------------------------------------------------------------ shadow + splat
+// The shadow stage of a batch that takes no guiding records (its gather would only splat and
+// record the NEE outcomes), one-chirp frames in an LDS tile: as each packet's traversal ends,
+// its receivers' visible contributions become events (the gather's make_event of the same
+// slots) listed per wave and splatted 64 at a time into the workgroup's frame tile, which is
+// added into its partial frame at the end, and the NEE records are written. Every NEE slot of
+// such a batch is tested in exactly one packet (without PO diffraction), so no gather stage
+// follows, and the splat work overlaps the traversal instead of waiting for its end.
+// LDS: the traversal stack, then (dynamic) the frame tile and the waves' event lists.
+size_t shadow_splat_lds_bytes(int N, int C) {
+    return gather_lds_bytes(N, C) + kWaveEventBytes;
+}
+
+#ifndef HSR_SHADOW_SPLAT_WAVES
+#define HSR_SHADOW_SPLAT_WAVES 1
+#endif
+template <bool kCount>
+__global__ __launch_bounds__(kWfBlock, HSR_SHADOW_SPLAT_WAVES) void k_wf_shadow_splat(TraceArgs a, WfArgs w, int depth, double2 *partials,
+                                                             bool count) {
+    __shared__ int32_t lds_stack[kStackShort * kWfBlock];
+    extern __shared__ double tile[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const bool guard = a.N >= 2 * kGuard;
+    const int tileDoubles = 2 * (guard ? guard_row(a.N) : a.N) * a.C;
+    for (int j = tid; j < tileDoubles; j += kWfBlock) tile[j] = 0.0;
+    WaveEvent *list = (WaveEvent *)(tile + tileDoubles) + (tid >> 6) * kWaveEvents;
+    int pending = 0;
+    __syncthreads();
+    int32_t spill[kStackSpill];
+    Stack st;
+    st.spill = spill;
+    st.set_column(lds_stack + tid);
+    st.sp = 0;
+    WfCounts *cnt = &w.cnt[depth];
+    const unsigned int n = cnt->n_shd.v;
+    WfCounter *heads = cnt->head_shd;
+    const int64_t P = w.P;
+    uint32_t cn = 0, ct = 0, cr = 0, cev = 0;
+    Packet pk;
+    int64_t slot0 = 0;
+    int rays = 0, c0 = 0;
+    int64_t path = 0;
+    bool has = false;
+    Claim cl = claim_begin();
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        if (!cl.dry && __popcll(idle) >= kRefillMin) {
+            const unsigned int my = claim_items(heads, n, idle, lane, cl);
+            if (my != ~0u) {
+                const int32_t e = w.q_shd[my];
+                packet_begin(a, w, e, pk, slot0, rays);
+                path = e >> 2;
+                c0 = (e & 3) * kPacket;
+                st.sp = 0;
+                has = true;
+                if (kCount) cr += rays;
+            }
+        }
+        if (!__any(has)) {
+            if (cl.dry) break;
+            continue;
+        }
+        bool done = false;
+        if (has) {
+            packet_run<kCount>(a.nodesF, a.tris, pk, st, kTravSteps, cn, ct);
+            if (pk.node == kDone) {
+                done = true;
+                has = false;
+            }
+        }
+        // the finished packets' receivers, one receiver per step (wave-uniform calls)
+        if (__any(done)) {
+            int64_t local = 0;
+            bool rec = false;
+            if (done && a.recNee && depth >= 1) {
+                local = w.base + w.si[I_LOCAL * P + path];
+                rec = local < a.recPaths;
+            }
+#pragma unroll
+            for (int r = 0; r < kPacket; ++r) {
+                bool ev_has = false;
+                Event ev{0, 0.0f, {0.0f, 0.0f}, 0};
+                if (done && r < rays) {
+                    const int64_t j = slot0 + r;
+                    const int stt = (int)w.s_st[j];
+                    const bool vis = (pk.alive >> r) & 1u;
+                    int8_t flag = vis ? 2 : 1;
+                    if (vis && (stt & S_SPLAT)) {
+                        flag = 3;
+                        const float2 v2 = w.s_v[j];
+                        ev = make_event(a, w.s_tn[j], cf{v2.x, v2.y}, a.sampleWeight, a.motion ? w.s_dv[j] : 0.0);
+                        ev_has = true;
+                        if (count) ++cev;
+                    }
+                    if (rec) a.recNee[(local * a.maxDepth + (depth - 1)) * a.C + a.chBase + c0 + r] = flag;
+                }
+                list_event(ev_has, ev, a.chBase + c0 + r, list, pending, tile, a.N, a.C, guard, lane);
+            }
+        }
+    }
+    list_flush(list, pending, tile, a.N, a.C, guard, lane);
+    if (kCount) {
+        wave_count(&a.counters[5], cn, lane);
+        wave_count(&a.counters[6], ct, lane);
+        wave_count(&a.counters[3], cr, lane);
+    }
+    if (count) wave_count(&a.counters[4], cev, lane);
+    __syncthreads();
+    if (guard)
+        tile_flush_g(tile, partials + (size_t)blockIdx.x * a.N * a.C, a.N, a.C, tid, kWfBlock);
+    else
+        tile_flush(tile, partials + (size_t)blockIdx.x * a.N * a.C, a.N, a.C, tid, kWfBlock);
 }
 
 // ------------------------------------------------------------------------ gather + splat
@@ -1837,6 +1959,14 @@ hipError_t prepare_wavefront(int N, int C, bool cube, bool noTile) {
             hipFuncSetAttribute((const void *)k_wf_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
         if (e != hipSuccess) return e;
     }
+    const size_t slds = cube ? 0 : shadow_splat_lds_bytes(N, C);
+    if (slds > 65536 && slds + kStackShort * kWfBlock * sizeof(int32_t) <= 163840) {
+        const void *fs[2] = {(const void *)k_wf_shadow_splat<false>, (const void *)k_wf_shadow_splat<true>};
+        for (const void *f : fs) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
+            if (e != hipSuccess) return e;
+        }
+    }
     const size_t tlds = tail_lds_bytes(N, C, noTile);
     if (tlds > 65536) {
         const void *fs[4] = {(const void *)k_wf_tail<false, false>, (const void *)k_wf_tail<false, true>,
@@ -1908,7 +2038,7 @@ static void step_params(const BatchStep &st, const BatchArgs &h, StepParams &sp)
 
 BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int gridWork, int gridGather,
                          int64_t partialDoubles, bool zeroStats, bool count, hipEvent_t *evClosest,
-                         hipEvent_t *evShadow, int tailDepth, bool decouple) {
+                         hipEvent_t *evShadow, int tailDepth, bool decouple, bool fuseSplat) {
     BatchPlan plan;
     auto &S = plan.steps;
     const int maxDepth = a.maxDepth;
@@ -2002,7 +2132,20 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
         if (evClosest) marker(BatchStep::kRecord, stream, evClosest[2 * plan.nClosest + 1]);
         ++plan.nClosest;
     };
+    // decoupled batches whose shadow stage splats (shadow_splat_eligible): one fused launch per
+    // vertex, no gather stage
+    const bool fused = decouple && fuseSplat && !a.cube && !a.noSplat && !a.poDiffraction;
+    const void *fShadowSplat = count ? (const void *)k_wf_shadow_splat<true> : (const void *)k_wf_shadow_splat<false>;
     auto shadow_gather = [&](int k) {
+        if (fused) {
+            if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
+            kernel(0, fShadowSplat, gridGather, kWfBlock, shadow_splat_lds_bytes(a.N, a.C), BatchStep::kGather, k & 1,
+                   k & 1, k, count ? 1 : 0);
+            named("hsr shadow + splat");
+            if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow + 1]);
+            ++plan.nShadow;
+            return;
+        }
         if (!a.poDiffraction) {
             if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
             kernel(0, fShadow, gShadow, kWfBlock, 0, BatchStep::kAWi, k & 1, k & 1, k);

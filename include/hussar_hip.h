@@ -382,6 +382,13 @@ hsr_status hsr_hip_last_stats(hsr_backend *backend, hsr_run_stats *out);
  *                        64 per triangle). A build that needs more leaves the BVH traversal to
  *                        that run's depth-0 rays, and the capacity grows for the next build. */
 #define HSR_OPT_PRIMARY_CAP 16
+/*  HSR_OPT_SHADOW_SPLAT  wavefront pipeline: 1 (default) = in a batch that takes no guiding
+ *                        records (a final pass, an unguided run) with a one-chirp frame tile, each
+ *                        vertex's shadow stage also splats its visible contributions (and writes
+ *                        the NEE records) as its packets finish, and no gather stage runs;
+ *                        0 = a gather stage after each shadow stage, as in training passes. The
+ *                        results are the same up to the order of the frame sums. */
+#define HSR_OPT_SHADOW_SPLAT 17
 hsr_status hsr_hip_set_option(hsr_backend *backend, int32_t option, int64_t value);
 
 /* Hit record of the last run (see HSR_OPT_RECORD_HITS): hit_ids int32[paths*maxDepth]

@@ -15,7 +15,7 @@ import pytest
 import hussar_amd as H
 from hussar_amd import scenes
 from hussar_amd._lib import (HsrError, OPT_COUNTERS, OPT_MARKERS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE,
-                             OPT_RECORD_HITS, OPT_TAIL)
+                             OPT_RECORD_HITS, OPT_SHADOW_SPLAT, OPT_TAIL)
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -360,6 +360,41 @@ def test_markers_leave_results_alone(dihedral):
     assert rel_l2(z1, z0) < 1e-12
     with pytest.raises(HsrError, match="INVALID"):
         b.set_option(OPT_MARKERS, 2)
+
+
+@pytest.mark.parametrize("scene", ["vehicle", "dihedral_direct"])
+def test_shadow_splat_equals_gather(scene):
+    """Batches without guiding records splat in their shadow stages (HSR_OPT_SHADOW_SPLAT, the
+    default) instead of a gather stage after each: the same hits and NEE records, and the same
+    frame up to the order of its sums; also with the direct paths traced (depth-0 NEE)."""
+    if scene == "vehicle":
+        mesh, _ = scenes.vehicle_mesh(seed=5, triangles=50_000)
+        sc = scenes.radar_array(scenes.dihedral_rf(), channels=4)
+        fc = H.FrameConfig(1, 256, 4)
+        kn = H.Knobs(maxDepth=4, doGuiding=False)
+    else:
+        # the direct path needs the receiver away from the transmitter (a path of length 0 has
+        # an infinite Green's function, as in the reference)
+        mesh = scenes.dihedral_mesh()
+        sc = scenes.dihedral_pose(20.0)
+        sc = H.Scene(rfConfig=sc.rfConfig, tx=sc.tx,
+                     rx=H.NFAntenna(sc.tx.position + np.float32([0, 0.2, 0]), sc.tx.rotation))
+        fc = H.FrameConfig(1, 256, 1)
+        kn = H.Knobs(maxDepth=3, doGuiding=False, onlyIndirect=False)
+    b = make_backend(mesh, fc, 1)
+    n = 1 << 18
+    out = []
+    for fused in (0, 1):
+        b.set_option(OPT_SHADOW_SPLAT, fused)
+        out.append(gpu_run(b, sc, kn, n, record=n))
+    (z0, tw0, r0), (z1, tw1, r1) = out
+    assert tw0 == tw1
+    np.testing.assert_array_equal(r1[0], r0[0])
+    np.testing.assert_array_equal(r1[1], r0[1])
+    assert (r1[1] == 3).sum() > 100  # visible and splatted contributions were recorded
+    assert rel_l2(z1, z0) < 1e-12
+    with pytest.raises(HsrError, match="INVALID"):
+        b.set_option(OPT_SHADOW_SPLAT, 2)
 
 
 @pytest.mark.parametrize("size_cm", [5, 15])
