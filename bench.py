@@ -75,7 +75,9 @@ CALIB_FRAME = (64, 4096, 16)
 CALIB_READ, CALIB_WRITE = 2 * 8 * (1 << 23), 8 * (1 << 23)
 PMC_PASSES = [("fetch", ["FETCH_SIZE"]), ("write", ["WRITE_SIZE"]),
               ("valu", ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "GRBM_GUI_ACTIVE"])]
-PMC_KERNELS = {"trav": "k_wf_trav<false>", "shadow": "k_wf_shadow<false>", "calib": "k_post_normalize"}
+# kernel name patterns per key (a dispatch counts when its name contains one of them)
+PMC_KERNELS = {"trav": ("k_wf_trav<false>",), "shadow": ("k_wf_shadow<false>", "k_wf_shadow_splat<false>"),
+               "calib": ("k_post_normalize",)}
 
 
 def counter_child(args):
@@ -122,10 +124,10 @@ def collect_counters(args):
               f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         if not rows:
             return None, f"pass {name}: rc {r.returncode}, no counter rows: {(r.stderr or '')[-300:]}"
-        for key, pat in PMC_KERNELS.items():
-            sel = [q for q in rows if pat in q.get("Kernel_Name", "")]
+        for key, pats in PMC_KERNELS.items():
+            sel = [q for q in rows if any(pat in q.get("Kernel_Name", "") for pat in pats)]
             if not sel:
-                return None, f"pass {name}: no dispatch of {pat}"
+                return None, f"pass {name}: no dispatch of {' or '.join(pats)}"
             ent = res.setdefault(key, {})
             ent["dispatches"] = len({q.get("Dispatch_Id") for q in sel})
             for c in ctrs:
@@ -154,7 +156,8 @@ def roofline(pipeline, acc, cnt, steps, pmc=None, pmc_error=None, clock_mhz=2400
         rays = cnt["rays_closest"]
         sh_bytes = (cnt["nodes_shadow"] * NODE_BYTES_ANY + cnt["tris_shadow"] * TRI_BYTES) / cnt["shadow_launches"]
         sh_ms = acc["shadow_ms"] / acc["shadow_launches"]
-        second = {"kernel": "k_wf_shadow<false> (packet any-hit traversal)", "kernel_ms": sh_ms,
+        second = {"kernel": "k_wf_shadow<false> and k_wf_shadow_splat<false> (packet any-hit traversal; the "
+                            "second also splats, in batches without guiding records)", "kernel_ms": sh_ms,
                   "algorithmic_bytes": sh_bytes, "achieved": sh_bytes / (sh_ms * 1e-3) / 1e9,
                   "rays_shadow": cnt["rays_shadow"], "nodes_visited": cnt["nodes_shadow"],
                   "tris_tested": cnt["tris_shadow"]}
