@@ -87,7 +87,7 @@ class HsrError(RuntimeError):
         self.status = status
 
 
-ABI_VERSION = 4  # HSR_ABI_VERSION of include/hussar_hip.h
+ABI_VERSION = 5  # HSR_ABI_VERSION of include/hussar_hip.h
 
 
 def lib():

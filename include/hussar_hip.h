@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define HSR_ABI_VERSION 4
+#define HSR_ABI_VERSION 5
 #define HSR_MAX_CHANNELS 16
 #define HSR_MAX_TX 4
 

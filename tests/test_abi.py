@@ -33,7 +33,7 @@ def test_library_exports_every_declared_function():
 
 def test_abi_version_and_error_string():
     L = _lib.lib()
-    assert L.hsr_abi_version() == _lib.ABI_VERSION == 4
+    assert L.hsr_abi_version() == _lib.ABI_VERSION == 5
     assert isinstance(L.hsr_last_error(), bytes)
 
 
