@@ -721,7 +721,15 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     const Marker passMark(b->optMarkers, b->optMarkers ? passName : "");
     if (knobs->correctPhase && !knobs->filteringSphere)  // the reference's Assert, path.h:318
         return fail(HSR_ERR_INVALID, "correctPhase needs filteringSphere");
-    if (!knobs->useGeometricalOptics) return fail(HSR_ERR_UNSUPPORTED, "only geometrical optics is supported");
+    // geometrical optics off: the reference Asserts it for correctPhase (path.h:317) and for the
+    // sphere filter (path.h:344), and its Halton sampler has dimensions 0-9 (halton.h:58-67,
+    // 1082-1084), which a path at depth 3 uses up (two per bounce from dimension 3)
+    if (!knobs->useGeometricalOptics && knobs->correctPhase)
+        return fail(HSR_ERR_INVALID, "correctPhase needs geometrical optics");
+    if (!knobs->useGeometricalOptics && knobs->filteringSphere)
+        return fail(HSR_ERR_INVALID, "the sphere filter needs geometrical optics (set filteringSphere = 0)");
+    if (!knobs->useGeometricalOptics && knobs->maxDepth > 3)
+        return fail(HSR_ERR_INVALID, "without geometrical optics maxDepth is at most 3 (Halton dimensions 0-9)");
     if (knobs->maxDepth < 0) return fail(HSR_ERR_INVALID, "negative maxDepth");
     if (b->optChain && b->optPipeline != 1)
         return fail(HSR_ERR_UNSUPPORTED, "the reference FP32 chain needs the wavefront pipeline");
@@ -769,6 +777,7 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
     a.primaryOrder = knobs->primaryOrder;
     a.produceDebugImage = knobs->produceDebugImage;
     a.correctPhase = knobs->correctPhase;
+    a.geometricalOptics = knobs->useGeometricalOptics != 0;
     a.filteringMin = knobs->filteringMin;
     a.filteringMax = knobs->filteringMax;
     a.filteringRadius = knobs->filteringRadius;

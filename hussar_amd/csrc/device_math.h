@@ -148,6 +148,22 @@ template <uint32_t B> __device__ inline float radical_inverse_b(uint64_t v) {
     return fminf((float)rev * invBaseN, 0x1.fffffep-1f);
 }
 
+// HaltonSampler::get1D of dimension dim >= 3 (halton.h:20-22, 56-67: bases 7 to 29 for
+// dimensions 3 to 9, the reference's last). Dimension 10 and up is the reference's Assert
+// (halton.h:1082-1084); the host refuses the knobs that would reach it, and 0 is returned.
+__device__ inline float halton_dim(uint64_t v, int dim) {
+    switch (dim) {
+        case 3: return radical_inverse_b<7>(v);
+        case 4: return radical_inverse_b<11>(v);
+        case 5: return radical_inverse_b<13>(v);
+        case 6: return radical_inverse_b<17>(v);
+        case 7: return radical_inverse_b<19>(v);
+        case 8: return radical_inverse_b<23>(v);
+        case 9: return radical_inverse_b<29>(v);
+        default: return 0.0f;
+    }
+}
+
 // ----------------------------------------------------------------- sin/cos
 // Cody–Waite reduction by pi/2 (fdlibm's two-part pio2_1 / pio2_1t, valid for |x| < 2^20)
 // and Taylor polynomials of degree 17/18 on |r| <= pi/4, in double without fused

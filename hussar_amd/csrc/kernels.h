@@ -46,7 +46,7 @@ struct TraceArgs {
     DevAntenna rx[kMaxChannels];
     // PathTracer knobs
     int32_t onlyIndirect, maxDepth, poDiffraction, doGuiding, doFiltering, filteringSphere, filterGuiding,
-        primaryOrder, produceDebugImage, correctPhase;
+        primaryOrder, produceDebugImage, correctPhase, geometricalOptics;
     float filteringMin, filteringMax, filteringRadius;
     int32_t phaseMode;  // IF phase: 0 FP64 (default), 1 FP32 as the reference, 2 double-float
     int32_t refChain;   // 1: trace with the reference's FP32 geometry chain (wavefront only)

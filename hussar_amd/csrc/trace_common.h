@@ -610,7 +610,7 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
         v = v * (float)(kPiD * dist);
         v = v / (float)(den * den);
         v = v / fourPi;
-    } else if (p.depth > 0) {
+    } else if (p.depth > 0 && a.geometricalOptics) {
         // GO density correction (path.h:330-341)
         v = v * (float)(p.r / p.cosTheta);
         v = v / fourPi;
@@ -710,7 +710,14 @@ __device__ inline NeeSetup nee_setup(const TraceArgs &a, const PathState &p, int
 // Hit update and specular reflection (path.h:413-442, integrator.h:164-168,
 // intersection.h:38): geometric normal (kernel.cu:86) face-forwarded (cpu.cpp:82-85), and
 // the segment length re-evaluated in double against the hit triangle's plane, both from the
-// float edges promoted to double. Returns false when the hit is grazing (path ends).
+// float edges promoted to double. Returns false when the hit is grazing (path ends), or, with
+// geometrical optics off, when the weighted field is zero (path.h:428-429).
+// Geometrical optics off (path.h:277-279, 420-427): the field is weighted by the emitter's
+// connect (1/(4πt) from the transmitter, emitter.h:188-193; (jk0 + 1/max(t, 1e-3))/(4πt) from
+// a surface, emitter.h:72-81) and by the hemisphere-to-surface density t²/cosθ, and the next
+// direction is SurfaceEmitter::sample (emitter.h:58-70): a uniform sphere sample of Halton
+// dimensions 3 + 2(k-1) and 4 + 2(k-1) at the new depth k, in get2D's order, with H = d x J,
+// J = 2 n x H_in, zero when d points below the surface.
 // kRef: the hit of the FP32 chain's ray at distance t32 (the traversal's), whose normal,
 // face-forwarding, grazing test and update are the reference's float arithmetic; the FP64
 // chain is refined on the same triangle.
@@ -748,11 +755,46 @@ __device__ inline bool apply_hit(const TraceArgs &a, PathState &p, int32_t slot,
     p.cosTheta = cosT;
     p.r = p.r + tg;
     p.time = p.time + tg / kC;
+    if (!a.geometricalOptics) {
+        const float tF = (float)tg;
+        const float fourPi = 4.0f * kPiF;
+        if (p.depth == 0) {
+            p.H = (1.0f / (fourPi * tF)) * p.H;
+        } else {
+            const float k0 = ((2.0f * kPiF) * p.freq) / 299792448.0f;
+            const cf green = cf{1.0f / fmaxf(tF, 1e-3f), k0} / (fourPi * tF);
+            p.H = p.H * green;
+        }
+        p.H = (float)(tg * tg / cosT) * p.H;
+        if (is_zero(p.H)) return false;
+    }
     p.n_in = n;
     p.H_in = p.H;
     p.d_in = p.d;
     p.o = p.o + tg * p.d;
     p.depth++;
+    if (!a.geometricalOptics) {
+        const uint64_t index = (uint64_t)(a.offset + p.local);
+        const int dimA = 3 + 2 * (p.depth - 1);
+        const float ua = halton_dim(index, dimA), ub = halton_dim(index, dimA + 1);
+        const float s0 = a.primaryOrder == 0 ? ub : ua, s1 = a.primaryOrder == 0 ? ua : ub;
+        p.d = uniform_sample_sphere((double)s0, (double)s1);
+        const cf3 J = 2.0f * cross(cast3<float>(n), p.H_in);
+        p.H = cross(cast3<float>(p.d), J);
+        bool below;
+        if (kRef) {
+            p.c32.r = p.c32.r + t32;
+            p.c32.n_in = n32;
+            p.c32.d_in = p.c32.d;
+            p.c32.o = p.c32.o + t32 * p.c32.d;
+            p.c32.d = uniform_sample_sphere_f(s0, s1);
+            below = dot(p.c32.d, n32) < 0.0f;
+        } else {
+            below = dot(p.d, n) < 0.0;
+        }
+        if (below) p.H = cf3{};
+        return true;
+    }
     const d3 wi = -p.d;
     p.d = (2.0 * dot(n, wi)) * n - wi;
     const f3 nF = cast3<float>(n);

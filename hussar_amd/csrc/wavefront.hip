@@ -866,6 +866,9 @@ __global__ __launch_bounds__(kWfBlock, HSR_HIT_WAVES) void k_wf_shade(TraceArgs 
                         p.c32.r = o.w;
                         p.c32.d = {d.x, d.y, d.z};
                     }
+                    // without geometrical optics the hit update weighs the field by the surface
+                    // Green's function, whose wave number needs the frequency
+                    if (!a.geometricalOptics) p.freq = w.sf[F_FREQ * w.P + i];
                     alive = apply_hit<kRef>(a, p, h.x, __int_as_float(h.y));
                 }
                 if (!alive && recording) load_record(w, i, p, depth == 0 && a.onlyIndirect);

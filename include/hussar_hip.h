@@ -93,8 +93,10 @@ typedef struct hsr_scene {
 } hsr_scene;
 
 /* PathTracer public knobs (path.h:153-169), same names and defaults (see
- * hsr_knobs_default). useGeometricalOptics=0 returns HSR_ERR_UNSUPPORTED (the reference marks
- * that branch "should always be true"); correctPhase=1 (path.h:316-329, 360-368) needs
+ * hsr_knobs_default). useGeometricalOptics=0 (path.h:277-279, 420-427: a uniform sphere sample
+ * per bounce) needs filteringSphere=0 and correctPhase=0 and at most maxDepth=3, as the
+ * reference asserts (path.h:317, 344; its Halton sampler's dimensions 0-9, halton.h:58-67),
+ * else HSR_ERR_INVALID; correctPhase=1 (path.h:316-329, 360-368) needs
  * filteringSphere=1, as the reference asserts (path.h:318), else HSR_ERR_INVALID; its
  * corrected values are real for real antenna patterns, so the splat skip rule
  * (integrator.h:121-124) drops them unless produceDebugImage is set, as in the reference.

@@ -51,7 +51,10 @@
 // filters) against closed forms of a plate vertex (tests/test_oracle_pins.py); the guide
 // step against hand-worked trees and the density it must build (∝ sqrt(stat/area) with the
 // second moment, ∝ stat/area without) and the refinement it must make (a leaf of share s > t
-// becomes a complete subtree of depth ceil(log2(s/t))).
+// becomes a complete subtree of depth ceil(log2(s/t))); geometrical optics off against an FP64
+// brute force of a two-plate corner written from path.h and emitter.h in numpy (plane hits in
+// closed form, contributions splatted with the pinned splat) and against the default branch at
+// one bounce, where both weight the same quantity (tests/test_oracle_go_off.py).
 // libguiding's source is absent, so that rule is the published SD-tree rule (Müller et al.
 // 2017) in binary form, not a reading of libguiding; Eigen's ulp-level order of mixed-type
 // products (Matrix33f·Vector3c, Vector3f.cross(Vector3c)) is parity unpinned (Eigen absent; the
@@ -215,6 +218,11 @@ static float radical_inverse(uint64_t v, int base) {
     return std::min(float(reversed) * invBaseN, oneMinusEps);
 }
 static const int kPrimes[10] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29};  // halton.h:56-67
+
+// HaltonSampler::get1D of dimension dim (halton.h:20-22, 56-67): the reference's switch has
+// dimensions 0-9; a higher one is its Assert (halton.h:1082-1084), which the library refuses
+// up front (useGeometricalOptics = 0 with maxDepth > 3), so it never comes here.
+static float halton_dim(uint64_t index, int dim) { return dim < 10 ? radical_inverse(index, kPrimes[dim]) : 0.0f; }
 
 // ---------------------------------------------------------------------------------------
 // sin/cos of the sphere warp, reproducible bit for bit on CPU and GPU: Cody–Waite reduction
@@ -860,9 +868,11 @@ inline void splat_event(P *frame, int M, int N, int C, int ch, P delta_t, Cx<P> 
 }
 
 // ---------------------------------------------------------------------------------------
-// PathTracer::sample (path.h:236-451) with useGeometricalOptics = true (the reference's
-// default; its other branch draws extra samples and is not restated); correctPhase as the
-// knobs say.
+// PathTracer::sample (path.h:236-451), both branches of useGeometricalOptics: specular
+// reflection (the reference's default), or with it off a uniform sphere sample per bounce
+// (SurfaceEmitter::sample, emitter.h:58-70, from Halton dimensions 3 + 2(k-1) and 4 + 2(k-1)
+// at depth k, path.h:277-279) weighted after each hit by the emitter's connect and the
+// hemisphere-to-surface density t²/cosθ (path.h:420-427); correctPhase as the knobs say.
 //
 // Four precision roles: T — BVH traversal and ray/triangle tests; Gd — the geometry chain
 // that fixes path lengths (sampled and reflected directions, hit points, NEE legs,
@@ -944,19 +954,22 @@ void filter_weight(const Knobs &knobs, Gd dphase, Cx<F> &v, Cx<F> &guidingWeight
     }
 }
 
-// The weight of an NEE contribution v from a bounce vertex (depth > 0, path.h:330-394): the GO
-// density correction v·r/cosθ/4π (path.h:330-341; false = skipped when cosθ < 1e-3), the phase
-// error and the filter, and the guiding weight update gw += v (path.h:392-394). o, d: the
-// vertex and the reflected direction; dn, rr: unit direction and distance to the receiver;
-// d_in, n_in: incoming direction and normal at the vertex; r: path length up to the vertex.
-// dphase_out (may be null) receives dphase.
+// The weight of an NEE contribution v from a bounce vertex (depth > 0, path.h:330-394): with
+// geometrical optics the density correction v·r/cosθ/4π (path.h:330-341; false = skipped when
+// cosθ < 1e-3), then the phase error and the filter, and the guiding weight update gw += v
+// (path.h:392-394). o, d: the vertex and the ray's direction (reflected, or sampled with
+// geometrical optics off); dn, rr: unit direction and distance to the receiver; d_in, n_in:
+// incoming direction and normal at the vertex; r: path length up to the vertex. dphase_out
+// (may be null) receives dphase.
 template <class Gd, class F>
 bool nee_weight(const Knobs &knobs, Gd r, Gd cosTheta, V3<Gd> o, V3<Gd> d, V3<Gd> dn, Gd rr, V3<Gd> d_in,
                 V3<Gd> n_in, Gd lambda, Cx<F> &v, Cx<F> &guidingWeight, Gd *dphase_out) {
     const F piF = K<F>::pi();
-    v = v * F(r / cosTheta);
-    v = v / (F(4) * piF);
-    if (cosTheta < Gd(1e-3)) return false;
+    if (knobs.useGeometricalOptics) {
+        v = v * F(r / cosTheta);
+        v = v / (F(4) * piF);
+        if (cosTheta < Gd(1e-3)) return false;
+    }
     const Gd dphase = phase_error<Gd>(knobs, r, o, d, dn, rr, d_in, n_in, lambda);
     if (dphase_out) *dphase_out = dphase;
     filter_weight<Gd, F>(knobs, dphase, v, guidingWeight);
@@ -1137,9 +1150,11 @@ template <class T, class Gd, class F, class P, bool kPromote, bool kDual = false
                         if constexpr (kDual) {
                             // the GO weight from the Gd chain, the grazing skip, phase error and
                             // filter from the FP32 chain (as the ref32 instance)
-                            v = v * F(r / cosTheta);
-                            v = v / (F(4) * piF);
-                            if (cos32 < 1e-3f) continue;
+                            if (knobs->useGeometricalOptics) {
+                                v = v * F(r / cosTheta);
+                                v = v / (F(4) * piF);
+                                if (cos32 < 1e-3f) continue;
+                            }
                             const float lambda32 = float(kSpeedOfLight) / float(freq);
                             const float dp = phase_error<float>(*knobs, r32, o32, d32, dn32, rr32, d32_in, n32_in,
                                                                 lambda32);
@@ -1189,6 +1204,20 @@ template <class T, class Gd, class F, class P, bool kPromote, bool kDual = false
             const Gd tg = refine ? plane_distance(id, o, d) : Gd(t);
             r = r + tg;
             time = time + P(tg) / P(kSpeedOfLight);  // addDistance, geometry.h:110
+            if (!knobs->useGeometricalOptics) {
+                // path.h:420-427: the emitter's connect (NFAntenna::connect 1/(4πt) from the
+                // transmitter, emitter.h:188-193; SurfaceEmitter::connect (jk0 + 1/max(t, 1e-3))
+                // / (4πt) from a surface, emitter.h:72-81), then the hemisphere-to-surface
+                // density t²/cosθ
+                const F tF = F(tg);
+                if (depth == 0) {
+                    H = (F(1) / ((F(4) * piF) * tF)) * H;
+                } else {
+                    const Cx<F> green = Cx<F>{F(1) / std::max(tF, F(1e-3)), k0} / ((F(4) * piF) * tF);
+                    H = H * green;
+                }
+                H = F(tg * tg / cosTheta) * H;
+            }
             if (is_zero(H)) break;
             n_in = n;
             H_in = H;
@@ -1203,6 +1232,25 @@ template <class T, class Gd, class F, class P, bool kPromote, bool kDual = false
                 o32 = o32 + t32 * d32;
                 const V3<float> wi32 = -d32;
                 d32 = (2.0f * dot(n32, wi32)) * n32 - wi32;
+            }
+            if (!knobs->useGeometricalOptics) {
+                // SurfaceEmitter::sample (emitter.h:58-62) with get2D (path.h:277-278): the
+                // direction from Halton dimensions 3 + 2(k-1), 4 + 2(k-1) in get2D's order, H
+                // from SurfaceEmitter::evaluate (emitter.h:64-70), J = 2 n x H_in, H = d x J (the
+                // 4π weight of the sphere pdf is overwritten by setH), zero below the surface
+                const int dimA = 3 + 2 * (depth - 1);
+                const float ua = halton_dim(index, dimA), ub = halton_dim(index, dimA + 1);
+                const float s0 = knobs->primaryOrder == 0 ? ub : ua, s1 = knobs->primaryOrder == 0 ? ua : ub;
+                d = uniform_sample_sphere<Gd>(Gd(s0), Gd(s1));
+                const CV3<F> J = F(2) * cross(vcast<F>(n_in), H_in);
+                H = cross(vcast<F>(d), J);
+                if constexpr (kDual) {
+                    d32 = uniform_sample_sphere<float>(s0, s1);
+                    if (dot(d32, n32_in) < 0.0f) H = CV3<F>{};
+                } else if (dot(d, n_in) < Gd(0)) {
+                    H = CV3<F>{};
+                }
+                continue;
             }
             // reflectRay (integrator.h:164-168) with Intersection::R (intersection.h:38)
             const V3<Gd> wi = -d;
@@ -1565,8 +1613,12 @@ int orc_render(void *mh, const SceneC *scene, const Knobs *knobs, const FrameCfg
                int64_t count, float sampleWeight, int isFinal, void *guide, int nthreads, double *frame_out,
                double *totalWeight, int32_t *hit_ids, int8_t *nee, int64_t record_paths, int64_t *counters) {
     if (!mh || !scene || !knobs || !fc || !frame_out) return 1;
-    if (!knobs->useGeometricalOptics) return 5;
     if (knobs->correctPhase && !knobs->filteringSphere) return 1;  // path.h:318
+    // geometrical optics off: the reference Asserts it for correctPhase (path.h:317) and the
+    // sphere filter (path.h:344), and its sampler has Halton dimensions 0-9 (halton.h:58-67),
+    // which depth 3 uses up
+    if (!knobs->useGeometricalOptics && (knobs->correctPhase || knobs->filteringSphere || knobs->maxDepth > 3))
+        return 1;
     if (fc->samplesPerChirp <= 0 || fc->channelCount <= 0) return 1;
     Counters c;
     render_for(prec)(*(Mesh *)mh, *scene, *knobs, *fc, offset, count, sampleWeight, isFinal != 0, (Guide *)guide,

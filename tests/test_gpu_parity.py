@@ -287,8 +287,11 @@ def test_empty_mesh_and_zero_depth(pipeline):
 def test_errors_and_interrupt(dihedral):
     mesh, b, fc = dihedral
     sc = scenes.dihedral_pose(0.0)
-    with pytest.raises(HsrError, match="UNSUPPORTED"):
-        b.run_raw(sc, H.Knobs(useGeometricalOptics=False, doGuiding=False), 0, 10)
+    # geometrical optics off with the sphere filter (path.h:344), correctPhase (path.h:317) or
+    # more bounces than the Halton dimensions (halton.h:58-67) allow
+    for kw in ({}, {"filteringSphere": False, "correctPhase": True}, {"filteringSphere": False, "maxDepth": 4}):
+        with pytest.raises(HsrError, match="INVALID"):
+            b.run_raw(sc, H.Knobs(**{"useGeometricalOptics": False, "doGuiding": False, "maxDepth": 3, **kw}), 0, 10)
     with pytest.raises(HsrError, match="INVALID"):  # path.h:318 asserts the sphere filter
         b.run_raw(sc, H.Knobs(correctPhase=True, filteringSphere=False, doGuiding=False), 0, 10)
     with pytest.raises(HsrError, match="INVALID"):
