@@ -74,7 +74,9 @@ SIMDS = 1024  # 256 CUs x 4 SIMDs
 CALIB_FRAME = (64, 4096, 16)
 CALIB_READ, CALIB_WRITE = 2 * 8 * (1 << 23), 8 * (1 << 23)
 PMC_PASSES = [("fetch", ["FETCH_SIZE"]), ("write", ["WRITE_SIZE"]),
-              ("valu", ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "GRBM_GUI_ACTIVE"])]
+              ("valu", ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "GRBM_GUI_ACTIVE"]),
+              ("texture", ["TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "GRBM_GUI_ACTIVE"])]
+CUS = 256
 # kernel name patterns per key (a dispatch counts when its name contains one of them)
 PMC_KERNELS = {"trav": ("k_wf_trav<false>",), "shadow": ("k_wf_shadow<false>", "k_wf_shadow_splat<false>"),
                "calib": ("k_post_normalize",)}
@@ -131,7 +133,9 @@ def collect_counters(args):
             ent = res.setdefault(key, {})
             ent["dispatches"] = len({q.get("Dispatch_Id") for q in sel})
             for c in ctrs:
-                ent[c] = sum(float(q["Counter_Value"]) for q in sel if q.get("Counter_Name") == c)
+                # each pass's own busy cycles under their own key (they come from another run)
+                k = c if c != "GRBM_GUI_ACTIVE" or name == "valu" else f"{c}:{name}"
+                ent[k] = sum(float(q["Counter_Value"]) for q in sel if q.get("Counter_Name") == c)
     return res, None
 
 
@@ -143,7 +147,9 @@ def roofline(pipeline, acc, cnt, steps, pmc=None, pmc_error=None, clock_mhz=2400
      * hbm: HBM bytes per launch measured by FETCH_SIZE and WRITE_SIZE, scaled by what they read
        for the calibration kernel of known size, per duration, against HBM;
      * valu: VALU wave instructions per launch (SQ_INSTS_VALU) per duration, against the SIMDs'
-       issue rate (SIMDS x clock / 4).
+       issue rate (SIMDS x clock / 4);
+     * texture: the busy shares of the CUs' texture address and data units (TA_TA_BUSY,
+       TD_TD_BUSY), which every vector memory load passes.
     `bound` names the largest measured fraction; the top-level achieved / peak / frac are its."""
     if pipeline == 1:
         # the closest-hit traversal launches' own visits: not the shadow traversal's nor those of
@@ -202,6 +208,11 @@ def roofline(pipeline, acc, cnt, steps, pmc=None, pmc_error=None, clock_mhz=2400
                        "instructions_per_visit": insts * 64.0 * lane / visits,
                        "note": "instructions per visit = active-lane VALU instructions per node visit or "
                                "triangle test"}
+        cyc = tr["GRBM_GUI_ACTIVE:texture"] / 8.0  # the kernel's busy cycles (the counter sums 8 XCDs)
+        out["texture"] = {"ta_busy": tr["TA_TA_BUSY_sum"] / CUS / cyc, "td_busy": tr["TD_TD_BUSY_sum"] / CUS / cyc,
+                          "note": "share of the kernel's cycles in which a CU's texture address (TA) and texture "
+                                  "data (TD) units are busy: every node and triangle load passes both, so a "
+                                  "high TD share beside a high VALU share means both bind"}
         out["shadow_kernel_counters"] = {
             "hbm_bytes_per_launch": (sh["FETCH_SIZE"] * f_read + sh["WRITE_SIZE"] * f_write) * 1024.0 /
             sh["dispatches"],

@@ -130,7 +130,7 @@ hsr_status hsr_hip_create(const float *vertices, int64_t vertexCount, const int3
     hsr_status s;
     if ((s = dalloc(&b->d_nodes, (size_t)bvh.nodeCount * 4)) != HSR_OK) return s;   // 4 float4 per node
     if ((s = dalloc(&b->d_nodesF, (size_t)bvh.nodeCount * 8)) != HSR_OK) return s;  // 8 float4 per node
-    if ((s = dalloc(&b->d_tris, (size_t)bvh.slotCount * 3)) != HSR_OK) return s;
+    if ((s = dalloc(&b->d_tris, bvh.tris.size() / 4)) != HSR_OK) return s;  // the slots and the sentinel
     HSR_HIP(hipMemcpy(b->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(float), hipMemcpyHostToDevice));
     HSR_HIP(hipMemcpy(b->d_nodesF, bvh.nodesF.data(), bvh.nodesF.size() * sizeof(float), hipMemcpyHostToDevice));
     HSR_HIP(hipMemcpy(b->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -331,7 +331,7 @@ void backend_release(hsr_backend *b) {
 }
 
 int64_t backend_mesh_bytes(const hsr_backend *b) {
-    return (int64_t)b->nodeCount * (64 + 128) + (int64_t)b->slotCount * 48 + (b->d_vel ? (int64_t)b->slotCount * 16 : 0);
+    return (int64_t)b->nodeCount * (64 + 128) + ((int64_t)b->slotCount + 1) * 48 + (b->d_vel ? (int64_t)b->slotCount * 16 : 0);
 }
 }  // namespace hsr
 

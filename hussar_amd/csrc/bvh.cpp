@@ -797,7 +797,7 @@ float dequant(float origin, uint32_t q, float scale) {
     return origin + prod;
 }
 
-bool quantize4(const std::vector<float> &n4, std::vector<float> &out) {
+bool quantize4(const std::vector<float> &n4, std::vector<float> &out, int32_t emptyRef) {
     const size_t count = n4.size() / 32;
     out.assign(16 * count, 0.0f);
     for (size_t nd = 0; nd < count; ++nd) {
@@ -805,8 +805,7 @@ bool quantize4(const std::vector<float> &n4, std::vector<float> &out) {
         int32_t refs[4];
         std::memcpy(refs, &q[24], 16);
         uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
-        float origin[3];
-        uint32_t biased[3];
+        float origin[3], scale[3];
         for (int k = 0; k < 3; ++k) {
             // rows: lo.x 0, hi.x 4, lo.y 8, hi.y 12, lo.z 16, hi.z 20
             const float *lo = &q[8 * k], *hi = &q[8 * k + 4];
@@ -816,6 +815,7 @@ bool quantize4(const std::vector<float> &n4, std::vector<float> &out) {
                     o = std::min(o, lo[i]);
                     top = std::max(top, hi[i]);
                 }
+            if (!(o <= top)) o = top = 0.0f;  // no child
             origin[k] = o;
             // smallest power-of-two step with which 255 steps reach the top
             int e = -120;
@@ -827,9 +827,9 @@ bool quantize4(const std::vector<float> &n4, std::vector<float> &out) {
                 if (dequant(o, 255, s) >= top) break;
             }
             const float s = std::ldexp(1.0f, e);
-            biased[k] = (uint32_t)(e + 127);
+            scale[k] = s;
             for (int i = 0; i < 4; ++i) {
-                uint32_t a = 0, b = 0;
+                uint32_t a = 255, b = 0;  // an unused slot: the inverted box
                 if (refs[i] != kNoChild) {
                     double fa = std::floor(((double)lo[i] - (double)o) / (double)s);
                     double fb = std::ceil(((double)hi[i] - (double)o) / (double)s);
@@ -847,10 +847,13 @@ bool quantize4(const std::vector<float> &n4, std::vector<float> &out) {
         d[0] = origin[0];
         d[1] = origin[1];
         d[2] = origin[2];
-        const uint32_t ex = biased[0] | (biased[1] << 8) | (biased[2] << 16);
-        std::memcpy(&d[3], &ex, 4);
-        const uint32_t w[8] = {qlo[0], qhi[0], qlo[1], qhi[1], qlo[2], qhi[2], 0u, 0u};
-        std::memcpy(&d[4], w, 32);
+        d[3] = scale[0];
+        const uint32_t w[6] = {qlo[0], qhi[0], qlo[1], qhi[1], qlo[2], qhi[2]};
+        std::memcpy(&d[4], w, 24);
+        d[10] = scale[1];
+        d[11] = scale[2];
+        for (int i = 0; i < 4; ++i)
+            if (refs[i] == kNoChild) refs[i] = emptyRef;
         std::memcpy(&d[12], refs, 16);
     }
     return true;
@@ -1009,12 +1012,18 @@ bool build_bvh(const float *V, int64_t nv, const int32_t *I, int64_t nt, FlatBVH
         }
     }
     collapse4(nodes, out.nodesF, out.depth4);
-    if (!quantize4(out.nodesF, out.nodes)) return false;
+    // the sentinel leaf: one slot after the last, a degenerate triangle (its slot must fit the
+    // 27 bits of a leaf's first slot)
+    if (slotCount >= (1 << 27) - 1) return false;
+    const int32_t sentinel = (int32_t)(0x80000000u | (uint32_t)slotCount);
+    if (!quantize4(out.nodesF, out.nodes, sentinel)) return false;
     out.nodeCount = (int32_t)(out.nodes.size() / 16);
     out.slotCount = slotCount;
     out.maxDepth = B.maxDepth.load();
     out.maxLeaf = B.maxLeaf.load();
-    out.tris.assign(12 * (size_t)slotCount, 0.0f);
+    out.tris.assign(12 * ((size_t)slotCount + 1), 0.0f);
+    const int32_t noId = -1;
+    std::memcpy(&out.tris[12 * (size_t)slotCount + 3], &noId, 4);
     for (int32_t s = 0; s < slotCount && nr > 0; ++s) {
         const int32_t t = slotTri.empty() ? refTri[B.order[s]] : slotTri[s];
         const float *a = V + 3 * (int64_t)I[3 * t + 0];

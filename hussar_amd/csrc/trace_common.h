@@ -80,12 +80,14 @@ struct Stack {
     }
 };
 
-// Four-wide BVH node (64 B = 4 float4) with 8-bit child boxes: origin.xyz + biased exponents
-// | qlo.x, qhi.x, qlo.y, qhi.y | qlo.z, qhi.z, -, - | child[4]; byte i of a q word belongs to
-// child i and decodes to origin + q * 2^(e-127) (product exact, sum rounded; the builder
-// chose q so that the decoded box contains the child's box). A child >= 0 is an inner node;
-// kNoChild an unused slot; any other negative value a leaf: bit 31 set, bits 27-30 count-1,
-// bits 0-26 first triangle slot.
+// Four-wide BVH node (64 B = 4 float4) with 8-bit child boxes: origin.xyz, scale.x | qlo.x,
+// qhi.x, qlo.y, qhi.y | qlo.z, qhi.z, scale.y, scale.z | child[4]; byte i of a q word belongs
+// to child i and decodes to origin + q * scale (scale a power of two: product exact, sum
+// rounded; the builder chose q so that the decoded box contains the child's box). A child >= 0
+// is an inner node; a negative value a leaf: bit 31 set, bits 27-30 count-1, bits 0-26 first
+// triangle slot. An unused child slot holds the inverted box (qlo 255, qhi 0) and the sentinel
+// leaf, a degenerate triangle that no ray hits, so the box test needs no check of the slot.
+// The float nodes (Node4, any-hit) mark unused slots with kNoChild.
 constexpr int32_t kNoChild = 0x7fffffff;
 
 __device__ inline void leaf_decode(int32_t c, int &first, int &count) {
@@ -183,10 +185,7 @@ __device__ inline QuadHit slab4q(const float4 *__restrict__ nodes, int32_t n, co
     qh.c[1] = ch.y;
     qh.c[2] = ch.z;
     qh.c[3] = ch.w;
-    const uint32_t ex = __float_as_uint(h.w);
-    const float ax = __uint_as_float((ex & 0xFFu) << 23) * r.invd.x;
-    const float ay = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.invd.y;
-    const float az = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.invd.z;
+    const float ax = h.w * r.invd.x, ay = pb.z * r.invd.y, az = pb.w * r.invd.z;
     const float bx = (h.x - r.o.x) * r.invd.x, by = (h.y - r.o.y) * r.invd.y, bz = (h.z - r.o.z) * r.invd.z;
     constexpr float kErr = 9.5367431640625e-07f;  // 2^-20
     const float ex_ = fabsf(bx) * kErr, ey_ = fabsf(by) * kErr, ez_ = fabsf(bz) * kErr;
@@ -208,7 +207,7 @@ __device__ inline QuadHit slab4q(const float4 *__restrict__ nodes, int32_t n, co
         const float fz = __builtin_fmaf((float)((fzw >> (8 * i)) & 0xFFu), az, bfz);
         const float t0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tnear));
         const float t1 = fminf(fminf(fx, fy), fz) * 1.0000004f;
-        const bool hit = t0 <= t1 && t0 <= tb && qh.c[i] != kNoChild;
+        const bool hit = t0 <= t1 && t0 <= tb;
         qh.k[i] = hit ? t0 : __builtin_inff();
         if (hit) qh.m |= 1u << i;
     }

@@ -40,6 +40,8 @@ float tri_test(const float *p, const Ray &r) {
 
 bool better(float t, int id, float tb, int best) { return t < INFINITY && (best < 0 || t < tb || (t == tb && id < best)); }
 
+long long sentinelVisits = 0;  // visits of the sentinel leaf of unused child slots
+
 int closest(const hsr::FlatBVH &B, const Ray &r, float &tb) {
     float inv[3];
     for (int k = 0; k < 3; ++k)
@@ -50,16 +52,12 @@ int closest(const hsr::FlatBVH &B, const Ray &r, float &tb) {
     while (true) {
         if (node >= 0) {
             const float *q = &B.nodes[16 * (size_t)node];
-            uint32_t ex;
-            std::memcpy(&ex, &q[3], 4);
+            const float scale[3] = {q[3], q[10], q[11]};
             float A[3], Bn[3], Bf[3];
             uint32_t nw[3], fw[3], w[6];
             std::memcpy(w, &q[4], 24);
             for (int k = 0; k < 3; ++k) {
-                uint32_t e = ((ex >> (8 * k)) & 0xFFu) << 23;
-                float s;
-                std::memcpy(&s, &e, 4);
-                A[k] = s * inv[k];
+                A[k] = scale[k] * inv[k];
                 const float b = (q[k] - r.o[k]) * inv[k];
                 const float err = std::fabs(b) * 9.5367431640625e-07f;
                 Bn[k] = b - err;
@@ -81,7 +79,7 @@ int closest(const hsr::FlatBVH &B, const Ray &r, float &tb) {
                 const float t0 = std::fmax(std::fmax(nn[0], nn[1]), std::fmax(nn[2], 5e-5f));
                 const float t1 = std::fmin(std::fmin(ff[0], ff[1]), ff[2]) * 1.0000004f;
                 te[i] = t0;
-                if (t0 <= t1 && t0 <= tb && ch[i] != 0x7fffffff) m |= 1u << i;
+                if (t0 <= t1 && t0 <= tb) m |= 1u << i;
             }
             if (m) {
                 std::vector<std::pair<float, int>> hit;
@@ -94,6 +92,7 @@ int closest(const hsr::FlatBVH &B, const Ray &r, float &tb) {
             }
         } else {
             const int first = node & 0x07FFFFFF, count = ((node >> 27) & 0xF) + 1;
+            if (first == B.slotCount) ++sentinelVisits;
             for (int i = first; i < first + count; ++i) {
                 const float *p = &B.tris[12 * (size_t)i];
                 const float t = tri_test(p, r);
@@ -186,7 +185,7 @@ int main(int argc, char **argv) {
         if (ii >= 0 && ii < nt) seen[(size_t)ii] = 1;
     }
     const long long missing = std::count(seen.begin(), seen.end(), (char)0);
-    std::printf("nodes %d slots %d depth4 %d rays %d hits %d mismatches %d missing %lld\n", B.nodeCount, B.slotCount,
-                B.depth4, NR, hits, bad, missing);
+    std::printf("nodes %d slots %d depth4 %d rays %d hits %d mismatches %d missing %lld sentinel %lld\n", B.nodeCount,
+                B.slotCount, B.depth4, NR, hits, bad, missing, sentinelVisits);
     return 0;
 }
