@@ -473,6 +473,7 @@ def urban_bench(args):
     pt.configureFrame(fc)
     b = H.HipBackend(mesh, pt)
     b.set_velocities(vel)
+    b.set_option(OPT_SHADOW_SPLAT, args.shadow_splat)
     if args.tail >= 0:
         b.set_option(OPT_TAIL, args.tail)
     run_step(b, pt, sc, sched)  # warm-up, and the trained tree of the accuracy sample

@@ -968,11 +968,12 @@ static hsr_status run_impl(hsr_backend *b, const hsr_scene *scene, const hsr_kno
         // a batch without guiding records runs its gather stages beside the path chain (the
         // decoupled plan), and its fused tail adds its frame tiles into partial rows of its own
         const bool decouple = !training;
-        // its shadow stages splat (no gather stage) when their frame tile, event lists and
+        // its shadow stages splat (no gather stage): a cube frame's events go to its event
+        // lists, a one-chirp frame's into a frame tile when the tile, the event lists and the
         // traversal stack fit a workgroup's LDS (HSR_OPT_SHADOW_SPLAT)
-        const bool fuseSplat = b->optShadowSplat && !cube &&
-                               hsr::shadow_splat_lds_bytes(N, C) + hsr::kStackShort * 256 * sizeof(int32_t) <=
-                                   (size_t)b->ldsPerBlock;
+        const bool fuseSplat = b->optShadowSplat &&
+                               (cube || hsr::shadow_splat_lds_bytes(N, C) + hsr::kStackShort * 256 * sizeof(int32_t) <=
+                                            (size_t)b->ldsPerBlock);
         const int tiles = gridGather * (decouple && tailDepth >= 1 ? 2 : 1);
         const size_t needP = (size_t)tiles * MD;
         if (!cube && needP > b->partialCap) {

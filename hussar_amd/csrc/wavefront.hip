@@ -1007,19 +1007,24 @@ size_t shadow_splat_lds_bytes(int N, int C) {
 #ifndef HSR_SHADOW_SPLAT_WAVES
 #define HSR_SHADOW_SPLAT_WAVES 1
 #endif
-template <bool kCount>
-__global__ __launch_bounds__(kWfBlock, HSR_SHADOW_SPLAT_WAVES) void k_wf_shadow_splat(TraceArgs a, WfArgs w, int depth, double2 *partials,
+// kCube: the instance for cube frames (their events go to the event lists): no frame tile and
+// no wave lists, held to 128 registers for 4 resident waves per SIMD (a few spill), which
+// measured 555 ms per urban frame against 578 at its natural 143 registers (3 waves)
+template <bool kCount, bool kCube>
+__global__ __launch_bounds__(kWfBlock, kCube ? 4 : HSR_SHADOW_SPLAT_WAVES) void k_wf_shadow_splat(TraceArgs a, WfArgs w, int depth, double2 *partials,
                                                              bool count) {
     __shared__ int32_t lds_stack[kStackShort * kWfBlock];
     extern __shared__ double tile[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    constexpr bool cube = kCube;  // events go to the cube's event lists, no tile
     const bool guard = a.N >= 2 * kGuard;
-    const int tileDoubles = 2 * (guard ? guard_row(a.N) : a.N) * a.C;
+    const int tileDoubles = cube ? 0 : 2 * (guard ? guard_row(a.N) : a.N) * a.C;
     for (int j = tid; j < tileDoubles; j += kWfBlock) tile[j] = 0.0;
     WaveEvent *list = (WaveEvent *)(tile + tileDoubles) + (tid >> 6) * kWaveEvents;
     int pending = 0;
     __syncthreads();
+    const int nrx = min(a.rxCount, a.C);
     int32_t spill[kStackSpill];
     Stack st;
     st.spill = spill;
@@ -1088,9 +1093,24 @@ __global__ __launch_bounds__(kWfBlock, HSR_SHADOW_SPLAT_WAVES) void k_wf_shadow_
                     }
                     if (rec) a.recNee[(local * a.maxDepth + (depth - 1)) * a.C + a.chBase + c0 + r] = flag;
                 }
-                list_event(ev_has, ev, a.chBase + c0 + r, list, pending, tile, a.N, a.C, guard, lane);
+                if (cube) {
+                    // cube_event appends a wave's events of one channel at a time
+                    for (int p0 = 0; p0 < nrx; p0 += kPacket)
+                        cube_event(ev_has && c0 == p0, ev, a.chBase + p0 + r, a, lane);
+                } else {
+                    list_event(ev_has, ev, a.chBase + c0 + r, list, pending, tile, a.N, a.C, guard, lane);
+                }
             }
         }
+    }
+    if (cube) {
+        if (kCount) {
+            wave_count(&a.counters[5], cn, lane);
+            wave_count(&a.counters[6], ct, lane);
+            wave_count(&a.counters[3], cr, lane);
+        }
+        if (count) wave_count(&a.counters[4], cev, lane);
+        return;
     }
     list_flush(list, pending, tile, a.N, a.C, guard, lane);
     if (kCount) {
@@ -1964,7 +1984,7 @@ hipError_t prepare_wavefront(int N, int C, bool cube, bool noTile) {
     }
     const size_t slds = cube ? 0 : shadow_splat_lds_bytes(N, C);
     if (slds > 65536 && slds + kStackShort * kWfBlock * sizeof(int32_t) <= 163840) {
-        const void *fs[2] = {(const void *)k_wf_shadow_splat<false>, (const void *)k_wf_shadow_splat<true>};
+        const void *fs[2] = {(const void *)k_wf_shadow_splat<false, false>, (const void *)k_wf_shadow_splat<true, false>};
         for (const void *f : fs) {
             const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
             if (e != hipSuccess) return e;
@@ -2137,13 +2157,17 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
     };
     // decoupled batches whose shadow stage splats (shadow_splat_eligible): one fused launch per
     // vertex, no gather stage
-    const bool fused = decouple && fuseSplat && !a.cube && !a.noSplat && !a.poDiffraction;
-    const void *fShadowSplat = count ? (const void *)k_wf_shadow_splat<true> : (const void *)k_wf_shadow_splat<false>;
+    const bool fused = decouple && fuseSplat && !a.noSplat && !a.poDiffraction;
+    const void *fShadowSplat = a.cube ? (count ? (const void *)k_wf_shadow_splat<true, true>
+                                               : (const void *)k_wf_shadow_splat<false, true>)
+                                      : (count ? (const void *)k_wf_shadow_splat<true, false>
+                                               : (const void *)k_wf_shadow_splat<false, false>);
     auto shadow_gather = [&](int k) {
         if (fused) {
             if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
-            kernel(0, fShadowSplat, gridGather, kWfBlock, shadow_splat_lds_bytes(a.N, a.C), BatchStep::kGather, k & 1,
-                   k & 1, k, count ? 1 : 0);
+            // cube frames: no frame tile, the shadow stage's persistent grid
+            kernel(0, fShadowSplat, a.cube ? gShadow : gridGather, kWfBlock,
+                   a.cube ? 0 : shadow_splat_lds_bytes(a.N, a.C), BatchStep::kGather, k & 1, k & 1, k, count ? 1 : 0);
             named("hsr shadow + splat");
             if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow + 1]);
             ++plan.nShadow;

@@ -14,8 +14,8 @@ import pytest
 
 import hussar_amd as H
 from hussar_amd import scenes
-from hussar_amd._lib import (HsrError, OPT_COUNTERS, OPT_MARKERS, OPT_PATHS_PER_LAUNCH, OPT_PIPELINE,
-                             OPT_RECORD_HITS, OPT_SHADOW_SPLAT, OPT_TAIL)
+from hussar_amd._lib import (HsrError, OPT_COUNTERS, OPT_CUBE_EVENTS, OPT_MARKERS, OPT_PATHS_PER_LAUNCH,
+                             OPT_PIPELINE, OPT_RECORD_HITS, OPT_SHADOW_SPLAT, OPT_TAIL)
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -365,12 +365,23 @@ def test_markers_leave_results_alone(dihedral):
         b.set_option(OPT_MARKERS, 2)
 
 
-@pytest.mark.parametrize("scene", ["vehicle", "dihedral_direct"])
+@pytest.mark.parametrize("scene", ["vehicle", "dihedral_direct", "cube", "cube_overflow"])
 def test_shadow_splat_equals_gather(scene):
     """Batches without guiding records splat in their shadow stages (HSR_OPT_SHADOW_SPLAT, the
     default) instead of a gather stage after each: the same hits and NEE records, and the same
-    frame up to the order of its sums; also with the direct paths traced (depth-0 NEE)."""
-    if scene == "vehicle":
+    frame up to the order of its sums; also with the direct paths traced (depth-0 NEE), and for
+    a moving 16-chirp cube, whose events go to the event lists (and, past a list's capacity, to
+    the cube with device atomics)."""
+    vel = None
+    if scene.startswith("cube"):
+        mesh, _ = scenes.vehicle_mesh(seed=5, triangles=20_000, range_lo=8.0, range_hi=12.0)
+        u = scenes.uniforms(11, 3 * mesh.triangle_count()).reshape(-1, 3)
+        vel = ((u - 0.5) * 40.0).astype(np.float32)
+        sc = scenes.radar_array(scenes.dihedral_rf(), channels=4)
+        sc.velocity = (0.0, 0.0, 8.0)
+        fc = H.FrameConfig(16, 64, 4)
+        kn = H.Knobs(maxDepth=3, doGuiding=False)
+    elif scene == "vehicle":
         mesh, _ = scenes.vehicle_mesh(seed=5, triangles=50_000)
         sc = scenes.radar_array(scenes.dihedral_rf(), channels=4)
         fc = H.FrameConfig(1, 256, 4)
@@ -385,6 +396,10 @@ def test_shadow_splat_equals_gather(scene):
         fc = H.FrameConfig(1, 256, 1)
         kn = H.Knobs(maxDepth=3, doGuiding=False, onlyIndirect=False)
     b = make_backend(mesh, fc, 1)
+    if vel is not None:
+        b.set_velocities(vel)
+    if scene == "cube_overflow":
+        b.set_option(OPT_CUBE_EVENTS, 4096)
     n = 1 << 18
     out = []
     for fused in (0, 1):
