@@ -1004,8 +1004,11 @@ size_t shadow_splat_lds_bytes(int N, int C) {
     return gather_lds_bytes(N, C) + kWaveEventBytes;
 }
 
+// resident waves per SIMD of the one-chirp instance: 3 (168 registers, a few spills; its LDS
+// allows 3 workgroups per CU at 256 bins x 4 channels) measured 14.5 ms per vehicle frame
+// against 14.8 at its natural 213 registers and 2 waves
 #ifndef HSR_SHADOW_SPLAT_WAVES
-#define HSR_SHADOW_SPLAT_WAVES 1
+#define HSR_SHADOW_SPLAT_WAVES 3
 #endif
 // kCube: the instance for cube frames (their events go to the event lists): no frame tile and
 // no wave lists, held to 128 registers for 4 resident waves per SIMD (a few spill), which
