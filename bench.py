@@ -78,7 +78,7 @@ PMC_PASSES = [("fetch", ["FETCH_SIZE"]), ("write", ["WRITE_SIZE"]),
               ("texture", ["TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "GRBM_GUI_ACTIVE"])]
 CUS = 256
 # kernel name patterns per key (a dispatch counts when its name contains one of them)
-PMC_KERNELS = {"trav": ("k_wf_trav<false>",), "shadow": ("k_wf_shadow<false>", "k_wf_shadow_splat<false>"),
+PMC_KERNELS = {"trav": ("k_wf_trav<false>",), "shadow": ("k_wf_shadow<false>", "k_wf_shadow_splat<false,"),
                "calib": ("k_post_normalize",)}
 
 
@@ -162,7 +162,7 @@ def roofline(pipeline, acc, cnt, steps, pmc=None, pmc_error=None, clock_mhz=2400
         rays = cnt["rays_closest"]
         sh_bytes = (cnt["nodes_shadow"] * NODE_BYTES_ANY + cnt["tris_shadow"] * TRI_BYTES) / cnt["shadow_launches"]
         sh_ms = acc["shadow_ms"] / acc["shadow_launches"]
-        second = {"kernel": "k_wf_shadow<false> and k_wf_shadow_splat<false> (packet any-hit traversal; the "
+        second = {"kernel": "k_wf_shadow<false> and k_wf_shadow_splat<false, *> (packet any-hit traversal; the "
                             "second also splats, in batches without guiding records)", "kernel_ms": sh_ms,
                   "algorithmic_bytes": sh_bytes, "achieved": sh_bytes / (sh_ms * 1e-3) / 1e9,
                   "rays_shadow": cnt["rays_shadow"], "nodes_visited": cnt["nodes_shadow"],
