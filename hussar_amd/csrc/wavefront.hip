@@ -1003,18 +1003,20 @@ __device__ inline void list_flush(const WaveEvent *list, int pending, double *ti
 size_t shadow_splat_lds_bytes(int N, int C) {
     return gather_lds_bytes(N, C) + kWaveEventBytes;
 }
+// the one-chirp instance's waves per SIMD: 3 when three workgroups fit a CU's 160 KiB of LDS
+static int shadow_splat_waves(int N, int C) {
+    return 3 * (shadow_splat_lds_bytes(N, C) + kStackShort * kWfBlock * sizeof(int32_t)) <= 163840 ? 3 : 2;
+}
 
-// resident waves per SIMD of the one-chirp instance: 3 (168 registers, a few spills; its LDS
-// allows 3 workgroups per CU at 256 bins x 4 channels) measured 14.5 ms per vehicle frame
-// against 14.8 at its natural 213 registers and 2 waves
-#ifndef HSR_SHADOW_SPLAT_WAVES
-#define HSR_SHADOW_SPLAT_WAVES 3
-#endif
+// kWaves: resident waves per SIMD the instance is held to. One-chirp frames: 3 (168
+// registers, a few spills) when the frame tile lets 3 workgroups share a CU's LDS (256 bins x
+// 4 channels: 14.5 ms per vehicle frame against 14.8 at the natural 213 registers and 2
+// waves), else 2, which larger tiles cannot exceed anyway (shadow_splat_waves).
 // kCube: the instance for cube frames (their events go to the event lists): no frame tile and
 // no wave lists, held to 128 registers for 4 resident waves per SIMD (a few spill), which
 // measured 555 ms per urban frame against 578 at its natural 143 registers (3 waves)
-template <bool kCount, bool kCube>
-__global__ __launch_bounds__(kWfBlock, kCube ? 4 : HSR_SHADOW_SPLAT_WAVES) void k_wf_shadow_splat(TraceArgs a, WfArgs w, int depth, double2 *partials,
+template <bool kCount, bool kCube, int kWaves>
+__global__ __launch_bounds__(kWfBlock, kWaves) void k_wf_shadow_splat(TraceArgs a, WfArgs w, int depth, double2 *partials,
                                                              bool count) {
     __shared__ int32_t lds_stack[kStackShort * kWfBlock];
     extern __shared__ double tile[];
@@ -1987,7 +1989,8 @@ hipError_t prepare_wavefront(int N, int C, bool cube, bool noTile) {
     }
     const size_t slds = cube ? 0 : shadow_splat_lds_bytes(N, C);
     if (slds > 65536 && slds + kStackShort * kWfBlock * sizeof(int32_t) <= 163840) {
-        const void *fs[2] = {(const void *)k_wf_shadow_splat<false, false>, (const void *)k_wf_shadow_splat<true, false>};
+        const void *fs[4] = {(const void *)k_wf_shadow_splat<false, false, 3>, (const void *)k_wf_shadow_splat<true, false, 3>,
+                             (const void *)k_wf_shadow_splat<false, false, 2>, (const void *)k_wf_shadow_splat<true, false, 2>};
         for (const void *f : fs) {
             const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
             if (e != hipSuccess) return e;
@@ -2161,10 +2164,11 @@ BatchPlan plan_wavefront(const TraceArgs &a, const WfArgs &w, int gridTrav, int 
     // decoupled batches whose shadow stage splats (shadow_splat_eligible): one fused launch per
     // vertex, no gather stage
     const bool fused = decouple && fuseSplat && !a.noSplat && !a.poDiffraction;
-    const void *fShadowSplat = a.cube ? (count ? (const void *)k_wf_shadow_splat<true, true>
-                                               : (const void *)k_wf_shadow_splat<false, true>)
-                                      : (count ? (const void *)k_wf_shadow_splat<true, false>
-                                               : (const void *)k_wf_shadow_splat<false, false>);
+    const bool w3 = shadow_splat_waves(a.N, a.C) == 3;
+    const void *fShadowSplat =
+        a.cube ? (count ? (const void *)k_wf_shadow_splat<true, true, 4> : (const void *)k_wf_shadow_splat<false, true, 4>)
+        : w3   ? (count ? (const void *)k_wf_shadow_splat<true, false, 3> : (const void *)k_wf_shadow_splat<false, false, 3>)
+               : (count ? (const void *)k_wf_shadow_splat<true, false, 2> : (const void *)k_wf_shadow_splat<false, false, 2>);
     auto shadow_gather = [&](int k) {
         if (fused) {
             if (evShadow) marker(BatchStep::kRecord, 0, evShadow[2 * plan.nShadow]);
